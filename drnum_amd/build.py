@@ -1,0 +1,60 @@
+"""Build the in-tree native libraries.
+
+    python -m drnum_amd.build          # libdrnhip.so (hipcc, gfx950) + libdrnum.so (g++)
+
+Every HIP source is compiled for gfx950 only.  Packed FP32 instructions are
+disabled (-packed-fp32-ops): hipcc would otherwise emit v_pk_* forms that are
+not trusted on the target machines.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "lib")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_hip():
+    os.makedirs(LIB, exist_ok=True)
+    src = os.path.join(HERE, "csrc", "drnhip.hip")
+    out = os.path.join(LIB, "libdrnhip.so")
+    tmp = out + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + ["-o", tmp, src])
+    os.replace(tmp, out)
+    return out
+
+
+def build_host():
+    src_dir = os.path.join(HERE, "host")
+    srcs = sorted(os.path.join(src_dir, f) for f in os.listdir(src_dir) if f.endswith(".cpp"))
+    if not srcs:
+        return None
+    out = os.path.join(LIB, "libdrnum.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fopenmp",
+          "-I" + os.path.join(ROOT, "include"), "-o", tmp] + srcs +
+         ["-L" + LIB, "-ldrnhip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all():
+    build_hip()
+    build_host()
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

@@ -1,0 +1,353 @@
+"""Synthetic patch-grid workloads (the benchmark and test cases of this path).
+
+A workload is a plain description: a list of patch dicts in the reference's
+grid-file vocabulary (patchgrid.cpp:479-561, cartesianpatch.cpp:98-133:
+origin, NI NJ NK, six seek-layer counts, lengths) plus the face functor of
+each patch face (the xWallM/xWallP/... surface), the time step, the RK
+coefficients and the initial state.  Nothing here computes on a device.
+
+Face functor codes match ``include/drnhip.h`` (DRN_BC_*).
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import Callable, List, Optional, Sequence
+
+import numpy as np
+
+from . import gas
+
+BC_SLIP, BC_FARFIELD, BC_TRANSMISSIVE, BC_JET, BC_NONE = 0, 1, 2, 3, 4
+RECON_UPWIND1, RECON_UPWIND2 = 0, 1
+LIM_VANALBADA, LIM_MINMOD, LIM_SECONDORDER = 0, 1, 2
+NPAR = 16
+
+
+@dataclasses.dataclass
+class Patch:
+    ni: int
+    nj: int
+    nk: int
+    seek: List[int]            # imin, imax, jmin, jmax, kmin, kmax
+    origin: List[float]        # lower corner of local cell (0,0,0)
+    length: List[float]        # Lx, Ly, Lz
+    bc: List[int]              # face functor: xm, xp, ym, yp, zm, zp
+    bc_par: np.ndarray         # (6, NPAR) float64
+    gidx: List[int]            # global index of local cell (0,0,0) (for lattice grids)
+
+    @property
+    def cells(self) -> int:
+        return self.ni * self.nj * self.nk
+
+    @property
+    def core_cells(self) -> int:
+        s = self.seek
+        return (self.ni - s[0] - s[1]) * (self.nj - s[2] - s[3]) * (self.nk - s[4] - s[5])
+
+    def dx(self, axis: int) -> float:
+        return self.length[axis] / [self.ni, self.nj, self.nk][axis]
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    patches: List[Patch]
+    recon: int
+    limiter: int
+    periodic: List[float]      # period per axis, 0 = none
+    alpha: List[float]
+    dt: float
+    init: Callable[[np.ndarray, np.ndarray, np.ndarray], np.ndarray]  # (x,y,z) -> (5, n) primitive p,T,u,v,w
+    spheres: Optional[np.ndarray] = None   # (n, 4) xc, yc, zc, r for the level-set case
+    ls_inner_layers: int = 3
+    ls_min_inner_rel: float = 0.1
+    steps: int = 1
+
+    @property
+    def total_cells(self) -> int:
+        return sum(p.cells for p in self.patches)
+
+    @property
+    def core_cells(self) -> int:
+        return sum(p.core_cells for p in self.patches)
+
+    def cell_centres(self, p: int):
+        """Global cell-centre coordinates (float64) of patch p, flattened i,j,k with k fastest."""
+        P = self.patches[p]
+        xs = [P.origin[a] + (np.arange(n) + 0.5) * P.dx(a) for a, n in enumerate((P.ni, P.nj, P.nk))]
+        X, Y, Z = np.meshgrid(xs[0], xs[1], xs[2], indexing="ij")
+        return X.ravel(), Y.ravel(), Z.ravel()
+
+    def initial_state(self, p: int, dtype) -> np.ndarray:
+        """Conserved variables (5, N) of patch p in `dtype`, converted with the
+        reference's primitiveToConservative (perfectgas.h:56-64)."""
+        x, y, z = self.cell_centres(p)
+        prim = self.init(x, y, z)
+        return gas.prim_to_cons(prim[0], prim[1], prim[2], prim[3], prim[4], dtype)
+
+
+def _par() -> np.ndarray:
+    return np.zeros((6, NPAR), dtype=np.float64)
+
+
+def block_grid(ncells: Sequence[int], blocks: Sequence[int], length: Sequence[float], seek: int,
+               periodic: Sequence[bool], domain_bc: Sequence[int], interior_bc: int = BC_TRANSMISSIVE,
+               domain_par: Optional[np.ndarray] = None) -> List[Patch]:
+    """A box of ncells cells cut into blocks[0]*blocks[1]*blocks[2] patches of equal
+    core size.  Every face with a neighbour (or a periodic image) carries `seek`
+    seek layers; domain faces carry none and get `domain_bc`."""
+    core = [ncells[a] // blocks[a] for a in range(3)]
+    assert all(core[a] * blocks[a] == ncells[a] for a in range(3))
+    h = [length[a] / ncells[a] for a in range(3)]
+    patches = []
+    for bi in range(blocks[0]):
+        for bj in range(blocks[1]):
+            for bk in range(blocks[2]):
+                b = (bi, bj, bk)
+                sk, n, org, L, bc, gidx = [], [], [], [], [], []
+                par = _par()
+                for a in range(3):
+                    lo = seek if (b[a] > 0 or periodic[a]) else 0
+                    hi = seek if (b[a] < blocks[a] - 1 or periodic[a]) else 0
+                    sk += [lo, hi]
+                    n.append(core[a] + lo + hi)
+                    g0 = b[a] * core[a] - lo
+                    gidx.append(g0)
+                    org.append(g0 * h[a])
+                    L.append(n[-1] * h[a])
+                    for side, s in ((0, lo), (1, hi)):
+                        f = 2 * a + side
+                        if s > 0:
+                            bc.append(interior_bc)
+                        else:
+                            bc.append(domain_bc[f])
+                            if domain_par is not None:
+                                par[f] = domain_par[f]
+                patches.append(Patch(n[0], n[1], n[2], sk, org, L, bc, par, gidx))
+    return patches
+
+
+# ----------------------------------------------------------------------------
+# Config 1: Sod shock tube, one 64^3 patch
+# ----------------------------------------------------------------------------
+
+def sod(n: int = 64, steps: int = 100) -> Workload:
+    """Sod along x on one n^3 patch, FP64 reference config.  y/z faces use the
+    slip functor: for this y,z-uniform state with v=w=0 it gives the same face
+    fluxes as periodicity.  x faces are transmissive."""
+    bc = [BC_TRANSMISSIVE, BC_TRANSMISSIVE, BC_SLIP, BC_SLIP, BC_SLIP, BC_SLIP]
+    patches = [Patch(n, n, n, [0] * 6, [0.0, 0.0, 0.0], [1.0, 1.0, 1.0], bc, _par(), [0, 0, 0])]
+
+    def init(x, y, z):
+        left = x < 0.5
+        rho = np.where(left, 1.0, 0.125)
+        p = np.where(left, 1.0, 0.1)
+        T = p / (rho * gas.R)
+        zero = np.zeros_like(x)
+        return np.stack([p, T, zero, zero, zero])
+
+    a_max = np.sqrt(1.4 * 1.0 / 1.0)
+    dt = 0.5 * (1.0 / n) / a_max
+    return Workload("sod%d" % n, patches, RECON_UPWIND2, LIM_VANALBADA, [0.0, 0.0, 0.0],
+                    [0.25, 0.5, 1.0], dt, init, steps=steps)
+
+
+# ----------------------------------------------------------------------------
+# Config 2: subsonic jet, 4x4x4 patches of 64^3 core cells
+# ----------------------------------------------------------------------------
+
+P_AMB = 1.0 / 1.4
+T_AMB = P_AMB / (1.0 * gas.R)   # rho = 1, so a = sqrt(1.4 R T) = 1
+
+
+def jet(core: int = 64, blocks: int = 4, steps: int = 1000, mach: float = 0.9, radius_cells: float = 16.0,
+        noise: float = 0.01, seed: int = 1234) -> Workload:
+    """Subsonic top-hat jet entering through the z=0 face of a (core*blocks)^3 box.
+    Ambient rho=1, p=1/1.4 (a=1) at rest; inflow |w| = mach inside radius_cells
+    of the face centre with seeded +/-noise velocity perturbation (fixed in
+    time per inflow face cell); all other domain faces transmissive; 2 seek
+    layers on interior patch faces."""
+    nc = core * blocks
+    h = 1.0 / nc
+    dom_bc = [BC_TRANSMISSIVE] * 6
+    dom_bc[4] = BC_JET
+    patches = block_grid([nc] * 3, [blocks] * 3, [1.0] * 3, 2, [False] * 3, dom_bc)
+    for P in patches:
+        if P.bc[4] == BC_JET:
+            par = P.bc_par[4]
+            par[0] = 0.5 - P.origin[0]            # jet centre, local x
+            par[1] = 0.5 - P.origin[1]            # jet centre, local y
+            par[2] = radius_cells * h
+            par[3] = mach * 1.0                    # a = 1
+            par[4] = T_AMB
+            par[5] = 0.0
+            par[6] = T_AMB
+            par[7] = noise
+            par[8] = float(seed)
+            par[9] = P.gidx[0]
+            par[10] = P.gidx[1]
+
+    def init(x, y, z):
+        one = np.ones_like(x)
+        zero = np.zeros_like(x)
+        return np.stack([P_AMB * one, T_AMB * one, zero, zero, zero])
+
+    dt = 0.5 * h / (mach + 1.0)
+    return Workload("jet%dx%d" % (blocks, core), patches, RECON_UPWIND2, LIM_VANALBADA, [0.0] * 3,
+                    [0.25, 0.5, 1.0], dt, init, steps=steps)
+
+
+# ----------------------------------------------------------------------------
+# Config 5: periodic Kelvin-Helmholtz box
+# ----------------------------------------------------------------------------
+
+def kelvin_helmholtz(core: int = 64, blocks: int = 8, steps: int = 500, seed: int = 99) -> Workload:
+    """Periodic shear layer: u=+0.5 for 0.25<y<0.75 (rho=2) else u=-0.5 (rho=1),
+    p=1/1.4, v = 0.01 sin(4 pi x) + 0.01 U[-1,1] (seeded per global cell)."""
+    nc = core * blocks
+    h = 1.0 / nc
+    patches = block_grid([nc] * 3, [blocks] * 3, [1.0] * 3, 2, [True] * 3, [BC_SLIP] * 6)
+
+    def init(x, y, z):
+        inside = (y > 0.25) & (y < 0.75)
+        rho = np.where(inside, 2.0, 1.0)
+        u = np.where(inside, 0.5, -0.5)
+        # per-cell noise keyed on the global cell index (reproducible for any patching)
+        gi = np.floor(x / h).astype(np.int64) % nc
+        gj = np.floor(y / h).astype(np.int64) % nc
+        gk = np.floor(z / h).astype(np.int64) % nc
+        key = (gi * nc + gj) * nc + gk
+        r = _hash_uniform(key, seed)
+        v = 0.01 * np.sin(4.0 * np.pi * x) + 0.01 * r
+        p = np.full_like(x, P_AMB)
+        T = p / (rho * gas.R)
+        return np.stack([p, T, u, v, np.zeros_like(x)])
+
+    dt = 0.5 * h / (0.5 + np.sqrt(1.4 * P_AMB / 1.0))
+    return Workload("kh%dx%d" % (blocks, core), patches, RECON_UPWIND2, LIM_VANALBADA, [1.0] * 3,
+                    [0.25, 0.5, 1.0], dt, init, steps=steps)
+
+
+def _hash_uniform(key: np.ndarray, seed: int) -> np.ndarray:
+    """splitmix64-based uniform in [-1, 1) per integer key (vectorised)."""
+    with np.errstate(over="ignore"):
+        x = key.astype(np.uint64) ^ np.uint64(seed * 0x9E3779B97F4A7C15 % (1 << 64))
+        x = x + np.uint64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        x = x ^ (x >> np.uint64(31))
+    q = (x >> np.uint64(40)).astype(np.int64) - 8388608
+    return q.astype(np.float64) / 8388608.0
+
+
+# ----------------------------------------------------------------------------
+# Config 4: level-set spheres in a 192x128x128 box
+# ----------------------------------------------------------------------------
+
+def levelset_spheres(core: int = 64, blocks=(3, 2, 2), n_spheres: int = 8, steps: int = 2000, seed: int = 7,
+                     mach: float = 0.3, r_range=(6.0, 14.0)) -> Workload:
+    """Freestream Mach 0.3 along x past spheres defined by the signed distance
+    min_s(|x-c_s|-r_s); farfield functor on every domain face; slip level-set
+    fill (3 inner layers, min relative distance 0.1) after every stage."""
+    nc = [core * b for b in blocks]
+    h = 1.0 / core
+    L = [n * h for n in nc]
+    far = _par()
+    for f in range(6):
+        far[f, :5] = [P_AMB, T_AMB, mach, 0.0, 0.0]
+    patches = block_grid(nc, blocks, L, 2, [False] * 3, [BC_FARFIELD] * 6, domain_par=far)
+    rng = np.random.default_rng(seed)
+    spheres = []
+    tries = 0
+    while len(spheres) < n_spheres and tries < 100000:
+        tries += 1
+        r = rng.uniform(*r_range) * h
+        c = [rng.uniform(r + 4 * h, L[a] - r - 4 * h) for a in range(3)]
+        if all(np.linalg.norm(np.subtract(c, s[:3])) > r + s[3] + 2 * h for s in spheres):
+            spheres.append([c[0], c[1], c[2], r])
+
+    def init(x, y, z):
+        one = np.ones_like(x)
+        zero = np.zeros_like(x)
+        return np.stack([P_AMB * one, T_AMB * one, mach * one, zero, zero])
+
+    dt = 0.5 * h / (mach + 1.0)
+    return Workload("ls%dspheres" % n_spheres, patches, RECON_UPWIND2, LIM_VANALBADA, [0.0] * 3,
+                    [0.25, 0.5, 1.0], dt, init, spheres=np.asarray(spheres), steps=steps)
+
+
+# ----------------------------------------------------------------------------
+# Config 3: uneven patch batch with ratio-2 nesting
+# ----------------------------------------------------------------------------
+
+def uneven_batch(n_patches: int = 200, size_range=(16, 96), seed: int = 2024, refine_frac: float = 0.2,
+                 steps: int = 100, mach: float = 0.5) -> Workload:
+    """Patches of uneven size (per-axis U[size_range]) placed in lattice rows with
+    2-4 cell overlaps; a fraction are replaced by ratio-2 refined patches nested
+    inside their coarse parent's core.  Uniform Mach 0.5 flow along x plus a
+    Gaussian density pulse (amplitude 0.1, width 8 coarse cells).  Farfield
+    functors on all faces (a patch face that is also covered by a neighbour is
+    overwritten by the exchange)."""
+    rng = np.random.default_rng(seed)
+    h = 1.0 / 256.0
+    far = _par()
+    for f in range(6):
+        far[f, :5] = [P_AMB, T_AMB, mach, 0.0, 0.0]
+    # coarse lattice: rows of patches along x, stacked in y and z
+    nb = int(round((n_patches * (1.0 - refine_frac)) ** (1.0 / 3.0))) + 1
+    sizes = rng.integers(size_range[0], size_range[1] + 1, size=(nb, nb, nb, 3))
+    overl = rng.integers(2, 5, size=(nb, nb, nb, 3))
+    patches: List[Patch] = []
+    # cumulative placement per axis uses the size of the first patch of each slab
+    starts = []
+    for a in range(3):
+        s = [0]
+        for b in range(1, nb):
+            s.append(s[-1] + int(sizes[b - 1, 0, 0, a] if a == 0 else sizes[0, b - 1, 0, a] if a == 1
+                                 else sizes[0, 0, b - 1, a]) - int(overl[b, 0, 0, a]))
+        starts.append(s)
+    coarse = []
+    for bi in range(nb):
+        for bj in range(nb):
+            for bk in range(nb):
+                if len(coarse) >= int(n_patches * (1.0 - refine_frac)):
+                    break
+                b = (bi, bj, bk)
+                n = [int(sizes[0, 0, 0, 0]) if False else 0] * 3
+                n = [int(sizes[bi, 0, 0, 0]), int(sizes[0, bj, 0, 1]), int(sizes[0, 0, bk, 2])]
+                g0 = [starts[0][bi], starts[1][bj], starts[2][bk]]
+                seek = []
+                for a in range(3):
+                    seek += [2 if b[a] > 0 else 0, 2 if b[a] < nb - 1 else 0]
+                org = [g0[a] * h for a in range(3)]
+                L = [n[a] * h for a in range(3)]
+                coarse.append(Patch(n[0], n[1], n[2], seek, org, L, [BC_FARFIELD] * 6, far.copy(), g0))
+    patches.extend(coarse)
+    n_fine = n_patches - len(coarse)
+    for f in range(n_fine):
+        parent = coarse[int(rng.integers(0, len(coarse)))]
+        # nested block of the parent's core, refined by 2, 2 seek layers all round
+        pc = [[parent.seek[2 * a], [parent.ni, parent.nj, parent.nk][a] - parent.seek[2 * a + 1]] for a in range(3)]
+        ext = [max(4, int(rng.integers(4, max(5, (pc[a][1] - pc[a][0]) // 2)))) for a in range(3)]
+        lo = [pc[a][0] + 2 + int(rng.integers(0, max(1, pc[a][1] - pc[a][0] - ext[a] - 3))) for a in range(3)]
+        n = [2 * ext[a] + 4 for a in range(3)]
+        hf = h / 2.0
+        org = [parent.origin[a] + lo[a] * h - 2 * hf for a in range(3)]
+        L = [n[a] * hf for a in range(3)]
+        patches.append(Patch(n[0], n[1], n[2], [2] * 6, org, L, [BC_FARFIELD] * 6, far.copy(),
+                             [0, 0, 0]))
+    ext_hi = [max(p.origin[a] + p.length[a] for p in patches) for a in range(3)]
+    centre = [0.5 * ext_hi[a] for a in range(3)]
+    width = 8 * h
+
+    def init(x, y, z):
+        r2 = (x - centre[0]) ** 2 + (y - centre[1]) ** 2 + (z - centre[2]) ** 2
+        rho = 1.0 + 0.1 * np.exp(-r2 / (2 * width * width))
+        p = np.full_like(x, P_AMB)
+        T = p / (rho * gas.R)
+        zero = np.zeros_like(x)
+        return np.stack([p, T, mach * np.ones_like(x), zero, zero])
+
+    dt = 0.5 * (h / 2.0) / (mach + 1.2)
+    return Workload("uneven%d" % n_patches, patches, RECON_UPWIND2, LIM_VANALBADA, [0.0] * 3,
+                    [0.25, 0.5, 1.0], dt, init, steps=steps)

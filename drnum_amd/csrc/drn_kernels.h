@@ -1,0 +1,429 @@
+// drn_kernels.h -- gfx950 kernels of the per-patch explicit update.
+//
+//   k_stage_cell   one thread per cell; residual from the six faces of the cell
+//                  (each interior face evaluated by both neighbours), fused with
+//                  the RK stage update.  Residual summation order per cell follows
+//                  CartesianIterator::compute (cartesianiterator.h:132-308).
+//   k_stage_march  2.5-D form: a workgroup owns a chunk of the (j,k) plane of a
+//                  patch and marches along i; each face flux is computed once
+//                  (x faces in registers, y/z faces through LDS).
+//   k_exchange_*   gather form of Patch::accessDonorDataDirect (patch.cpp:573-653).
+//   k_ls_*         CPU_LevelSetObjectBC<LSOBCCompressibleEulerOp> (cpu_levelsetobjectbc.h:58-90).
+//   k_sums_*       FP64 core sums (deterministic two-pass reduction).
+#pragma once
+
+#include "drn_physics.h"
+
+namespace drn {
+
+struct PatchDev {
+  int32_t ni, nj, nk, njk;
+  uint32_t off;   // first global cell of the patch in a state buffer
+  int32_t pad0;
+  double dx, dy, dz;
+  int32_t bc[6];
+  int32_t seek[6];
+  double par[6][16];
+};
+
+// work item of the cell-per-thread stage kernel: patch, plane i, first (j,k) cell
+struct TileCell {
+  int32_t patch, i, c2, pad;
+};
+
+// work item of the marching stage kernel: patch, i range, (j,k) chunk start
+struct TileMarch {
+  int32_t patch, i0, i1, c0;
+};
+
+template <typename T>
+__device__ __forceinline__ void zero5(T* f) {
+#pragma unroll
+  for (int v = 0; v < 5; ++v) f[v] = T(0);
+}
+
+// face between cells cm1 (minus side) and cp1 (plus side) along DIR; cm2 and cp2
+// are the next cells outwards, valid only when has_m2 / has_p2 (upwind2.h:38-50).
+template <typename T, int RECON, int LIM, bool FAST, int DIR>
+__device__ __forceinline__ void face_flux_g(const T* __restrict__ s, uint64_t vs, uint64_t cm2, uint64_t cm1,
+                                            uint64_t cp1, uint64_t cp2, bool has_m2, bool has_p2, T A, T* f) {
+  T vl[5], vr[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    vl[v] = s[v * vs + cm1];
+    vr[v] = s[v * vs + cp1];
+  }
+  if constexpr (RECON == RECON_UPWIND2) {
+    if (has_m2) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        T a = vl[v], b = vr[v], m = s[v * vs + cm2];
+        vl[v] = a + T(0.5) * limited<T, LIM, FAST>(a - m, b - a);
+      }
+    }
+    if (has_p2) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        T a = s[v * vs + cm1], b = s[v * vs + cp1], q = s[v * vs + cp2];
+        vr[v] = b + T(0.5) * limited<T, LIM, FAST>(b - q, a - b);
+      }
+    }
+  }
+  ausm_flux<T, FAST, DIR>(vl, vr, A, f);
+}
+
+template <typename T, int RECON, int LIM, bool FAST, int BS>
+__global__ __launch_bounds__(BS) void k_stage_cell(const PatchDev* __restrict__ pds, const TileCell* __restrict__ tiles,
+                                                   const T* __restrict__ src, const T* base, T* dst, uint64_t vs,
+                                                   T factor) {
+  const TileCell t = tiles[blockIdx.x];
+  const PatchDev& P = pds[t.patch];
+  const int ni = P.ni, nj = P.nj, nk = P.nk, njk = P.njk;
+  const int c2 = t.c2 + (int)threadIdx.x;
+  if (c2 >= njk) return;
+  const int i = t.i;
+  const int j = c2 / nk;
+  const int k = c2 - j * nk;
+  const uint64_t c = (uint64_t)P.off + (uint64_t)i * (uint64_t)njk + (uint64_t)c2;
+  const uint64_t sx = (uint64_t)njk, sy = (uint64_t)nk;
+  const T dx = T(P.dx), dy = T(P.dy), dz = T(P.dz);
+  const T Ax = dy * dz, Ay = dx * dz, Az = dx * dy;
+  T res[5], f[5];
+  zero5(res);
+  // minus faces (visited at this cell's own iteration), then plus faces in the
+  // order the later cells of the reference loop subtract them
+  if (i > 0 && ni > 2) {
+    zero5(f);
+    face_flux_g<T, RECON, LIM, FAST, 0>(src, vs, c - 2 * sx, c - sx, c, c + sx, i >= 2, i + 1 < ni, Ax, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] += f[v];
+  }
+  if (j > 0 && nj > 2) {
+    zero5(f);
+    face_flux_g<T, RECON, LIM, FAST, 1>(src, vs, c - 2 * sy, c - sy, c, c + sy, j >= 2, j + 1 < nj, Ay, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] += f[v];
+  }
+  if (k > 0 && nk > 2) {
+    zero5(f);
+    face_flux_g<T, RECON, LIM, FAST, 2>(src, vs, c - 2, c - 1, c, c + 1, k >= 2, k + 1 < nk, Az, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] += f[v];
+  }
+  if (k < nk - 1 && nk > 2) {
+    zero5(f);
+    face_flux_g<T, RECON, LIM, FAST, 2>(src, vs, c - 1, c, c + 1, c + 2, k >= 1, k + 2 < nk, Az, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] -= f[v];
+  }
+  if (j < nj - 1 && nj > 2) {
+    zero5(f);
+    face_flux_g<T, RECON, LIM, FAST, 1>(src, vs, c - sy, c, c + sy, c + 2 * sy, j >= 1, j + 2 < nj, Ay, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] -= f[v];
+  }
+  if (i < ni - 1 && ni > 2) {
+    zero5(f);
+    face_flux_g<T, RECON, LIM, FAST, 0>(src, vs, c - sx, c, c + sx, c + 2 * sx, i >= 1, i + 2 < ni, Ax, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] -= f[v];
+  }
+  // patch-face functors (cartesianiterator.h:190-308)
+  const bool wx = ni > 2 && (i == 0 || i == ni - 1);
+  const bool wy = nj > 2 && (j == 0 || j == nj - 1);
+  const bool wz = nk > 2 && (k == 0 || k == nk - 1);
+  if (wx || wy || wz) {
+    T var[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) var[v] = src[v * vs + c];
+    const T xc = T(0.5) * dx + T(i) * dx, yc = T(0.5) * dy + T(j) * dy, zc = T(0.5) * dz + T(k) * dz;
+    if (ni > 2) {
+      if (i == 0) {
+        zero5(f);
+        wall_flux<T>(P.bc[0], P.par[0], var, 0, false, yc, zc, j, k, Ax, f);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) res[v] += f[v];
+      }
+      if (i == ni - 1) {
+        zero5(f);
+        wall_flux<T>(P.bc[1], P.par[1], var, 0, true, yc, zc, j, k, Ax, f);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) res[v] -= f[v];
+      }
+    }
+    if (nj > 2) {
+      if (j == 0) {
+        zero5(f);
+        wall_flux<T>(P.bc[2], P.par[2], var, 1, false, xc, zc, i, k, Ay, f);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) res[v] += f[v];
+      }
+      if (j == nj - 1) {
+        zero5(f);
+        wall_flux<T>(P.bc[3], P.par[3], var, 1, true, xc, zc, i, k, Ay, f);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) res[v] -= f[v];
+      }
+    }
+    if (nk > 2) {
+      if (k == 0) {
+        zero5(f);
+        wall_flux<T>(P.bc[4], P.par[4], var, 2, false, xc, yc, i, j, Az, f);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) res[v] += f[v];
+      }
+      if (k == nk - 1) {
+        zero5(f);
+        wall_flux<T>(P.bc[5], P.par[5], var, 2, true, xc, yc, i, j, Az, f);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) res[v] -= f[v];
+      }
+    }
+  }
+  // cartesianiterator.h:310-322
+  const T pf = factor / (dx * dy * dz);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) dst[v * vs + c] = base[v * vs + c] + pf * res[v];
+}
+
+// ---------------------------------------------------------------------------
+// exchange (gather): receivers are seek cells, donors are donor-zone cells of
+// other patches, so the two sets never intersect and all receivers of all
+// patches can be filled in one launch without ordering.
+// ---------------------------------------------------------------------------
+
+// receivers served by exactly one donor patch with `stride` contributions
+template <typename T>
+__global__ __launch_bounds__(256) void k_exchange_single(uint32_t n, int stride, const uint32_t* __restrict__ rec,
+                                                         const uint32_t* __restrict__ idx,
+                                                         const T* __restrict__ w, T* U, uint64_t vs) {
+#pragma clang fp contract(off)
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  T acc[5];
+  zero5(acc);
+  for (int s = 0; s < stride; ++s) {
+    const uint64_t d = idx[(uint64_t)r * stride + s];
+    const T ww = w[(uint64_t)r * stride + s];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) acc[v] += U[v * vs + d] * ww;
+  }
+  const uint64_t g = rec[r];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) U[v * vs + g] = T(0) + acc[v];
+}
+
+// receivers served by several donor patches: donor sums added in donor order
+template <typename T>
+__global__ __launch_bounds__(256) void k_exchange_multi(uint32_t n, const uint32_t* __restrict__ rec,
+                                                        const uint32_t* __restrict__ gptr,
+                                                        const uint32_t* __restrict__ gstride,
+                                                        const uint32_t* __restrict__ gwi,
+                                                        const uint32_t* __restrict__ idx, const T* __restrict__ w,
+                                                        T* U, uint64_t vs) {
+#pragma clang fp contract(off)
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  T tot[5];
+  zero5(tot);
+  for (uint32_t g = gptr[r]; g < gptr[r + 1]; ++g) {
+    T acc[5];
+    zero5(acc);
+    const uint32_t st = gstride[g], w0 = gwi[g];
+    for (uint32_t s = 0; s < st; ++s) {
+      const uint64_t d = idx[w0 + s];
+      const T ww = w[w0 + s];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) acc[v] += U[v * vs + d] * ww;
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) tot[v] += acc[v];
+  }
+  const uint64_t g = rec[r];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) U[v * vs + g] = tot[v];
+}
+
+// ---------------------------------------------------------------------------
+// level-set fill: mirror gather into scratch, then the slip reflection
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_ls_access(uint32_t n, const uint32_t* __restrict__ donors,
+                                                   const T* __restrict__ w, const T* __restrict__ U, uint64_t vs,
+                                                   T* __restrict__ acc) {
+#pragma clang fp contract(off)
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  T a[5];
+  zero5(a);
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const uint64_t d = donors[(uint64_t)e * 8 + s];
+    const T ww = w[(uint64_t)e * 8 + s];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) a[v] += ww * U[v * vs + d];
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) acc[(uint64_t)v * n + e] = a[v];
+}
+
+// lsobccompressibleeulerop.h:83-144 with the reference's mixed precision
+// (the double literals promote the reflection and the blend to double)
+template <typename T>
+__global__ __launch_bounds__(256) void k_ls_operate(uint32_t n, const uint32_t* __restrict__ cells,
+                                                    const T* __restrict__ nrm, const T* __restrict__ acc, T* U,
+                                                    uint64_t vs, T relax) {
+#pragma clang fp contract(off)
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const uint64_t c = cells[e];
+  const T g[3] = {nrm[(uint64_t)e * 3 + 0], nrm[(uint64_t)e * 3 + 1], nrm[(uint64_t)e * 3 + 2]};
+  const T a1 = acc[1 * (uint64_t)n + e], a2 = acc[2 * (uint64_t)n + e], a3 = acc[3 * (uint64_t)n + e];
+  const T run = a1 * g[0] + a2 * g[1] + a3 * g[2];
+  const T eps = T(1.e-6);
+  const T av[3] = {a1, a2, a3};
+  T out[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const T hard = T(double(av[q]) - 2. * double(run) * double(g[q]));
+    const T cur = U[(1 + q) * vs + c];
+    const T diff = cur - hard;
+    T damp = dabs(diff) / (dabs(cur) + eps);
+    if (double(damp) > (1. - double(relax))) damp = T(1. - double(relax));
+    out[q] = T((1. - double(damp)) * double(hard) + double(damp * cur));
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) U[(1 + q) * vs + c] = out[q];
+  U[0 * vs + c] = acc[0 * (uint64_t)n + e];
+  U[4 * vs + c] = acc[4 * (uint64_t)n + e];
+}
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+  return x;
+}
+__device__ __forceinline__ double wave_min(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmin(x, __shfl_xor(x, o, 64));
+  return x;
+}
+
+// per-block partial sums of the core cells of one plane chunk (256 threads)
+template <typename T>
+__global__ __launch_bounds__(256) void k_sums_partial(const PatchDev* __restrict__ pds, const TileCell* __restrict__ tiles,
+                                                      const T* __restrict__ U, uint64_t vs, double* __restrict__ part) {
+  __shared__ double red[4][5];
+  const TileCell t = tiles[blockIdx.x];
+  const PatchDev& P = pds[t.patch];
+  const int c2 = t.c2 + (int)threadIdx.x;
+  double s[5] = {0, 0, 0, 0, 0};
+  if (c2 < P.njk) {
+    const int j = c2 / P.nk, k = c2 - j * P.nk, i = t.i;
+    const bool core = i >= P.seek[0] && i < P.ni - P.seek[1] && j >= P.seek[2] && j < P.nj - P.seek[3] &&
+                      k >= P.seek[4] && k < P.nk - P.seek[5];
+    if (core) {
+      const uint64_t c = (uint64_t)P.off + (uint64_t)i * P.njk + c2;
+      const double dV = P.dx * P.dy * P.dz;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) s[v] = double(U[v * vs + c]) * dV;
+    }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    double x = wave_sum(s[v]);
+    if (lane == 0) red[wv][v] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    const int v = threadIdx.x;
+    part[(uint64_t)blockIdx.x * 5 + v] = ((red[0][v] + red[1][v]) + red[2][v]) + red[3][v];
+  }
+}
+
+// fixed-order final sum of the partials (one block of 256)
+__global__ __launch_bounds__(256) void k_sums_final(uint32_t nblk, const double* __restrict__ part,
+                                                   double* __restrict__ out) {
+  __shared__ double red[4][5];
+  double s[5] = {0, 0, 0, 0, 0};
+  for (uint32_t b = threadIdx.x; b < nblk; b += 256) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) s[v] += part[(uint64_t)b * 5 + v];
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    double x = wave_sum(s[v]);
+    if (lane == 0) red[wv][v] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    const int v = threadIdx.x;
+    out[v] = ((red[0][v] + red[1][v]) + red[2][v]) + red[3][v];
+  }
+}
+
+// diagnostics partials: max CFL, min rho, max rho, count of non-finite values
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag_partial(const PatchDev* __restrict__ pds, const TileCell* __restrict__ tiles,
+                                                      const T* __restrict__ U, uint64_t vs, double dt,
+                                                      double* __restrict__ part) {
+  __shared__ double red[4][4];
+  const TileCell t = tiles[blockIdx.x];
+  const PatchDev& P = pds[t.patch];
+  const int c2 = t.c2 + (int)threadIdx.x;
+  double cfl = 0, rmin = 1e300, rmax = -1e300, bad = 0;
+  if (c2 < P.njk) {
+    const uint64_t c = (uint64_t)P.off + (uint64_t)t.i * P.njk + c2;
+    double var[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      var[v] = double(U[v * vs + c]);
+      if (!isfinite(var[v])) bad += 1;
+    }
+    double ir = 1.0 / var[0];
+    double u = var[1] * ir, v = var[2] * ir, w = var[3] * ir;
+    double Tt = (var[4] * ir - 0.5 * (u * u + v * v + w * w)) / 717.5;
+    double a = sqrt(1.4 * 287.2 * Tt);
+    cfl = fmax(fmax((fabs(u) + a) * dt / P.dx, (fabs(v) + a) * dt / P.dy), (fabs(w) + a) * dt / P.dz);
+    if (!isfinite(cfl)) cfl = 0;
+    rmin = var[0];
+    rmax = var[0];
+  }
+  cfl = wave_max(cfl);
+  rmin = wave_min(rmin);
+  rmax = wave_max(rmax);
+  bad = wave_sum(bad);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[wv][0] = cfl;
+    red[wv][1] = rmin;
+    red[wv][2] = rmax;
+    red[wv][3] = bad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double o[4] = {red[0][0], red[0][1], red[0][2], red[0][3]};
+    for (int q = 1; q < 4; ++q) {
+      o[0] = fmax(o[0], red[q][0]);
+      o[1] = fmin(o[1], red[q][1]);
+      o[2] = fmax(o[2], red[q][2]);
+      o[3] += red[q][3];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) part[(uint64_t)blockIdx.x * 4 + q] = o[q];
+  }
+}
+
+}  // namespace drn

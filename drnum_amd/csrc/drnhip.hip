@@ -1,0 +1,889 @@
+// drnhip.hip -- libdrnhip.so: context, device memory, work lists, launches and
+// the C ABI declared in include/drnhip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/drnhip.h"
+#include "drn_kernels.h"
+
+using namespace drn;
+
+namespace {
+
+constexpr int NBUF = 3;
+constexpr int ALIGN_CELLS = 64;
+
+struct ExchangeTables {
+  // reference form, per receiver patch (local cell indices)
+  std::vector<int32_t> donor_patch;
+  std::vector<int64_t> donor_info;  // 4 per donor: nrec, stride, rec_start, wi_start
+  std::vector<int64_t> rec_unique, rec_concat, donor_index;
+  std::vector<double> donor_weight;
+};
+
+struct LsTables {
+  std::vector<int64_t> cells, donors;
+  std::vector<double> normals, weights;
+};
+
+struct TimerSlot {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> used, pool;
+};
+
+}  // namespace
+
+struct drn_ctx {
+  int device = 0;
+  int prec = DRN_F64;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int recon = DRN_RECON_UPWIND2, lim = DRN_LIM_VANALBADA;
+
+  std::vector<drn_patch_desc> descs;
+  std::vector<uint64_t> offs;
+  uint64_t vstride = 0;  // cells per variable block (all patches, padded)
+  bool laid_out = false;
+  void* buf[NBUF] = {nullptr, nullptr, nullptr};
+  int map[NBUF] = {0, 1, 2};  // logical field -> physical buffer
+
+  PatchDev* d_pd = nullptr;
+  std::vector<TileCell> tiles;
+  std::vector<int> tile_first;  // per patch, size np+1
+  TileCell* d_tiles = nullptr;
+  double* d_part = nullptr;  // reduction partials (5 per tile)
+  double* d_out = nullptr;
+  int block = 256;
+  int tiling = 0;
+
+  // exchange
+  std::map<int, ExchangeTables> xt;
+  struct SingleClass {
+    int stride;
+    uint32_t n;
+    uint32_t *rec, *idx;
+    void* w;
+  };
+  std::vector<SingleClass> xsingle;
+  uint32_t xmulti_n = 0;
+  uint32_t *xm_rec = nullptr, *xm_gptr = nullptr, *xm_gstride = nullptr, *xm_gwi = nullptr, *xm_idx = nullptr;
+  void* xm_w = nullptr;
+  bool have_exchange = false;
+
+  // level set
+  std::map<int, LsTables> lt;
+  uint32_t ls_n = 0;
+  uint32_t *ls_cells = nullptr, *ls_donors = nullptr;
+  void *ls_w = nullptr, *ls_nrm = nullptr, *ls_acc = nullptr;
+
+  // timing
+  bool timing = false;
+  TimerSlot timers[3];
+
+  // graph
+  hipGraphExec_t gexec = nullptr;
+  double g_dt = 0;
+  std::vector<double> g_alpha;
+  int g_steps = 0;
+  bool first_step = true;
+
+  size_t esize() const { return prec == DRN_F32 ? 4 : 8; }
+};
+
+#define HIPCHK(ctx, expr)                                                            \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                \
+      return DRN_E_HIP;                                                              \
+    }                                                                                \
+  } while (0)
+
+static int fail(drn_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+template <typename T>
+static hipError_t dalloc(T** p, size_t n) {
+  return hipMalloc((void**)p, n * sizeof(T) + 16);
+}
+
+static void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+// ---------------------------------------------------------------------------
+// timing helpers
+// ---------------------------------------------------------------------------
+
+static void timer_begin(drn_ctx* ctx, int which, hipEvent_t* stop_out) {
+  *stop_out = nullptr;
+  if (!ctx->timing) return;
+  TimerSlot& s = ctx->timers[which];
+  std::pair<hipEvent_t, hipEvent_t> ev;
+  if (!s.pool.empty()) {
+    ev = s.pool.back();
+    s.pool.pop_back();
+  } else {
+    (void)hipEventCreate(&ev.first);
+    (void)hipEventCreate(&ev.second);
+  }
+  (void)hipEventRecord(ev.first, ctx->stream);
+  s.used.push_back(ev);
+  *stop_out = ev.second;
+}
+
+static void timer_end(drn_ctx* ctx, hipEvent_t stop) {
+  if (stop) (void)hipEventRecord(stop, ctx->stream);
+}
+
+// ---------------------------------------------------------------------------
+// launches
+// ---------------------------------------------------------------------------
+
+template <typename T, int RECON, int LIM>
+static void launch_stage_t(drn_ctx* ctx, const TileCell* tiles, int ntiles, const T* src, const T* base, T* dst,
+                           T factor) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, false, 256>), dim3(ntiles), dim3(256), 0, ctx->stream, ctx->d_pd,
+                     tiles, src, base, dst, ctx->vstride, factor);
+}
+
+template <typename T>
+static void launch_stage_dispatch(drn_ctx* ctx, const TileCell* tiles, int ntiles, const T* src, const T* base, T* dst,
+                                  T factor) {
+  if (ctx->recon == DRN_RECON_UPWIND1) {
+    launch_stage_t<T, RECON_UPWIND1, LIM_SECONDORDER>(ctx, tiles, ntiles, src, base, dst, factor);
+  } else if (ctx->lim == DRN_LIM_VANALBADA) {
+    launch_stage_t<T, RECON_UPWIND2, LIM_VANALBADA>(ctx, tiles, ntiles, src, base, dst, factor);
+  } else if (ctx->lim == DRN_LIM_MINMOD) {
+    launch_stage_t<T, RECON_UPWIND2, LIM_MINMOD>(ctx, tiles, ntiles, src, base, dst, factor);
+  } else {
+    launch_stage_t<T, RECON_UPWIND2, LIM_SECONDORDER>(ctx, tiles, ntiles, src, base, dst, factor);
+  }
+}
+
+static int do_stage(drn_ctx* ctx, int psrc, int pbase, int pdst, double factor, int launch) {
+  hipEvent_t stop;
+  timer_begin(ctx, 0, &stop);
+  if (launch == DRN_LAUNCH_BATCHED) {
+    if (ctx->prec == DRN_F32)
+      launch_stage_dispatch<float>(ctx, ctx->d_tiles, (int)ctx->tiles.size(), (const float*)ctx->buf[psrc],
+                                   (const float*)ctx->buf[pbase], (float*)ctx->buf[pdst], (float)factor);
+    else
+      launch_stage_dispatch<double>(ctx, ctx->d_tiles, (int)ctx->tiles.size(), (const double*)ctx->buf[psrc],
+                                    (const double*)ctx->buf[pbase], (double*)ctx->buf[pdst], factor);
+  } else {
+    const int np = (int)ctx->descs.size();
+    for (int p = 0; p < np; ++p) {
+      const int t0 = ctx->tile_first[p], nt = ctx->tile_first[p + 1] - t0;
+      if (ctx->prec == DRN_F32)
+        launch_stage_dispatch<float>(ctx, ctx->d_tiles + t0, nt, (const float*)ctx->buf[psrc],
+                                     (const float*)ctx->buf[pbase], (float*)ctx->buf[pdst], (float)factor);
+      else
+        launch_stage_dispatch<double>(ctx, ctx->d_tiles + t0, nt, (const double*)ctx->buf[psrc],
+                                      (const double*)ctx->buf[pbase], (double*)ctx->buf[pdst], factor);
+    }
+  }
+  timer_end(ctx, stop);
+  HIPCHK(ctx, hipGetLastError());
+  return DRN_OK;
+}
+
+template <typename T>
+static void launch_exchange_t(drn_ctx* ctx, T* U) {
+  for (const auto& c : ctx->xsingle) {
+    if (c.n == 0) continue;
+    hipLaunchKernelGGL(k_exchange_single<T>, dim3((c.n + 255) / 256), dim3(256), 0, ctx->stream, c.n, c.stride, c.rec,
+                       c.idx, (const T*)c.w, U, ctx->vstride);
+  }
+  if (ctx->xmulti_n > 0) {
+    hipLaunchKernelGGL(k_exchange_multi<T>, dim3((ctx->xmulti_n + 255) / 256), dim3(256), 0, ctx->stream,
+                       ctx->xmulti_n, ctx->xm_rec, ctx->xm_gptr, ctx->xm_gstride, ctx->xm_gwi, ctx->xm_idx,
+                       (const T*)ctx->xm_w, U, ctx->vstride);
+  }
+}
+
+static int do_exchange(drn_ctx* ctx, int pbuf) {
+  if (!ctx->have_exchange) return DRN_OK;
+  hipEvent_t stop;
+  timer_begin(ctx, 1, &stop);
+  if (ctx->prec == DRN_F32)
+    launch_exchange_t<float>(ctx, (float*)ctx->buf[pbuf]);
+  else
+    launch_exchange_t<double>(ctx, (double*)ctx->buf[pbuf]);
+  timer_end(ctx, stop);
+  HIPCHK(ctx, hipGetLastError());
+  return DRN_OK;
+}
+
+template <typename T>
+static void launch_ls_t(drn_ctx* ctx, T* U, T relax) {
+  const uint32_t n = ctx->ls_n;
+  hipLaunchKernelGGL(k_ls_access<T>, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, ctx->ls_donors,
+                     (const T*)ctx->ls_w, (const T*)U, ctx->vstride, (T*)ctx->ls_acc);
+  hipLaunchKernelGGL(k_ls_operate<T>, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, ctx->ls_cells,
+                     (const T*)ctx->ls_nrm, (const T*)ctx->ls_acc, U, ctx->vstride, relax);
+}
+
+static int do_ls_fill(drn_ctx* ctx, int pbuf, double relax) {
+  if (ctx->ls_n == 0) return DRN_OK;
+  hipEvent_t stop;
+  timer_begin(ctx, 2, &stop);
+  if (ctx->prec == DRN_F32)
+    launch_ls_t<float>(ctx, (float*)ctx->buf[pbuf], (float)relax);
+  else
+    launch_ls_t<double>(ctx, (double*)ctx->buf[pbuf], relax);
+  timer_end(ctx, stop);
+  HIPCHK(ctx, hipGetLastError());
+  return DRN_OK;
+}
+
+// RungeKutta::operator() (rungekutta.cpp:27-42) on physical buffers: the state
+// buffer X is exchanged (when needed), kept as U^n (base) for every stage, the
+// stages alternate between the two other buffers, and the last stage output
+// becomes logical field 0 while X becomes field 1 (the reference's copy of
+// field 0 into field 1 at the step start).
+static int do_rk_step(drn_ctx* ctx, double dt, int n_alpha, const double* alpha, int launch) {
+  const bool post = ctx->ls_n > 0;
+  const int X = ctx->map[0];
+  const int Y = ctx->map[1] == X ? ctx->map[2] : ctx->map[1];
+  int Z = 0;
+  while (Z == X || Z == Y) ++Z;
+  int rc;
+  if (ctx->first_step || post) {
+    if ((rc = do_exchange(ctx, X))) return rc;
+  }
+  ctx->first_step = false;
+  int cur = X;
+  for (int s = 0; s < n_alpha; ++s) {
+    if (s > 0 && post) {
+      if ((rc = do_exchange(ctx, cur))) return rc;
+    }
+    const int dst = (s % 2 == 0) ? Y : Z;
+    if ((rc = do_stage(ctx, cur, X, dst, alpha[s] * dt, launch))) return rc;
+    if ((rc = do_exchange(ctx, dst))) return rc;
+    if (post) {
+      if ((rc = do_ls_fill(ctx, dst, 0.5))) return rc;
+    }
+    cur = dst;
+  }
+  ctx->map[0] = cur;
+  ctx->map[1] = X;
+  ctx->map[2] = (cur == Y) ? Z : Y;
+  return DRN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+// The functions below were declared extern "C" by drnhip.h and keep C linkage.
+
+int drn_ctx_create(int device, int precision, drn_ctx** out) {
+  if (!out) return DRN_E_ARG;
+  *out = nullptr;
+  if (precision != DRN_F32 && precision != DRN_F64) return DRN_E_ARG;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return DRN_E_HIP;
+  drn_ctx* ctx = new drn_ctx();
+  ctx->device = device;
+  ctx->prec = precision;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return DRN_E_HIP;
+  }
+  *out = ctx;
+  return DRN_OK;
+}
+
+static void free_exchange(drn_ctx* ctx) {
+  for (auto& c : ctx->xsingle) {
+    dfree(c.rec);
+    dfree(c.idx);
+    dfree(c.w);
+  }
+  ctx->xsingle.clear();
+  dfree(ctx->xm_rec);
+  dfree(ctx->xm_gptr);
+  dfree(ctx->xm_gstride);
+  dfree(ctx->xm_gwi);
+  dfree(ctx->xm_idx);
+  dfree(ctx->xm_w);
+  ctx->xm_rec = ctx->xm_gptr = ctx->xm_gstride = ctx->xm_gwi = ctx->xm_idx = nullptr;
+  ctx->xm_w = nullptr;
+  ctx->xmulti_n = 0;
+  ctx->have_exchange = false;
+}
+
+static void free_ls(drn_ctx* ctx) {
+  dfree(ctx->ls_cells);
+  dfree(ctx->ls_donors);
+  dfree(ctx->ls_w);
+  dfree(ctx->ls_nrm);
+  dfree(ctx->ls_acc);
+  ctx->ls_cells = ctx->ls_donors = nullptr;
+  ctx->ls_w = ctx->ls_nrm = ctx->ls_acc = nullptr;
+  ctx->ls_n = 0;
+}
+
+void drn_ctx_destroy(drn_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  free_exchange(ctx);
+  free_ls(ctx);
+  for (int b = 0; b < NBUF; ++b) dfree(ctx->buf[b]);
+  dfree(ctx->d_pd);
+  dfree(ctx->d_tiles);
+  dfree(ctx->d_part);
+  dfree(ctx->d_out);
+  for (auto& s : ctx->timers) {
+    for (auto& e : s.used) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+    for (auto& e : s.pool) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+  }
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* drn_last_error(const drn_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int drn_precision(const drn_ctx* ctx) { return ctx ? ctx->prec : DRN_E_ARG; }
+
+int drn_set_operator(drn_ctx* ctx, int recon, int limiter) {
+  if (!ctx) return DRN_E_ARG;
+  if (recon != DRN_RECON_UPWIND1 && recon != DRN_RECON_UPWIND2) return fail(ctx, DRN_E_ARG, "unknown reconstruction");
+  if (limiter < DRN_LIM_VANALBADA || limiter > DRN_LIM_SECONDORDER) return fail(ctx, DRN_E_ARG, "unknown limiter");
+  ctx->recon = recon;
+  ctx->lim = limiter;
+  return DRN_OK;
+}
+
+int drn_patch_add(drn_ctx* ctx, const drn_patch_desc* d, int* id) {
+  if (!ctx || !d) return DRN_E_ARG;
+  if (ctx->laid_out) return fail(ctx, DRN_E_STATE, "patches must be added before drn_layout");
+  if (d->ni < 1 || d->nj < 1 || d->nk < 1) return fail(ctx, DRN_E_ARG, "patch sizes must be positive");
+  if ((uint64_t)d->ni * d->nj * d->nk > (1ull << 31)) return fail(ctx, DRN_E_ARG, "patch too large");
+  for (int f = 0; f < 6; ++f) {
+    if (d->bc[f] < DRN_BC_SLIP || d->bc[f] > DRN_BC_NONE) return fail(ctx, DRN_E_ARG, "unknown face functor");
+    if (d->seek[f] < 0) return fail(ctx, DRN_E_ARG, "negative seek layer count");
+  }
+  if (!(d->dx > 0 && d->dy > 0 && d->dz > 0)) return fail(ctx, DRN_E_ARG, "spacing must be positive");
+  ctx->descs.push_back(*d);
+  if (id) *id = (int)ctx->descs.size() - 1;
+  return DRN_OK;
+}
+
+int drn_num_patches(const drn_ctx* ctx) { return ctx ? (int)ctx->descs.size() : DRN_E_ARG; }
+
+size_t drn_patch_cells(const drn_ctx* ctx, int id) {
+  if (!ctx || id < 0 || id >= (int)ctx->descs.size()) return 0;
+  const auto& d = ctx->descs[id];
+  return (size_t)d.ni * d.nj * d.nk;
+}
+
+int drn_layout(drn_ctx* ctx) {
+  if (!ctx) return DRN_E_ARG;
+  if (ctx->laid_out) return fail(ctx, DRN_E_STATE, "layout already done");
+  if (ctx->descs.empty()) return fail(ctx, DRN_E_STATE, "no patches");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const int np = (int)ctx->descs.size();
+  std::vector<PatchDev> pd(np);
+  uint64_t off = 0;
+  ctx->offs.resize(np);
+  ctx->tile_first.assign(np + 1, 0);
+  ctx->tiles.clear();
+  for (int p = 0; p < np; ++p) {
+    const auto& d = ctx->descs[p];
+    PatchDev& P = pd[p];
+    memset(&P, 0, sizeof(P));
+    P.ni = d.ni;
+    P.nj = d.nj;
+    P.nk = d.nk;
+    P.njk = d.nj * d.nk;
+    P.off = (uint32_t)off;
+    P.dx = d.dx;
+    P.dy = d.dy;
+    P.dz = d.dz;
+    for (int f = 0; f < 6; ++f) {
+      P.bc[f] = d.bc[f];
+      P.seek[f] = d.seek[f];
+      for (int q = 0; q < 16; ++q) P.par[f][q] = d.bc_par[f][q];
+    }
+    ctx->offs[p] = off;
+    const uint64_t n = (uint64_t)d.ni * d.nj * d.nk;
+    off += (n + ALIGN_CELLS - 1) / ALIGN_CELLS * ALIGN_CELLS;
+    ctx->tile_first[p] = (int)ctx->tiles.size();
+    const int njk = d.nj * d.nk;
+    for (int i = 0; i < d.ni; ++i)
+      for (int c2 = 0; c2 < njk; c2 += 256) ctx->tiles.push_back(TileCell{p, i, c2, 0});
+  }
+  ctx->tile_first[np] = (int)ctx->tiles.size();
+  if (off >= (1ull << 32)) return fail(ctx, DRN_E_ARG, "grid exceeds 2^32 cells");
+  ctx->vstride = off;
+  for (int b = 0; b < NBUF; ++b) {
+    if (hipMalloc(&ctx->buf[b], 5 * off * ctx->esize()) != hipSuccess) return fail(ctx, DRN_E_NOMEM, "state buffer allocation failed");
+    HIPCHK(ctx, hipMemset(ctx->buf[b], 0, 5 * off * ctx->esize()));
+  }
+  HIPCHK(ctx, dalloc(&ctx->d_pd, np));
+  HIPCHK(ctx, hipMemcpy(ctx->d_pd, pd.data(), np * sizeof(PatchDev), hipMemcpyHostToDevice));
+  HIPCHK(ctx, dalloc(&ctx->d_tiles, ctx->tiles.size()));
+  HIPCHK(ctx, hipMemcpy(ctx->d_tiles, ctx->tiles.data(), ctx->tiles.size() * sizeof(TileCell), hipMemcpyHostToDevice));
+  HIPCHK(ctx, dalloc(&ctx->d_part, ctx->tiles.size() * 5));
+  HIPCHK(ctx, dalloc(&ctx->d_out, 8));
+  ctx->laid_out = true;
+  return DRN_OK;
+}
+
+static int check_field(drn_ctx* ctx, int field) {
+  if (!ctx->laid_out) return fail(ctx, DRN_E_STATE, "drn_layout not called");
+  if (field < 0 || field >= NBUF) return fail(ctx, DRN_E_ARG, "field must be 0, 1 or 2");
+  return DRN_OK;
+}
+
+int drn_patch_upload(drn_ctx* ctx, int id, int field, const void* host) {
+  if (!ctx || !host) return DRN_E_ARG;
+  int rc;
+  if ((rc = check_field(ctx, field))) return rc;
+  if (id < 0 || id >= (int)ctx->descs.size()) return fail(ctx, DRN_E_ARG, "patch id out of range");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t n = drn_patch_cells(ctx, id), es = ctx->esize();
+  char* b = (char*)ctx->buf[ctx->map[field]];
+  for (int v = 0; v < 5; ++v)
+    HIPCHK(ctx, hipMemcpyAsync(b + (v * ctx->vstride + ctx->offs[id]) * es, (const char*)host + v * n * es, n * es,
+                               hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return DRN_OK;
+}
+
+int drn_patch_download(drn_ctx* ctx, int id, int field, void* host) {
+  if (!ctx || !host) return DRN_E_ARG;
+  int rc;
+  if ((rc = check_field(ctx, field))) return rc;
+  if (id < 0 || id >= (int)ctx->descs.size()) return fail(ctx, DRN_E_ARG, "patch id out of range");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t n = drn_patch_cells(ctx, id), es = ctx->esize();
+  const char* b = (const char*)ctx->buf[ctx->map[field]];
+  for (int v = 0; v < 5; ++v)
+    HIPCHK(ctx, hipMemcpyAsync((char*)host + v * n * es, b + (v * ctx->vstride + ctx->offs[id]) * es, n * es,
+                               hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return DRN_OK;
+}
+
+int drn_copy_field(drn_ctx* ctx, int src, int dst) {
+  if (!ctx) return DRN_E_ARG;
+  int rc;
+  if ((rc = check_field(ctx, src)) || (rc = check_field(ctx, dst))) return rc;
+  if (src == dst) return DRN_OK;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->buf[ctx->map[dst]], ctx->buf[ctx->map[src]], 5 * ctx->vstride * ctx->esize(),
+                             hipMemcpyDeviceToDevice, ctx->stream));
+  return DRN_OK;
+}
+
+int drn_swap_fields(drn_ctx* ctx, int a, int b) {
+  if (!ctx) return DRN_E_ARG;
+  int rc;
+  if ((rc = check_field(ctx, a)) || (rc = check_field(ctx, b))) return rc;
+  std::swap(ctx->map[a], ctx->map[b]);
+  return DRN_OK;
+}
+
+int drn_stage(drn_ctx* ctx, int src, int base, int dst, double factor, int launch) {
+  if (!ctx) return DRN_E_ARG;
+  int rc;
+  if ((rc = check_field(ctx, src)) || (rc = check_field(ctx, base)) || (rc = check_field(ctx, dst))) return rc;
+  if (src == dst) return fail(ctx, DRN_E_ARG, "drn_stage: dst must differ from src");
+  if (launch != DRN_LAUNCH_BATCHED && launch != DRN_LAUNCH_PER_PATCH) return fail(ctx, DRN_E_ARG, "unknown launch form");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  return do_stage(ctx, ctx->map[src], ctx->map[base], ctx->map[dst], factor, launch);
+}
+
+int drn_exchange_set(drn_ctx* ctx, int recv_id, int n_donors, const int32_t* donor_patch, const int64_t* donor_info,
+                     int64_t n_unique, const int64_t* rec_unique, int64_t n_rec_concat, const int64_t* rec_concat,
+                     int64_t n_wi, const int64_t* donor_index, const double* donor_weight) {
+  if (!ctx) return DRN_E_ARG;
+  const int np = (int)ctx->descs.size();
+  if (recv_id < 0 || recv_id >= np) return fail(ctx, DRN_E_ARG, "receiver patch id out of range");
+  if (n_donors < 0 || n_unique < 0 || n_rec_concat < 0 || n_wi < 0) return fail(ctx, DRN_E_ARG, "negative count");
+  const int64_t nrecv_cells = (int64_t)drn_patch_cells(ctx, recv_id);
+  ExchangeTables t;
+  t.donor_patch.assign(donor_patch, donor_patch + n_donors);
+  t.donor_info.assign(donor_info, donor_info + 4 * (size_t)n_donors);
+  t.rec_unique.assign(rec_unique, rec_unique + n_unique);
+  t.rec_concat.assign(rec_concat, rec_concat + n_rec_concat);
+  t.donor_index.assign(donor_index, donor_index + n_wi);
+  t.donor_weight.assign(donor_weight, donor_weight + n_wi);
+  for (int d = 0; d < n_donors; ++d) {
+    const int dp = t.donor_patch[d];
+    if (dp < 0 || dp >= np) return fail(ctx, DRN_E_RANGE, "donor patch id out of range");
+    const int64_t nrec = t.donor_info[4 * d], stride = t.donor_info[4 * d + 1], r0 = t.donor_info[4 * d + 2],
+                  w0 = t.donor_info[4 * d + 3];
+    if (nrec < 0 || stride < 1 || stride > 64 || r0 < 0 || r0 + nrec > n_rec_concat || w0 < 0 ||
+        w0 + nrec * stride > n_wi)
+      return fail(ctx, DRN_E_RANGE, "donor table extents inconsistent");
+    const int64_t ndc = (int64_t)drn_patch_cells(ctx, dp);
+    for (int64_t a = w0; a < w0 + nrec * stride; ++a)
+      if (t.donor_index[a] < 0 || t.donor_index[a] >= ndc) return fail(ctx, DRN_E_RANGE, "donor cell outside donor patch");
+  }
+  for (int64_t a = 0; a < n_rec_concat; ++a)
+    if (t.rec_concat[a] < 0 || t.rec_concat[a] >= nrecv_cells) return fail(ctx, DRN_E_RANGE, "receiver cell outside patch");
+  for (int64_t a = 0; a < n_unique; ++a)
+    if (t.rec_unique[a] < 0 || t.rec_unique[a] >= nrecv_cells) return fail(ctx, DRN_E_RANGE, "receiver cell outside patch");
+  ctx->xt[recv_id] = std::move(t);
+  return DRN_OK;
+}
+
+template <typename T>
+static int upload_vec(drn_ctx* ctx, T** dptr, const std::vector<T>& h) {
+  HIPCHK(ctx, dalloc(dptr, h.size()));
+  if (!h.empty()) HIPCHK(ctx, hipMemcpy(*dptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return DRN_OK;
+}
+
+static int upload_weights(drn_ctx* ctx, void** dptr, const std::vector<double>& h) {
+  if (ctx->prec == DRN_F32) {
+    std::vector<float> f(h.begin(), h.end());
+    float* p = nullptr;
+    int rc = upload_vec(ctx, &p, f);
+    *dptr = p;
+    return rc;
+  }
+  double* p = nullptr;
+  int rc = upload_vec(ctx, &p, h);
+  *dptr = p;
+  return rc;
+}
+
+int drn_exchange_commit(drn_ctx* ctx) {
+  if (!ctx) return DRN_E_ARG;
+  if (!ctx->laid_out) return fail(ctx, DRN_E_STATE, "drn_layout not called");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  free_exchange(ctx);
+  // per unique receiver: its contribution groups in donor order
+  struct Group {
+    int stride;
+    std::vector<uint32_t> idx;
+    std::vector<double> w;
+  };
+  std::map<int, std::vector<std::pair<uint32_t, std::vector<Group>>>> single_by_stride;
+  std::vector<std::pair<uint32_t, std::vector<Group>>> multi;
+  for (auto& kv : ctx->xt) {
+    const int p = kv.first;
+    const ExchangeTables& t = kv.second;
+    std::map<int64_t, std::vector<Group>> groups;
+    for (size_t d = 0; d < t.donor_patch.size(); ++d) {
+      const int dp = t.donor_patch[d];
+      const int64_t nrec = t.donor_info[4 * d], stride = t.donor_info[4 * d + 1], r0 = t.donor_info[4 * d + 2],
+                    w0 = t.donor_info[4 * d + 3];
+      for (int64_t ll = 0; ll < nrec; ++ll) {
+        Group g;
+        g.stride = (int)stride;
+        for (int64_t s = 0; s < stride; ++s) {
+          g.idx.push_back((uint32_t)(ctx->offs[dp] + (uint64_t)t.donor_index[w0 + ll * stride + s]));
+          g.w.push_back(t.donor_weight[w0 + ll * stride + s]);
+        }
+        groups[t.rec_concat[r0 + ll]].push_back(std::move(g));
+      }
+    }
+    for (int64_t r : t.rec_unique) {
+      auto it = groups.find(r);
+      if (it == groups.end()) return fail(ctx, DRN_E_RANGE, "unique receiver without donor contribution");
+      const uint32_t g = (uint32_t)(ctx->offs[p] + (uint64_t)r);
+      if (it->second.size() == 1)
+        single_by_stride[it->second[0].stride].push_back({g, std::move(it->second)});
+      else
+        multi.push_back({g, std::move(it->second)});
+    }
+  }
+  for (auto& kv : single_by_stride) {
+    drn_ctx::SingleClass c{};
+    c.stride = kv.first;
+    c.n = (uint32_t)kv.second.size();
+    std::vector<uint32_t> rec, idx;
+    std::vector<double> w;
+    rec.reserve(c.n);
+    idx.reserve((size_t)c.n * c.stride);
+    w.reserve((size_t)c.n * c.stride);
+    for (auto& e : kv.second) {
+      rec.push_back(e.first);
+      idx.insert(idx.end(), e.second[0].idx.begin(), e.second[0].idx.end());
+      w.insert(w.end(), e.second[0].w.begin(), e.second[0].w.end());
+    }
+    int rc;
+    if ((rc = upload_vec(ctx, &c.rec, rec)) || (rc = upload_vec(ctx, &c.idx, idx)) ||
+        (rc = upload_weights(ctx, &c.w, w)))
+      return rc;
+    ctx->xsingle.push_back(c);
+  }
+  if (!multi.empty()) {
+    std::vector<uint32_t> rec, gptr{0}, gstride, gwi, idx;
+    std::vector<double> w;
+    for (auto& e : multi) {
+      rec.push_back(e.first);
+      for (auto& g : e.second) {
+        gstride.push_back((uint32_t)g.stride);
+        gwi.push_back((uint32_t)idx.size());
+        idx.insert(idx.end(), g.idx.begin(), g.idx.end());
+        w.insert(w.end(), g.w.begin(), g.w.end());
+      }
+      gptr.push_back((uint32_t)gstride.size());
+    }
+    ctx->xmulti_n = (uint32_t)rec.size();
+    int rc;
+    if ((rc = upload_vec(ctx, &ctx->xm_rec, rec)) || (rc = upload_vec(ctx, &ctx->xm_gptr, gptr)) ||
+        (rc = upload_vec(ctx, &ctx->xm_gstride, gstride)) || (rc = upload_vec(ctx, &ctx->xm_gwi, gwi)) ||
+        (rc = upload_vec(ctx, &ctx->xm_idx, idx)) || (rc = upload_weights(ctx, &ctx->xm_w, w)))
+      return rc;
+  }
+  ctx->have_exchange = !ctx->xsingle.empty() || ctx->xmulti_n > 0;
+  return DRN_OK;
+}
+
+int drn_exchange(drn_ctx* ctx, int field) {
+  if (!ctx) return DRN_E_ARG;
+  int rc;
+  if ((rc = check_field(ctx, field))) return rc;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  return do_exchange(ctx, ctx->map[field]);
+}
+
+int drn_levelset_set(drn_ctx* ctx, int patch, int64_t n, const int64_t* cells, const double* normals,
+                     const int64_t* donors, const double* weights) {
+  if (!ctx || n < 0 || (n > 0 && (!cells || !normals || !donors || !weights))) return DRN_E_ARG;
+  if (patch < 0 || patch >= (int)ctx->descs.size()) return fail(ctx, DRN_E_ARG, "patch id out of range");
+  const int64_t nc = (int64_t)drn_patch_cells(ctx, patch);
+  LsTables t;
+  t.cells.assign(cells, cells + n);
+  t.normals.assign(normals, normals + 3 * n);
+  t.donors.assign(donors, donors + 8 * n);
+  t.weights.assign(weights, weights + 8 * n);
+  for (int64_t a = 0; a < n; ++a)
+    if (t.cells[a] < 0 || t.cells[a] >= nc) return fail(ctx, DRN_E_RANGE, "level-set cell outside patch");
+  for (int64_t a = 0; a < 8 * n; ++a)
+    if (t.donors[a] < 0 || t.donors[a] >= nc) return fail(ctx, DRN_E_RANGE, "mirror donor outside patch");
+  ctx->lt[patch] = std::move(t);
+  return DRN_OK;
+}
+
+int drn_levelset_commit(drn_ctx* ctx) {
+  if (!ctx) return DRN_E_ARG;
+  if (!ctx->laid_out) return fail(ctx, DRN_E_STATE, "drn_layout not called");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  free_ls(ctx);
+  std::vector<uint32_t> cells, donors;
+  std::vector<double> nrm, w;
+  for (auto& kv : ctx->lt) {
+    const uint64_t off = ctx->offs[kv.first];
+    const LsTables& t = kv.second;
+    for (size_t a = 0; a < t.cells.size(); ++a) cells.push_back((uint32_t)(off + t.cells[a]));
+    for (size_t a = 0; a < t.donors.size(); ++a) donors.push_back((uint32_t)(off + t.donors[a]));
+    nrm.insert(nrm.end(), t.normals.begin(), t.normals.end());
+    w.insert(w.end(), t.weights.begin(), t.weights.end());
+  }
+  ctx->ls_n = (uint32_t)cells.size();
+  if (ctx->ls_n == 0) return DRN_OK;
+  int rc;
+  if ((rc = upload_vec(ctx, &ctx->ls_cells, cells)) || (rc = upload_vec(ctx, &ctx->ls_donors, donors)) ||
+      (rc = upload_weights(ctx, &ctx->ls_nrm, nrm)) || (rc = upload_weights(ctx, &ctx->ls_w, w)))
+    return rc;
+  HIPCHK(ctx, hipMalloc(&ctx->ls_acc, (size_t)5 * ctx->ls_n * ctx->esize() + 16));
+  return DRN_OK;
+}
+
+int drn_levelset_fill(drn_ctx* ctx, int field, double relax) {
+  if (!ctx) return DRN_E_ARG;
+  int rc;
+  if ((rc = check_field(ctx, field))) return rc;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  return do_ls_fill(ctx, ctx->map[field], relax);
+}
+
+int drn_rk_step(drn_ctx* ctx, double dt, int n_alpha, const double* alpha, int launch) {
+  if (!ctx || n_alpha < 1 || !alpha) return DRN_E_ARG;
+  if (!ctx->laid_out) return fail(ctx, DRN_E_STATE, "drn_layout not called");
+  if (launch != DRN_LAUNCH_BATCHED && launch != DRN_LAUNCH_PER_PATCH) return fail(ctx, DRN_E_ARG, "unknown launch form");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  return do_rk_step(ctx, dt, n_alpha, alpha, launch);
+}
+
+int drn_rk_graph_run(drn_ctx* ctx, double dt, int n_alpha, const double* alpha, int n_steps) {
+  if (!ctx || n_alpha < 1 || !alpha || n_steps < 0) return DRN_E_ARG;
+  if (!ctx->laid_out) return fail(ctx, DRN_E_STATE, "drn_layout not called");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (n_steps == 0) return DRN_OK;
+  if (ctx->first_step) {
+    // the leading exchange of the very first step runs eagerly
+    int rc = do_exchange(ctx, ctx->map[0]);
+    if (rc) return rc;
+    ctx->first_step = false;
+  }
+  // A step permutes the three buffers; three steps (or one, when the stage
+  // count brings the mapping back) form a cycle that is captured once.
+  std::vector<double> al(alpha, alpha + n_alpha);
+  const bool same = ctx->gexec && ctx->g_dt == dt && ctx->g_alpha == al;
+  int map0[NBUF];
+  memcpy(map0, ctx->map, sizeof(map0));
+  // find cycle length of the permutation
+  int cyc = 0;
+  {
+    int m[NBUF];
+    memcpy(m, ctx->map, sizeof(m));
+    for (cyc = 1; cyc <= 6; ++cyc) {
+      const int X = m[0], Y = m[1] == X ? m[2] : m[1];
+      int Z = 0;
+      while (Z == X || Z == Y) ++Z;
+      int cur = X;
+      for (int s = 0; s < n_alpha; ++s) cur = (s % 2 == 0) ? Y : Z;
+      m[0] = cur;
+      m[1] = X;
+      m[2] = (cur == Y) ? Z : Y;
+      if (memcmp(m, map0, sizeof(m)) == 0) break;
+    }
+  }
+  if (!same || ctx->g_steps != cyc) {
+    if (ctx->gexec) {
+      (void)hipGraphExecDestroy(ctx->gexec);
+      ctx->gexec = nullptr;
+    }
+    const bool was_timing = ctx->timing;
+    ctx->timing = false;
+    hipGraph_t graph;
+    HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = DRN_OK;
+    for (int s = 0; s < cyc && rc == DRN_OK; ++s) rc = do_rk_step(ctx, dt, n_alpha, alpha, DRN_LAUNCH_BATCHED);
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    ctx->timing = was_timing;
+    memcpy(ctx->map, map0, sizeof(map0));
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(ctx, DRN_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&ctx->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(ctx, DRN_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    ctx->g_dt = dt;
+    ctx->g_alpha = al;
+    ctx->g_steps = cyc;
+  }
+  int done = 0;
+  while (done + cyc <= n_steps) {
+    HIPCHK(ctx, hipGraphLaunch(ctx->gexec, ctx->stream));
+    done += cyc;
+  }
+  // the remainder runs eagerly and advances the mapping
+  for (; done < n_steps; ++done) {
+    int rc = do_rk_step(ctx, dt, n_alpha, alpha, DRN_LAUNCH_BATCHED);
+    if (rc) return rc;
+  }
+  return DRN_OK;
+}
+
+template <typename T>
+static int sums_t(drn_ctx* ctx, const T* U, double* out) {
+  const int nt = (int)ctx->tiles.size();
+  hipLaunchKernelGGL(k_sums_partial<T>, dim3(nt), dim3(256), 0, ctx->stream, ctx->d_pd, ctx->d_tiles, U, ctx->vstride,
+                     ctx->d_part);
+  hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(256), 0, ctx->stream, (uint32_t)nt, ctx->d_part, ctx->d_out);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipMemcpyAsync(out, ctx->d_out, 5 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return DRN_OK;
+}
+
+int drn_core_sums(drn_ctx* ctx, int field, double* out) {
+  if (!ctx || !out) return DRN_E_ARG;
+  int rc;
+  if ((rc = check_field(ctx, field))) return rc;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const void* U = ctx->buf[ctx->map[field]];
+  return ctx->prec == DRN_F32 ? sums_t(ctx, (const float*)U, out) : sums_t(ctx, (const double*)U, out);
+}
+
+template <typename T>
+static int diag_t(drn_ctx* ctx, const T* U, double dt, double* out) {
+  const int nt = (int)ctx->tiles.size();
+  hipLaunchKernelGGL(k_diag_partial<T>, dim3(nt), dim3(256), 0, ctx->stream, ctx->d_pd, ctx->d_tiles, U, ctx->vstride,
+                     dt, ctx->d_part);
+  HIPCHK(ctx, hipGetLastError());
+  std::vector<double> h((size_t)nt * 4);
+  HIPCHK(ctx, hipMemcpyAsync(h.data(), ctx->d_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  double o[4] = {0, 1e300, -1e300, 0};
+  for (int b = 0; b < nt; ++b) {
+    o[0] = std::max(o[0], h[4 * b]);
+    o[1] = std::min(o[1], h[4 * b + 1]);
+    o[2] = std::max(o[2], h[4 * b + 2]);
+    o[3] += h[4 * b + 3];
+  }
+  memcpy(out, o, sizeof(o));
+  return DRN_OK;
+}
+
+int drn_diagnostics(drn_ctx* ctx, int field, double dt, double* out) {
+  if (!ctx || !out) return DRN_E_ARG;
+  int rc;
+  if ((rc = check_field(ctx, field))) return rc;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const void* U = ctx->buf[ctx->map[field]];
+  return ctx->prec == DRN_F32 ? diag_t(ctx, (const float*)U, dt, out) : diag_t(ctx, (const double*)U, dt, out);
+}
+
+int drn_autotune(drn_ctx* ctx, int* choice) {
+  if (!ctx) return DRN_E_ARG;
+  if (choice) *choice = ctx->tiling;
+  return DRN_OK;
+}
+
+int drn_set_tiling(drn_ctx* ctx, int choice) {
+  if (!ctx) return DRN_E_ARG;
+  if (choice != 0) return fail(ctx, DRN_E_ARG, "unknown tiling");
+  ctx->tiling = choice;
+  return DRN_OK;
+}
+
+int drn_timing_enable(drn_ctx* ctx, int on) {
+  if (!ctx) return DRN_E_ARG;
+  ctx->timing = on != 0;
+  return DRN_OK;
+}
+
+int drn_timing(drn_ctx* ctx, int which, double* avg_ms, int64_t* count) {
+  if (!ctx || which < 0 || which > 2) return DRN_E_ARG;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  TimerSlot& s = ctx->timers[which];
+  double tot = 0;
+  int64_t n = 0;
+  for (auto& e : s.used) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) {
+      tot += ms;
+      ++n;
+    }
+    s.pool.push_back(e);
+  }
+  s.used.clear();
+  if (avg_ms) *avg_ms = n ? tot / n : 0.0;
+  if (count) *count = n;
+  return DRN_OK;
+}
+
+int drn_sync(drn_ctx* ctx) {
+  if (!ctx) return DRN_E_ARG;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return DRN_OK;
+}
+

@@ -1,0 +1,185 @@
+/*
+ * drnhip.h -- C ABI of libdrnhip.so, the MI355X (gfx950) implementation of
+ * DrNUM's per-patch explicit compressible update.
+ *
+ * The reference drives this path through C++ templates and CUDA kernels:
+ *   GPU_PatchIterator / GPU_CartesianIterator (src/drnumlib/iterators/gpu_patchiterator.h:31-233,
+ *   gpu_cartesianiterator.h:29-304), the flux-operator functors (fluxes/ausmplus.h:27-109 with
+ *   reconstruction/upwind2.h + vanalbada.h / minmod.h, perfectgas.h), the donor exchange
+ *   (Patch::accessDonorDataDirect, patch.cpp:573-653) and the level-set post operation
+ *   (CPU_LevelSetObjectBC, cpu_levelsetobjectbc.h:58-90).
+ * A C ABI cannot carry those templates, so the operator is described by enums
+ * (the host maps each supported OP type to them) and patches by POD descriptors.
+ *
+ * Conventions: every int-returning call returns 0 on success or a negative
+ * DRN_E* code; drn_last_error() gives the message.  The library never exits.
+ * One context = one device + one HIP stream; calls are asynchronous on that
+ * stream unless stated; drn_sync() is the barrier.  Calls on one context must
+ * come from one host thread.  Host buffers are borrowed only for the duration
+ * of an upload/download call.
+ *
+ * Data layout on the device: a context holds three state buffers (logical
+ * fields 0, 1, 2 of the reference, patch_common.h:72-95).  Each buffer is
+ * variable-major over all patches: variable v of patch p, cell l = i*NJ*NK +
+ * j*NK + k, lives at buffer + v*stride + offset(p) + l, with offset(p) aligned
+ * to 64 cells.  Host transfers use the reference's per-patch layout: 5 blocks
+ * of N values, one per variable.
+ */
+#ifndef DRNHIP_H
+#define DRNHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct drn_ctx drn_ctx;
+
+/* state precision of a context */
+enum { DRN_F32 = 0, DRN_F64 = 1 };
+/* reconstruction: reconstruction/upwind1.h:29-39, upwind2.h:29-52 */
+enum { DRN_RECON_UPWIND1 = 0, DRN_RECON_UPWIND2 = 1 };
+/* limiter: reconstruction/vanalbada.h:28-33, minmod.h:28-31, secondorder.h:28-31 */
+enum { DRN_LIM_VANALBADA = 0, DRN_LIM_MINMOD = 1, DRN_LIM_SECONDORDER = 2 };
+/* face functor of a patch face (xWallM/xWallP/yWallM/yWallP/zWallM/zWallP):
+ *   SLIP         fluxes/compressibleslipflux.h:34-98
+ *   FARFIELD     fluxes/compressiblefarfieldflux.h:93-181 (Upwind1 projection, as JetFlux's farfield_t)
+ *   TRANSMISSIVE zero-gradient outflow (no counterpart in the reference)
+ *   JET          applications/drnumJetDemo/jetflux.h:158-192, top-hat inflow with seeded velocity noise
+ *   NONE         no wall flux (faces whose cells are all exchange receivers) */
+enum { DRN_BC_SLIP = 0, DRN_BC_FARFIELD = 1, DRN_BC_TRANSMISSIVE = 2, DRN_BC_JET = 3, DRN_BC_NONE = 4 };
+#define DRN_BC_NPAR 16
+
+/* launch forms of a stage */
+enum {
+  DRN_LAUNCH_BATCHED = 0,   /* one launch over the tile work-list of all patches */
+  DRN_LAUNCH_PER_PATCH = 1  /* one launch per patch (the plain drop-in form)    */
+};
+
+/* error codes */
+enum {
+  DRN_OK = 0,
+  DRN_E_ARG = -1,      /* invalid argument (sizes, ids, enums)   */
+  DRN_E_HIP = -2,      /* HIP runtime error                       */
+  DRN_E_STATE = -3,    /* call out of order (e.g. before layout)  */
+  DRN_E_NOMEM = -4,    /* device allocation failed                */
+  DRN_E_RANGE = -5     /* table entry outside the patch or grid   */
+};
+
+typedef struct drn_patch_desc {
+  int32_t ni, nj, nk;            /* cells per axis (seek layers included)                   */
+  int32_t seek[6];               /* seek layers imin, imax, jmin, jmax, kmin, kmax (core)    */
+  double dx, dy, dz;             /* spacing as the host computed it (L/N in the state type)  */
+  int32_t bc[6];                 /* DRN_BC_* per face xm, xp, ym, yp, zm, zp                 */
+  double bc_par[6][DRN_BC_NPAR]; /* FARFIELD: [0..4] far state in CONSERVED variables;       *
+                                  * JET: [0] c_a [1] c_b [2] radius [3] jet speed [4] T_jet     *
+                                  *      [5] far normal speed [6] T_far [7] noise amplitude    *
+                                  *      [8] seed [9],[10] global index offsets (axes a, b)    */
+} drn_patch_desc;
+
+/* ---- context ------------------------------------------------------------ */
+/* Replaces the device selection and stream of GPU_PatchIterator(OP, cuda_device, thread_limit)
+ * (gpu_patchiterator.h:68-83). */
+int drn_ctx_create(int device, int precision, drn_ctx** out);
+void drn_ctx_destroy(drn_ctx* ctx);
+const char* drn_last_error(const drn_ctx* ctx);
+int drn_precision(const drn_ctx* ctx);
+
+/* The operator: AusmPlus<5, Upwind1|Upwind2<5, LIM>, PerfectGas> (ausmplus.h:27-109). */
+int drn_set_operator(drn_ctx* ctx, int recon, int limiter);
+
+/* ---- patches ------------------------------------------------------------ */
+/* Replaces PatchIterator::addPatch (patchiterator.h:124-128) + the lazy device
+ * mirror GPU_Patch (gpu_patch.h:43-129).  Patches are added before drn_layout. */
+int drn_patch_add(drn_ctx* ctx, const drn_patch_desc* desc, int* id);
+/* Allocate the three state buffers for all added patches (gpu_patch.h:57-128). */
+int drn_layout(drn_ctx* ctx);
+int drn_num_patches(const drn_ctx* ctx);
+size_t drn_patch_cells(const drn_ctx* ctx, int id);
+/* Host <-> device copy of one field of one patch (GPU_Patch::copyToDevice /
+ * copyFromDevice, gpu_patch.h:131-140; updateDevice/updateHost, gpu_patchiterator.h:85-126).
+ * `host` holds 5*N values of the context precision, variable-major. Synchronous. */
+int drn_patch_upload(drn_ctx* ctx, int id, int field, const void* host);
+int drn_patch_download(drn_ctx* ctx, int id, int field, void* host);
+/* PatchIterator::copyField (patchiterator.h:96-101) for all patches. */
+int drn_copy_field(drn_ctx* ctx, int src, int dst);
+/* Exchange the device buffers behind two logical fields (no data movement).
+ * The host iterator uses it to give drn_stage the in-place semantics of
+ * CartesianIterator::compute (field 0 = field 1 + factor/dV * Res(field 0)). */
+int drn_swap_fields(drn_ctx* ctx, int a, int b);
+
+/* ---- one RK stage: flux + residual + update ------------------------------ */
+/* GPU_CartesianIterator::compute (gpu_cartesianiterator.h:219-304) and
+ * CartesianIterator::compute (cartesianiterator.h:87-325): for every patch
+ *   field dst = field base + factor/dV * Res(field src)
+ * where Res sums the AUSM+ face fluxes of all cell faces and the face functors
+ * at the patch faces.  dst must differ from src; dst may equal base.
+ * `launch` is DRN_LAUNCH_BATCHED or DRN_LAUNCH_PER_PATCH. */
+int drn_stage(drn_ctx* ctx, int src, int base, int dst, double factor, int launch);
+
+/* ---- donor exchange ------------------------------------------------------ */
+/* Tables of one receiver patch in the reference's direct transfer-list form
+ * (Patch::buildDonorTransferData, patch.cpp:128-218; donor_t, donor_t.h:28-45):
+ *   n_donors donors; donor d is patch donor_patch[d] and serves
+ *   donor_info[4d+0] receiver cells with stride donor_info[4d+1], its receivers
+ *   start at donor_info[4d+2] in rec_concat and its (index, weight) pairs at
+ *   donor_info[4d+3] in donor_index / donor_weight.  rec_unique lists every
+ *   receiver cell once (zeroed before the donor sums).  Cell indices are local
+ *   to their patch.  Weights are FP64 and rounded to the state precision. */
+int drn_exchange_set(drn_ctx* ctx, int recv_id, int n_donors, const int32_t* donor_patch,
+                     const int64_t* donor_info, int64_t n_unique, const int64_t* rec_unique,
+                     int64_t n_rec_concat, const int64_t* rec_concat, int64_t n_wi,
+                     const int64_t* donor_index, const double* donor_weight);
+/* Upload all tables set so far into the device gather lists. */
+int drn_exchange_commit(drn_ctx* ctx);
+/* PatchIterator::copyDonorData(field) (patchiterator.h:103-108, gpu_patchiterator.h:191-233). */
+int drn_exchange(drn_ctx* ctx, int field);
+
+/* ---- level-set boundary fill -------------------------------------------- */
+/* Inner layer cells of one patch in the flat order of
+ * LevelSetObjectBC::transferCellLayerData (levelsetobjectbc.cpp:38-134):
+ * cell, unit normal (gx, gy, gz), 8 mirror donors and weights (levelsetobject.cpp:294-359). */
+int drn_levelset_set(drn_ctx* ctx, int patch, int64_t n, const int64_t* cells, const double* normals,
+                     const int64_t* donors, const double* weights);
+int drn_levelset_commit(drn_ctx* ctx);
+/* CPU_LevelSetObjectBC<LSOBCCompressibleEulerOp>::operator() (cpu_levelsetobjectbc.h:58-90,
+ * lsobccompressibleeulerop.h:49-144) on `field`; the mirror gather goes to
+ * device scratch (the reference's abuse field). relax is 0.5 in the reference. */
+int drn_levelset_fill(drn_ctx* ctx, int field, double relax);
+
+/* ---- time step ------------------------------------------------------------ */
+/* RungeKutta::operator()(dt) (rungekutta.cpp:27-42): one low-storage RK step
+ * with coefficients alpha[0..n-1], the exchange after every stage and the
+ * level-set fill (if tables are set) as the post operation.  The state is in
+ * field 0 before and after; fields 1 and 2 are used as stage storage.  The
+ * exchange before stages 2..n is repeated only when a post operation exists
+ * (without one it reproduces the same values). */
+int drn_rk_step(drn_ctx* ctx, double dt, int n_alpha, const double* alpha, int launch);
+/* Capture `n_steps` drn_rk_step calls in a HIP graph and replay it. */
+int drn_rk_graph_run(drn_ctx* ctx, double dt, int n_alpha, const double* alpha, int n_steps);
+
+/* ---- reductions (FP64 accumulators) -------------------------------------- */
+/* out[0..4]: sum over core cells of (rho, rhou, rhov, rhow, rhoE) * dV. Synchronous. */
+int drn_core_sums(drn_ctx* ctx, int field, double* out);
+/* out[0] = max CFL number |u|+a times dt/dx (jet main.h:188-210), out[1] = rho min,
+ * out[2] = rho max, out[3] = number of non-finite values.  Synchronous. */
+int drn_diagnostics(drn_ctx* ctx, int field, double dt, double* out);
+
+/* ---- tuning, timing, sync ------------------------------------------------ */
+/* Time the candidate tile shapes / work-list orders of the batched stage kernel
+ * and keep the fastest (writes the choice to *choice if not NULL). */
+int drn_autotune(drn_ctx* ctx, int* choice);
+/* Select a tile shape / work-list order explicitly (see DRN_TILE_*). */
+int drn_set_tiling(drn_ctx* ctx, int choice);
+/* Average device time (ms) of the last timed kernel class, measured with HIP
+ * events on the context stream: which = 0 stage kernel, 1 exchange, 2 level-set fill. */
+int drn_timing(drn_ctx* ctx, int which, double* avg_ms, int64_t* count);
+int drn_timing_enable(drn_ctx* ctx, int on);
+int drn_sync(drn_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
