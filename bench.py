@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Benchmark: Mcell-updates/s per RK stage of the 64-patch subsonic jet on one MI355X.
+
+Workload: 256^3 core cells as 4x4x4 patches of 64^3 core cells with two seek
+layers on every interior patch face (patches stored as 66 or 68 cells per axis,
+19,248,832 stored cells), AUSM+ with MUSCL/van Albada, 3-stage RK
+(alpha = 1/4, 1/2, 1), exchange after every stage, top-hat Mach-0.9 inflow on
+the z=0 face, transmissive elsewhere.  A "step" is one RK step over all patches.
+
+  value       FP32 batched form (one launch per stage over the tile work-list of
+              all 64 patches), state resident in HBM, timed region = K steps.
+  fp64_plain  FP64 plain per-patch form (one launch per patch per stage).
+  roofline    stage kernel: algorithmic bytes per launch (10 or 15 values of
+              4 B per stored cell: stage 1 reads U^n only) / average launch time
+              from HIP events on the context stream over the timed region.
+  cpu_baseline the oracle (C restatement of the reference iterator, OpenMP over
+              patches) on this host, one RK step of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mcell-updates/s per RK stage + % HBM peak; FP32-batched vs FP64-plain"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def setup(w, dtype, launch_tables=True):
+    from drnum_amd import hip, host
+
+    hg = host.HostGrid.from_workload(w, fp64_geometry=True)
+    hg.compute_dependencies(w.periodic)
+    ctx = hip.Context(dtype)
+    ctx.set_operator(w.recon, w.limiter)
+    hg.to_device(ctx)
+    if launch_tables:
+        hg.upload_exchange(ctx)
+    for p in range(len(w.patches)):
+        ctx.upload(p, 0, w.initial_state(p, dtype))
+    return hg, ctx
+
+
+def stage_bytes_per_step(cells, esize, n_stages):
+    # stage 1 reads U^n once (src == base) and writes 5 values; later stages read src and base
+    vals = 10 + 15 * (n_stages - 1)
+    return cells * vals * esize
+
+
+def run_form(w, dtype, steps, warmup, launch, label):
+    from drnum_amd import hip
+
+    hg, ctx = setup(w, dtype)
+    cells = w.total_cells
+    esize = np.dtype(dtype).itemsize
+    S = len(w.alpha)
+    for _ in range(warmup):
+        ctx.rk_step(w.dt, w.alpha, launch)
+    ctx.sync()
+    ctx.timing(0)
+    ctx.timing(1)
+    ctx.timing_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.rk_step(w.dt, w.alpha, launch)
+    ctx.sync()
+    t1 = time.perf_counter()
+    ctx.timing_enable(False)
+    st_ms, st_n = ctx.timing(0)
+    ex_ms, ex_n = ctx.timing(1)
+    wall = t1 - t0
+    diag = ctx.diagnostics(0, w.dt)
+    log("%s: %d steps %.3f s, stage kernel avg %.4f ms (%d), exchange avg %.4f ms (%d), diag %s"
+        % (label, steps, wall, st_ms, st_n, ex_ms, ex_n, diag))
+    if diag["nonfinite"] > 0:
+        raise RuntimeError("%s: non-finite values after the timed steps" % label)
+    res = dict(wall=wall, ms_per_step=1e3 * wall / steps, mcups=cells * S * steps / wall / 1e6,
+               stage_ms=st_ms, stage_n=st_n, exch_ms=ex_ms, exch_n=ex_n)
+    # per launch the stage kernel advances all stored cells once
+    bytes_step = stage_bytes_per_step(cells, esize, S)
+    res["stage_bytes_per_launch"] = bytes_step / S
+    if launch == hip.DRN_LAUNCH_BATCHED:
+        res["stage_gbs"] = res["stage_bytes_per_launch"] / (st_ms * 1e-3) / 1e9 if st_ms > 0 else 0.0
+    else:
+        # per-patch form: st_ms is one launch over all patches of one stage
+        res["stage_gbs"] = res["stage_bytes_per_launch"] / (st_ms * 1e-3) / 1e9 if st_ms > 0 else 0.0
+    # graph replay of the same form (batched only): captured steps, no per-launch host work
+    if launch == hip.DRN_LAUNCH_BATCHED:
+        ctx.sync()
+        tg0 = time.perf_counter()
+        ctx.rk_graph_run(w.dt, w.alpha, steps)
+        ctx.sync()
+        tg1 = time.perf_counter()
+        res["graph_ms_per_step"] = 1e3 * (tg1 - tg0) / steps
+        res["graph_mcups"] = cells * S * steps / (tg1 - tg0) / 1e6
+    # STREAM-style copy of one state buffer on the same device
+    ctx.copy_field(0, 2)
+    ctx.sync()
+    ctx.timing(3)
+    ctx.timing_enable(True)
+    for _ in range(10):
+        ctx.copy_field(0, 2)
+    ctx.timing_enable(False)
+    cp_ms, cp_n = ctx.timing(3)
+    nbytes = 2 * 5 * ctx_vstride(ctx, w) * esize
+    res["copy_gbs"] = nbytes / (cp_ms * 1e-3) / 1e9 if cp_ms > 0 else 0.0
+    ctx.close()
+    return res
+
+
+def ctx_vstride(ctx, w):
+    # cells per variable block: patches padded to 64 cells
+    return sum((p.cells + 63) // 64 * 64 for p in w.patches)
+
+
+def cpu_baseline(w, dtype, threads):
+    from oracle import oracle as O
+
+    og = O.OracleGrid(w, dtype, threads=threads)
+    og.init_state()
+    og.compute_dependencies()
+    t0 = time.perf_counter()
+    og.rk_step()
+    t1 = time.perf_counter()
+    cells = w.total_cells
+    return dict(value=cells * len(w.alpha) / (t1 - t0) / 1e6, unit="Mcell-updates/s", cores=threads,
+                kind="port", sample="1 RK step (3 stages + 6 exchanges) of the same 64-patch jet, %s oracle, "
+                                    "OpenMP over patches, %.1f s" % (np.dtype(dtype).name, t1 - t0))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--fp64-steps", type=int, default=0, help="steps of the FP64 plain form (default: steps)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--blocks", type=int, default=4)
+    ap.add_argument("--core", type=int, default=64)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (patch sharding across GPUs is "
+              "described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from drnum_amd import configs, hip
+
+    w = configs.jet(core=args.core, blocks=args.blocks)
+    log("workload %s: %d patches, %d stored cells, %d core cells" % (w.name, len(w.patches), w.total_cells,
+                                                                      w.core_cells))
+    f32 = run_form(w, np.float32, args.steps, args.warmup, hip.DRN_LAUNCH_BATCHED, "fp32-batched")
+    f64 = run_form(w, np.float64, args.fp64_steps or args.steps, args.warmup, hip.DRN_LAUNCH_PER_PATCH,
+                   "fp64-plain")
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(w, np.float32, threads)
+        log("cpu baseline", cpu)
+
+    ach = f32["stage_gbs"]
+    out = {
+        "metric": METRIC,
+        "value": round(f32["mcups"], 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(f32["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "subsonic jet, 256^3 core as 4x4x4 patches of 64^3 core + 2 seek layers on interior "
+                        "faces; AUSM+ MUSCL van Albada, 3-stage RK, exchange every stage",
+            "patches": len(w.patches),
+            "cells_stored": w.total_cells,
+            "cells_core": w.core_cells,
+            "stages": len(w.alpha),
+            "form": "fp32 batched (one stage launch over all patches)",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(ach, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "fp64_plain": {
+            "value": round(f64["mcups"], 2),
+            "ms_per_step": round(f64["ms_per_step"], 4),
+            "stage_kernel_gbs": round(f64["stage_gbs"], 1),
+            "fp32_batched_over_fp64_plain": round(f32["mcups"] / f64["mcups"], 3),
+        },
+        "graph": {
+            "ms_per_step": round(f32["graph_ms_per_step"], 4),
+            "value": round(f32["graph_mcups"], 2),
+        },
+        "stage_kernel_ms": round(f32["stage_ms"], 4),
+        "exchange_ms": round(f32["exch_ms"], 4),
+        "stream_copy_gbs": round(f32["copy_gbs"], 1),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

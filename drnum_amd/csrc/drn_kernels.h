@@ -186,6 +186,12 @@ __global__ __launch_bounds__(BS) void k_stage_cell(const PatchDev* __restrict__ 
   for (int v = 0; v < 5; ++v) dst[v * vs + c] = base[v * vs + c] + pf * res[v];
 }
 
+// plain vectorised copy (PatchIterator::copyField); also the STREAM-style
+// reference kernel whose bytes are known exactly (16 B per lane per iteration)
+__global__ __launch_bounds__(256) void k_copy(uint64_t n16, const uint4* __restrict__ src, uint4* __restrict__ dst) {
+  for (uint64_t q = blockIdx.x * 256ull + threadIdx.x; q < n16; q += (uint64_t)gridDim.x * 256ull) dst[q] = src[q];
+}
+
 // ---------------------------------------------------------------------------
 // exchange (gather): receivers are seek cells, donors are donor-zone cells of
 // other patches, so the two sets never intersect and all receivers of all

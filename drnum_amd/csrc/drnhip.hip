@@ -83,7 +83,7 @@ struct drn_ctx {
 
   // timing
   bool timing = false;
-  TimerSlot timers[3];
+  TimerSlot timers[4];
 
   // graph
   hipGraphExec_t gexec = nullptr;
@@ -489,8 +489,15 @@ int drn_copy_field(drn_ctx* ctx, int src, int dst) {
   int rc;
   if ((rc = check_field(ctx, src)) || (rc = check_field(ctx, dst))) return rc;
   if (src == dst) return DRN_OK;
-  HIPCHK(ctx, hipMemcpyAsync(ctx->buf[ctx->map[dst]], ctx->buf[ctx->map[src]], 5 * ctx->vstride * ctx->esize(),
-                             hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  // buffers hold 5*vstride values, vstride a multiple of 64 cells: a whole number of 16-byte units
+  const uint64_t n16 = 5 * ctx->vstride * ctx->esize() / 16;
+  hipEvent_t stop;
+  timer_begin(ctx, 3, &stop);
+  hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, ctx->stream, n16, (const uint4*)ctx->buf[ctx->map[src]],
+                     (uint4*)ctx->buf[ctx->map[dst]]);
+  timer_end(ctx, stop);
+  HIPCHK(ctx, hipGetLastError());
   return DRN_OK;
 }
 
@@ -862,7 +869,7 @@ int drn_timing_enable(drn_ctx* ctx, int on) {
 }
 
 int drn_timing(drn_ctx* ctx, int which, double* avg_ms, int64_t* count) {
-  if (!ctx || which < 0 || which > 2) return DRN_E_ARG;
+  if (!ctx || which < 0 || which > 3) return DRN_E_ARG;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   TimerSlot& s = ctx->timers[which];
   double tot = 0;

@@ -105,7 +105,6 @@ class Context:
         if rc != 0:
             raise DrnError("drn_ctx_create failed (%d): no usable HIP device %d" % (rc, device))
         self.h = h
-        self.cells = []
 
     def close(self):
         if getattr(self, "h", None):
@@ -138,19 +137,22 @@ class Context:
         d.dx, d.dy, d.dz = (float(x) for x in dxyz)
         pid = C.c_int()
         self._chk(self.L.drn_patch_add(self.h, C.byref(d), C.byref(pid)), "drn_patch_add")
-        self.cells.append(ni * nj * nk)
         return pid.value
 
     def layout(self):
         self._chk(self.L.drn_layout(self.h), "drn_layout")
 
+    def patch_cells(self, pid) -> int:
+        return int(self.L.drn_patch_cells(self.h, pid))
+
     def upload(self, pid, field, data: np.ndarray):
         a = np.ascontiguousarray(data, dtype=self.dtype).reshape(-1)
-        assert a.size == 5 * self.cells[pid]
+        if a.size != 5 * self.patch_cells(pid):
+            raise DrnError("upload: patch %d holds %d cells, got %d values" % (pid, self.patch_cells(pid), a.size))
         self._chk(self.L.drn_patch_upload(self.h, pid, field, _p(a)), "drn_patch_upload")
 
     def download(self, pid, field=0) -> np.ndarray:
-        a = np.empty(5 * self.cells[pid], dtype=self.dtype)
+        a = np.empty(5 * self.patch_cells(pid), dtype=self.dtype)
         self._chk(self.L.drn_patch_download(self.h, pid, field, _p(a)), "drn_patch_download")
         return a.reshape(5, -1)
 

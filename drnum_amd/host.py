@@ -1,0 +1,181 @@
+"""ctypes binding of libdrnum.so (include/drnum_host.h): the host-side patch grid,
+donor tables and level-set tables of the drop-in, built in FP64 geometry."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import configs, hip
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libdrnum.so")
+
+_I64P = C.POINTER(C.c_int64)
+_DP = C.POINTER(C.c_double)
+_SIGS = {
+    "drnh_grid_create": (C.c_void_p, [C.c_int]),
+    "drnh_grid_destroy": (None, [C.c_void_p]),
+    "drnh_last_error": (C.c_char_p, [C.c_void_p]),
+    "drnh_grid_add_patch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int32), _DP, _DP]),
+    "drnh_grid_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
+    "drnh_grid_num_patches": (C.c_int, [C.c_void_p]),
+    "drnh_grid_patch_info": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int32), _DP]),
+    "drnh_grid_set_faces": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int32), _DP]),
+    "drnh_grid_compute_dependencies": (C.c_int, [C.c_void_p, _DP, C.c_int]),
+    "drnh_grid_num_receivers": (C.c_int64, [C.c_void_p, C.c_int]),
+    "drnh_grid_num_no_donor": (C.c_int64, [C.c_void_p, C.c_int]),
+    "drnh_grid_receivers": (C.c_int, [C.c_void_p, C.c_int, _I64P]),
+    "drnh_grid_num_donors": (C.c_int, [C.c_void_p, C.c_int]),
+    "drnh_grid_donor_info": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _I64P]),
+    "drnh_grid_num_concat": (C.c_int64, [C.c_void_p, C.c_int, C.c_int]),
+    "drnh_grid_rec_concat": (C.c_int, [C.c_void_p, C.c_int, _I64P]),
+    "drnh_grid_donor_index": (C.c_int, [C.c_void_p, C.c_int, _I64P]),
+    "drnh_grid_donor_weight": (C.c_int, [C.c_void_p, C.c_int, _DP]),
+    "drnh_grid_levelset_spheres": (C.c_int, [C.c_void_p, C.c_int, _DP, C.c_int, C.c_int, C.c_double]),
+    "drnh_grid_levelset_values": (C.c_int, [C.c_void_p, C.c_int, _DP]),
+    "drnh_grid_ls_num_cells": (C.c_int64, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "drnh_grid_ls_cells": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _I64P, _DP, _I64P, _DP]),
+    "drnh_grid_min_ch_length": (C.c_double, [C.c_void_p]),
+    "drnh_grid_to_device": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "drnh_grid_upload_exchange": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "drnh_grid_upload_levelset": (C.c_int, [C.c_void_p, C.c_void_p]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libdrnum.so is not built (%s); run python -m drnum_amd.build" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class HostGrid:
+    """The drop-in's PatchGrid: patches, face functors, donor tables, level set."""
+
+    def __init__(self, fp64_geometry: bool = True):
+        self.L = lib()
+        self.h = self.L.drnh_grid_create(1 if fp64_geometry else 0)
+        self.fp64 = fp64_geometry
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.drnh_grid_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc < 0:
+            m = self.L.drnh_last_error(self.h)
+            raise RuntimeError("%s failed: %s" % (what, m.decode() if m else ""))
+        return rc
+
+    @classmethod
+    def from_workload(cls, w: configs.Workload, fp64_geometry: bool = True) -> "HostGrid":
+        g = cls(fp64_geometry)
+        for P in w.patches:
+            seek = np.array(P.seek, dtype=np.int32)
+            o = np.array(P.origin, dtype=np.float64)
+            L = np.array(P.length, dtype=np.float64)
+            pid = g._chk(g.L.drnh_grid_add_patch(g.h, P.ni, P.nj, P.nk, _ptr(seek, C.c_int32), _ptr(o, C.c_double),
+                                                 _ptr(L, C.c_double)), "drnh_grid_add_patch")
+            g.set_faces(pid, P.bc, P.bc_par)
+        return g
+
+    def read(self, path, scale=1.0):
+        self._chk(self.L.drnh_grid_read(self.h, path.encode(), scale), "drnh_grid_read")
+
+    def num_patches(self):
+        return self.L.drnh_grid_num_patches(self.h)
+
+    def patch_info(self, p):
+        oi = np.zeros(9, np.int32)
+        od = np.zeros(9, np.float64)
+        self._chk(self.L.drnh_grid_patch_info(self.h, p, _ptr(oi, C.c_int32), _ptr(od, C.c_double)), "patch_info")
+        return dict(n=oi[:3].tolist(), seek=oi[3:].tolist(), origin=od[:3].tolist(), length=od[3:6].tolist(),
+                    delta=od[6:].tolist())
+
+    def set_faces(self, p, bc, par):
+        b = np.ascontiguousarray(bc, dtype=np.int32)
+        q = np.ascontiguousarray(par, dtype=np.float64).reshape(-1)
+        self._chk(self.L.drnh_grid_set_faces(self.h, p, _ptr(b, C.c_int32), _ptr(q, C.c_double)), "set_faces")
+
+    def compute_dependencies(self, period=(0.0, 0.0, 0.0), threads: int = 8):
+        per = np.array(period, dtype=np.float64)
+        self._chk(self.L.drnh_grid_compute_dependencies(self.h, _ptr(per, C.c_double), threads),
+                  "drnh_grid_compute_dependencies")
+
+    def tables(self, p):
+        L, h = self.L, self.h
+        nrec = L.drnh_grid_num_receivers(h, p)
+        recv = np.zeros(max(nrec, 1), np.int64)
+        L.drnh_grid_receivers(h, p, _ptr(recv, C.c_int64))
+        nd = L.drnh_grid_num_donors(h, p)
+        donors = []
+        info = np.zeros(5, np.int64)
+        for d in range(nd):
+            L.drnh_grid_donor_info(h, p, d, _ptr(info, C.c_int64))
+            donors.append(tuple(int(x) for x in info))
+        nrc = L.drnh_grid_num_concat(h, p, 0)
+        nwi = L.drnh_grid_num_concat(h, p, 1)
+        rc = np.zeros(max(nrc, 1), np.int64)
+        di = np.zeros(max(nwi, 1), np.int64)
+        dw = np.zeros(max(nwi, 1), np.float64)
+        L.drnh_grid_rec_concat(h, p, _ptr(rc, C.c_int64))
+        L.drnh_grid_donor_index(h, p, _ptr(di, C.c_int64))
+        L.drnh_grid_donor_weight(h, p, _ptr(dw, C.c_double))
+        return dict(receivers=recv[:nrec], donors=donors, rec_concat=rc[:nrc], donor_index=di[:nwi],
+                    donor_weight=dw[:nwi], no_donor=int(L.drnh_grid_num_no_donor(h, p)))
+
+    def levelset_spheres(self, spheres, n_inner=3, n_outer=0, min_inner_rel=0.1):
+        s = np.ascontiguousarray(spheres, dtype=np.float64).reshape(-1)
+        self._chk(self.L.drnh_grid_levelset_spheres(self.h, s.size // 4, _ptr(s, C.c_double), n_inner, n_outer,
+                                                    min_inner_rel), "drnh_grid_levelset_spheres")
+
+    def levelset_values(self, p, n):
+        out = np.zeros(n, np.float64)
+        self._chk(self.L.drnh_grid_levelset_values(self.h, p, _ptr(out, C.c_double)), "levelset_values")
+        return out
+
+    def ls_cells(self, p, layer, outer=False):
+        n = self.L.drnh_grid_ls_num_cells(self.h, p, layer, int(outer))
+        cells = np.zeros(max(n, 1), np.int64)
+        g = np.zeros((max(n, 1), 4), np.float64)
+        donors = np.zeros((max(n, 1), 8), np.int64)
+        w = np.zeros((max(n, 1), 8), np.float64)
+        self._chk(self.L.drnh_grid_ls_cells(self.h, p, layer, int(outer), _ptr(cells, C.c_int64), _ptr(g, C.c_double),
+                                            _ptr(donors, C.c_int64), _ptr(w, C.c_double)), "ls_cells")
+        return dict(cells=cells[:n], g=g[:n], donors=donors[:n], weights=w[:n])
+
+    def min_ch_length(self):
+        return self.L.drnh_grid_min_ch_length(self.h)
+
+    # ---- hand-over to a device context ------------------------------------
+    def to_device(self, ctx: hip.Context):
+        self._chk(self.L.drnh_grid_to_device(self.h, ctx.h), "drnh_grid_to_device")
+
+    def upload_exchange(self, ctx: hip.Context):
+        self._chk(self.L.drnh_grid_upload_exchange(self.h, ctx.h), "drnh_grid_upload_exchange")
+
+    def upload_levelset(self, ctx: hip.Context):
+        self._chk(self.L.drnh_grid_upload_levelset(self.h, ctx.h), "drnh_grid_upload_levelset")

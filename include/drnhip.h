@@ -173,8 +173,9 @@ int drn_diagnostics(drn_ctx* ctx, int field, double dt, double* out);
 int drn_autotune(drn_ctx* ctx, int* choice);
 /* Select a tile shape / work-list order explicitly (see DRN_TILE_*). */
 int drn_set_tiling(drn_ctx* ctx, int choice);
-/* Average device time (ms) of the last timed kernel class, measured with HIP
- * events on the context stream: which = 0 stage kernel, 1 exchange, 2 level-set fill. */
+/* Average device time (ms) of the launches of one kernel class recorded since
+ * the previous call, measured with HIP events on the context stream while
+ * timing is enabled: which = 0 stage, 1 exchange, 2 level-set fill, 3 field copy. */
 int drn_timing(drn_ctx* ctx, int which, double* avg_ms, int64_t* count);
 int drn_timing_enable(drn_ctx* ctx, int on);
 int drn_sync(drn_ctx* ctx);

@@ -1,0 +1,85 @@
+/*
+ * drnum_host.h -- C ABI of libdrnum.so, the host side of the drop-in: patch
+ * grid, grid-file reader, donor tables and level-set tables (FP64 geometry),
+ * and the hand-over of both table sets to a libdrnhip context.
+ *
+ * The reference builds these on the host in C++ (PatchGrid::readGrid,
+ * patchgrid.cpp:479-561; PatchGrid::computeDependencies, patchgrid.cpp:119-259;
+ * LevelSetObject::update/extractBCellLayers, levelsetobject.cpp:44-388) and its
+ * GPU iterator copies them to the device (GPU_Patch, gpu_patch.h:43-173).  This
+ * C ABI is what a non-C++ caller (the ctypes tests and benchmark) binds; C++
+ * callers use drnum_amd/host/grid.h and iterators.h directly.
+ *
+ * Return codes: 0 success, negative on error (message via drnh_last_error).
+ */
+#ifndef DRNUM_HOST_H
+#define DRNUM_HOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "drnhip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct drnh_grid drnh_grid;
+
+/* geometry precision: 1 = FP64 (product), 0 = FP32 (the reference's default `real`) */
+drnh_grid* drnh_grid_create(int fp64_geometry);
+void drnh_grid_destroy(drnh_grid* g);
+const char* drnh_last_error(const drnh_grid* g);
+
+/* CartesianPatch set-up (cartesianpatch.cpp:98-161): cells per axis including
+ * seek layers, seek layers per face, origin and lengths; returns the patch id. */
+int drnh_grid_add_patch(drnh_grid* g, int ni, int nj, int nk, const int32_t* seek6, const double* origin3,
+                        const double* length3);
+/* PatchGrid::readGrid (patchgrid.cpp:479-561). */
+int drnh_grid_read(drnh_grid* g, const char* path, double scale);
+int drnh_grid_num_patches(const drnh_grid* g);
+/* out_i[9] = ni, nj, nk, seek[6]; out_d[9] = origin[3], length[3], dx, dy, dz */
+int drnh_grid_patch_info(const drnh_grid* g, int p, int32_t* out_i, double* out_d);
+/* face functors of a patch (DRN_BC_* and DRN_BC_NPAR parameters per face) */
+int drnh_grid_set_faces(drnh_grid* g, int p, const int32_t* bc6, const double* par6x16);
+
+/* PatchGrid::computeDependencies; period[3] > 0 adds periodic images per axis
+ * (not in the reference). */
+int drnh_grid_compute_dependencies(drnh_grid* g, const double* period3, int threads);
+int64_t drnh_grid_num_receivers(const drnh_grid* g, int p);
+int64_t drnh_grid_num_no_donor(const drnh_grid* g, int p);
+int drnh_grid_receivers(const drnh_grid* g, int p, int64_t* out);
+int drnh_grid_num_donors(const drnh_grid* g, int p);
+/* out[5] = donor patch, receiver cells, stride, receiver start, index/weight start */
+int drnh_grid_donor_info(const drnh_grid* g, int p, int d, int64_t* out);
+/* which = 0: concatenated receiver list length, 1: index/weight list length */
+int64_t drnh_grid_num_concat(const drnh_grid* g, int p, int which);
+int drnh_grid_rec_concat(const drnh_grid* g, int p, int64_t* out);
+int drnh_grid_donor_index(const drnh_grid* g, int p, int64_t* out);
+int drnh_grid_donor_weight(const drnh_grid* g, int p, double* out);
+
+/* Level set min_s(|x - c_s| - r_s) of spheres (xc, yc, zc, r) x n (spherelevelset.cpp,
+ * combilevelsetor.cpp); layers as LevelSetObject(.., n_inner, n_outer, min_inner_rel, -1). */
+int drnh_grid_levelset_spheres(drnh_grid* g, int n_spheres, const double* spheres, int n_inner, int n_outer,
+                               double min_inner_rel);
+int drnh_grid_levelset_values(const drnh_grid* g, int p, double* out);
+int64_t drnh_grid_ls_num_cells(const drnh_grid* g, int p, int layer, int outer);
+/* per cell: cell; gvals[4] = G, gx, gy, gz; donors[8]; weights[8] */
+int drnh_grid_ls_cells(const drnh_grid* g, int p, int layer, int outer, int64_t* cells, double* gvals,
+                       int64_t* donors, double* weights);
+
+/* PatchGrid::computeMinChLength (patchgrid.cpp:659-671) */
+double drnh_grid_min_ch_length(const drnh_grid* g);
+
+/* Add every patch of the grid to a libdrnhip context (in grid order, with the
+ * face functors set by drnh_grid_set_faces) and lay the context out. */
+int drnh_grid_to_device(drnh_grid* g, drn_ctx* ctx);
+/* Hand the donor tables (after compute_dependencies) to the context and commit. */
+int drnh_grid_upload_exchange(drnh_grid* g, drn_ctx* ctx);
+/* Hand the inner level-set layers (flat order patch, layer, list) to the context and commit. */
+int drnh_grid_upload_levelset(drnh_grid* g, drn_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
