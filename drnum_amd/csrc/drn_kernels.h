@@ -186,6 +186,221 @@ __global__ __launch_bounds__(BS) void k_stage_cell(const PatchDev* __restrict__ 
   for (int v = 0; v < 5; ++v) dst[v * vs + c] = base[v * vs + c] + pf * res[v];
 }
 
+// ---------------------------------------------------------------------------
+// marching stage kernel
+// ---------------------------------------------------------------------------
+
+// MUSCL state at a face from the three cells on one side (upwind2.h:29-52):
+// a = cell next to the face, m = next cell outwards (valid when hm), b = cell across
+template <typename T, int RECON, int LIM, bool FAST>
+__device__ __forceinline__ T recon1(T a, T m, T b, bool hm) {
+  if constexpr (RECON == RECON_UPWIND2) {
+    if (hm) return a + T(0.5) * limited<T, LIM, FAST>(a - m, b - a);
+  }
+  return a;
+}
+
+// AUSM+ flux with a run-time normal direction (1 = y, 2 = z) for the LDS tasks
+template <typename T, bool FAST>
+__device__ __forceinline__ void ausm_flux_rt(const T* vl, const T* vr, int dir, T A, T* f) {
+  if (dir == 1)
+    ausm_flux<T, FAST, 1>(vl, vr, A, f);
+  else
+    ausm_flux<T, FAST, 2>(vl, vr, A, f);
+}
+
+// A workgroup of BS threads owns BS consecutive cells c0.. of the (j,k) plane of
+// one patch (plane index c = j*NK + k) and marches over the planes [i0, i1).
+// Per plane i:
+//   x faces: each thread keeps U(i-1..i+2) of its column in registers and
+//            computes F(i+1/2); F(i-1/2) is carried from the previous plane.
+//   y, z faces: the plane with a 2*NK halo is staged in LDS; the faces needed
+//            by the chunk (BS + NK y faces, BS + rows + 1 z faces) are computed
+//            once each and written to LDS.
+//   update:  dst = base + factor/dV * (Fx- - Fx+ + Fy- - Fy+ + Fz- - Fz+).
+// Patch-face functors are evaluated as the fluxes of the boundary faces.
+// Requires NI, NJ, NK >= 3 (the host routes thinner patches to k_stage_cell).
+template <typename T, int RECON, int LIM, bool FAST, int BS>
+__global__ __launch_bounds__(BS) void k_stage_march(const PatchDev* __restrict__ pds,
+                                                    const TileMarch* __restrict__ tiles, const T* __restrict__ src,
+                                                    const T* base, T* dst, uint64_t vs, T factor, int lds_cells,
+                                                    int lds_faces) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char drn_smem[];
+  T* sU = reinterpret_cast<T*>(drn_smem);  // [5][lds_cells]
+  T* sF = sU + 5 * lds_cells;               // [5][lds_faces]
+  const TileMarch t = tiles[blockIdx.x];
+  const PatchDev& P = pds[t.patch];
+  const int ni = P.ni, nj = P.nj, nk = P.nk, njk = P.njk;
+  const int c0 = t.c0;
+  const int cnt = min(BS, njk - c0);
+  const int tid = threadIdx.x;
+  const bool own = tid < cnt;
+  const int c = c0 + tid;
+  const int j = own ? c / nk : 0;
+  const int k = own ? c - j * nk : 0;
+  const int h0 = max(0, c0 - 2 * nk), h1 = min(njk, c0 + cnt + 2 * nk), hn = h1 - h0;
+  const int nY = cnt + nk;
+  const int jfirst = c0 / nk, jlast = (c0 + cnt - 1) / nk;
+  const int zf0 = c0 + jfirst;
+  const int nZ = cnt + 1 + (jlast - jfirst);
+  const T dx = T(P.dx), dy = T(P.dy), dz = T(P.dz);
+  const T Ax = dy * dz, Ay = dx * dz, Az = dx * dy;
+  const T pf = factor / (dx * dy * dz);
+  const uint64_t pb = (uint64_t)P.off;
+  const uint64_t col = pb + (uint64_t)c;
+  const uint64_t sx = (uint64_t)njk;
+  const int i0 = t.i0, i1 = t.i1;
+
+  // register window of the own column: W[q] = U(i - 1 + q)
+  T W[4][5];
+  auto loadW = [&](int q, int ip) {
+    if (own && ip >= 0 && ip < ni) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) W[q][v] = src[v * vs + col + (uint64_t)ip * sx];
+    }
+  };
+  // x face between planes ia-1 and ia of the own column, window holds ia-2..ia+1 in W[0..3]
+  auto xface = [&](int ia, T* f) {
+    zero5(f);
+    if (ia == 0) {
+      wall_flux<T>(P.bc[0], P.par[0], W[2], 0, false, T(0.5) * dy + T(j) * dy, T(0.5) * dz + T(k) * dz, j, k, Ax, f);
+    } else if (ia == ni) {
+      wall_flux<T>(P.bc[1], P.par[1], W[1], 0, true, T(0.5) * dy + T(j) * dy, T(0.5) * dz + T(k) * dz, j, k, Ax, f);
+    } else {
+      const bool hm = ia >= 2, hp = ia + 1 < ni;
+      T vl[5], vr[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        vl[v] = recon1<T, RECON, LIM, FAST>(W[1][v], W[0][v], W[2][v], hm);
+        vr[v] = recon1<T, RECON, LIM, FAST>(W[2][v], W[3][v], W[1][v], hp);
+      }
+      ausm_flux<T, FAST, 0>(vl, vr, Ax, f);
+    }
+  };
+
+  T fm[5], fp[5];
+  loadW(0, i0 - 2);
+  loadW(1, i0 - 1);
+  loadW(2, i0);
+  loadW(3, i0 + 1);
+  if (own) xface(i0, fm);
+  // shift to W = U(i0-1 .. i0+2)
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    W[0][v] = W[1][v];
+    W[1][v] = W[2][v];
+    W[2][v] = W[3][v];
+  }
+  loadW(3, i0 + 2);
+
+  for (int i = i0; i < i1; ++i) {
+    __syncthreads();  // previous plane's LDS readers are done
+    const uint64_t prow = pb + (uint64_t)i * sx + (uint64_t)h0;
+    for (int q = tid; q < hn; q += BS) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) sU[v * lds_cells + q] = src[v * vs + prow + q];
+    }
+    __syncthreads();
+    const T xi = T(0.5) * dx + T(i) * dx;
+    // y and z faces of the chunk
+    for (int tk = tid; tk < nY + nZ; tk += BS) {
+      T f[5];
+      zero5(f);
+      int dir, cm1, cp1, face_wall = -1;  // face_wall: functor face id, cell in cp1 (minus walls) / cm1 (plus)
+      bool hm = false, hp = false;
+      int cm2 = 0, cp2 = 0;
+      int ja = 0, kb = 0;
+      if (tk < nY) {
+        dir = 1;
+        const int fy = c0 + tk;
+        const int jf = fy / nk;
+        cp1 = fy;
+        cm1 = fy - nk;
+        if (jf == 0) {
+          face_wall = 2;
+          ja = jf;
+          kb = fy - jf * nk;
+        } else if (jf == nj) {
+          face_wall = 3;
+          ja = jf - 1;
+          kb = fy - jf * nk;
+        } else {
+          hm = jf >= 2;
+          hp = jf + 1 < nj;
+          cm2 = cm1 - nk;
+          cp2 = cp1 + nk;
+        }
+      } else {
+        dir = 2;
+        const int zf = zf0 + (tk - nY);
+        const int jz = zf / (nk + 1);
+        const int kz = zf - jz * (nk + 1);
+        cm1 = jz * nk + kz - 1;
+        cp1 = cm1 + 1;
+        if (kz == 0) {
+          face_wall = 4;
+          ja = jz;
+          kb = 0;
+        } else if (kz == nk) {
+          face_wall = 5;
+          ja = jz;
+          kb = nk - 1;
+        } else {
+          hm = kz >= 2;
+          hp = kz + 1 < nk;
+          cm2 = cm1 - 1;
+          cp2 = cp1 + 1;
+        }
+      }
+      if (face_wall >= 0) {
+        const bool plus = (face_wall & 1) != 0;
+        const int cell = (plus ? cm1 : cp1) - h0;
+        T var[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) var[v] = sU[v * lds_cells + cell];
+        if (dir == 1)
+          wall_flux<T>(P.bc[face_wall], P.par[face_wall], var, 1, plus, xi, T(0.5) * dz + T(kb) * dz, i, kb, Ay, f);
+        else
+          wall_flux<T>(P.bc[face_wall], P.par[face_wall], var, 2, plus, xi, T(0.5) * dy + T(ja) * dy, i, ja, Az, f);
+      } else {
+        T vl[5], vr[5];
+        const int qm1 = cm1 - h0, qp1 = cp1 - h0, qm2 = hm ? cm2 - h0 : qm1, qp2 = hp ? cp2 - h0 : qp1;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          const T a = sU[v * lds_cells + qm1], b = sU[v * lds_cells + qp1];
+          const T m = sU[v * lds_cells + qm2], q = sU[v * lds_cells + qp2];
+          vl[v] = recon1<T, RECON, LIM, FAST>(a, m, b, hm);
+          vr[v] = recon1<T, RECON, LIM, FAST>(b, q, a, hp);
+        }
+        ausm_flux_rt<T, FAST>(vl, vr, dir, dir == 1 ? Ay : Az, f);
+      }
+#pragma unroll
+      for (int v = 0; v < 5; ++v) sF[v * lds_faces + tk] = f[v];
+    }
+    // x face i+1/2 of the own column
+    if (own) xface(i + 1, fp);
+    __syncthreads();
+    if (own) {
+      const int qy = c - c0, qz = nY + (c + j) - zf0;
+      const uint64_t g = col + (uint64_t)i * sx;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const T r = (fm[v] - fp[v]) + (sF[v * lds_faces + qy] - sF[v * lds_faces + qy + nk]) +
+                    (sF[v * lds_faces + qz] - sF[v * lds_faces + qz + 1]);
+        dst[v * vs + g] = base[v * vs + g] + pf * r;
+      }
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        fm[v] = fp[v];
+        W[0][v] = W[1][v];
+        W[1][v] = W[2][v];
+        W[2][v] = W[3][v];
+      }
+    }
+    loadW(3, i + 3);
+  }
+}
+
 // plain vectorised copy (PatchIterator::copyField); also the STREAM-style
 // reference kernel whose bytes are known exactly (16 B per lane per iteration)
 __global__ __launch_bounds__(256) void k_copy(uint64_t n16, const uint4* __restrict__ src, uint4* __restrict__ dst) {

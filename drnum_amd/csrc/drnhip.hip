@@ -58,8 +58,18 @@ struct drn_ctx {
   TileCell* d_tiles = nullptr;
   double* d_part = nullptr;  // reduction partials (5 per tile)
   double* d_out = nullptr;
-  int block = 256;
   int tiling = 0;
+  int arith = -1;  // DRN_ARITH_*; -1 = precision default
+  // marching work-lists: natural order (patch-contiguous) and launch order
+  std::vector<TileMarch> mt_nat;
+  std::vector<int> mt_first;  // per patch, size np+1 (into mt_nat)
+  TileMarch* d_mt_nat = nullptr;
+  TileMarch* d_mt_launch = nullptr;
+  int n_mt = 0;
+  int lds_cells = 0, lds_faces = 0;
+  std::vector<int> thin;      // patches routed to the cell kernel
+  std::vector<TileCell> thin_tiles;
+  TileCell* d_thin_tiles = nullptr;
 
   // exchange
   std::map<int, ExchangeTables> xt;
@@ -147,52 +157,176 @@ static void timer_end(drn_ctx* ctx, hipEvent_t stop) {
 // launches
 // ---------------------------------------------------------------------------
 
-template <typename T, int RECON, int LIM>
-static void launch_stage_t(drn_ctx* ctx, const TileCell* tiles, int ntiles, const T* src, const T* base, T* dst,
+constexpr int MARCH_BS = 256;
+
+// tiling choices: (i-range length, work-list order); the last entry is the
+// cell-per-thread kernel (every interior face evaluated twice)
+struct TilingChoice {
+  int irange;  // planes per work item, 0 = whole patch
+  int xcd;     // 1 = segment the work list so that neighbouring chunks share an XCD
+  int cell;    // 1 = k_stage_cell
+};
+static const TilingChoice kTilings[] = {{0, 0, 0}, {0, 1, 0}, {32, 0, 0}, {32, 1, 0}, {16, 0, 0},
+                                        {16, 1, 0}, {8, 0, 0},  {8, 1, 0},  {0, 0, 1}};
+static const int kNumTilings = sizeof(kTilings) / sizeof(kTilings[0]);
+
+template <typename T, int RECON, int LIM, bool FAST>
+static void launch_march_t(drn_ctx* ctx, const TileMarch* tiles, int ntiles, const T* src, const T* base, T* dst,
                            T factor) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, false, 256>), dim3(ntiles), dim3(256), 0, ctx->stream, ctx->d_pd,
+  const size_t lds = (size_t)5 * (ctx->lds_cells + ctx->lds_faces) * sizeof(T);
+  hipLaunchKernelGGL((k_stage_march<T, RECON, LIM, FAST, MARCH_BS>), dim3(ntiles), dim3(MARCH_BS), lds, ctx->stream,
+                     ctx->d_pd, tiles, src, base, dst, ctx->vstride, factor, ctx->lds_cells, ctx->lds_faces);
+}
+
+template <typename T, int RECON, int LIM, bool FAST>
+static void launch_cell_t(drn_ctx* ctx, const TileCell* tiles, int ntiles, const T* src, const T* base, T* dst,
+                          T factor) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, FAST, 256>), dim3(ntiles), dim3(256), 0, ctx->stream, ctx->d_pd,
                      tiles, src, base, dst, ctx->vstride, factor);
 }
 
+// one launch of either kernel over a tile range (march tiles or cell tiles)
+template <typename T, int RECON, int LIM, bool FAST>
+static void launch_range(drn_ctx* ctx, bool cell, const void* tiles, int ntiles, const T* src, const T* base, T* dst,
+                         T factor) {
+  if (cell)
+    launch_cell_t<T, RECON, LIM, FAST>(ctx, (const TileCell*)tiles, ntiles, src, base, dst, factor);
+  else
+    launch_march_t<T, RECON, LIM, FAST>(ctx, (const TileMarch*)tiles, ntiles, src, base, dst, factor);
+}
+
+template <typename T, bool FAST>
+static void launch_dispatch(drn_ctx* ctx, bool cell, const void* tiles, int ntiles, const T* src, const T* base, T* dst,
+                            T factor) {
+  if (ctx->recon == DRN_RECON_UPWIND1)
+    launch_range<T, RECON_UPWIND1, LIM_SECONDORDER, FAST>(ctx, cell, tiles, ntiles, src, base, dst, factor);
+  else if (ctx->lim == DRN_LIM_VANALBADA)
+    launch_range<T, RECON_UPWIND2, LIM_VANALBADA, FAST>(ctx, cell, tiles, ntiles, src, base, dst, factor);
+  else if (ctx->lim == DRN_LIM_MINMOD)
+    launch_range<T, RECON_UPWIND2, LIM_MINMOD, FAST>(ctx, cell, tiles, ntiles, src, base, dst, factor);
+  else
+    launch_range<T, RECON_UPWIND2, LIM_SECONDORDER, FAST>(ctx, cell, tiles, ntiles, src, base, dst, factor);
+}
+
+static bool fast_math(const drn_ctx* ctx) {
+  return ctx->arith < 0 ? ctx->prec == DRN_F32 : ctx->arith == DRN_ARITH_FAST;
+}
+
 template <typename T>
-static void launch_stage_dispatch(drn_ctx* ctx, const TileCell* tiles, int ntiles, const T* src, const T* base, T* dst,
-                                  T factor) {
-  if (ctx->recon == DRN_RECON_UPWIND1) {
-    launch_stage_t<T, RECON_UPWIND1, LIM_SECONDORDER>(ctx, tiles, ntiles, src, base, dst, factor);
-  } else if (ctx->lim == DRN_LIM_VANALBADA) {
-    launch_stage_t<T, RECON_UPWIND2, LIM_VANALBADA>(ctx, tiles, ntiles, src, base, dst, factor);
-  } else if (ctx->lim == DRN_LIM_MINMOD) {
-    launch_stage_t<T, RECON_UPWIND2, LIM_MINMOD>(ctx, tiles, ntiles, src, base, dst, factor);
+static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int launch) {
+  const T* src = (const T*)ctx->buf[psrc];
+  const T* base = (const T*)ctx->buf[pbase];
+  T* dst = (T*)ctx->buf[pdst];
+  const bool fast = fast_math(ctx);
+  const bool cell = kTilings[ctx->tiling].cell != 0;
+  auto go = [&](bool c, const void* tl, int n) {
+    if (fast)
+      launch_dispatch<T, true>(ctx, c, tl, n, src, base, dst, factor);
+    else
+      launch_dispatch<T, false>(ctx, c, tl, n, src, base, dst, factor);
+  };
+  if (launch == DRN_LAUNCH_BATCHED) {
+    if (cell) {
+      go(true, ctx->d_tiles, (int)ctx->tiles.size());
+    } else {
+      go(false, ctx->d_mt_launch, ctx->n_mt);
+      go(true, ctx->d_thin_tiles, (int)ctx->thin_tiles.size());
+    }
   } else {
-    launch_stage_t<T, RECON_UPWIND2, LIM_SECONDORDER>(ctx, tiles, ntiles, src, base, dst, factor);
+    const int np = (int)ctx->descs.size();
+    for (int p = 0; p < np; ++p) {
+      const bool thin = std::find(ctx->thin.begin(), ctx->thin.end(), p) != ctx->thin.end();
+      if (cell || thin) {
+        const int t0 = ctx->tile_first[p];
+        go(true, ctx->d_tiles + t0, ctx->tile_first[p + 1] - t0);
+      } else {
+        const int t0 = ctx->mt_first[p];
+        go(false, ctx->d_mt_nat + t0, ctx->mt_first[p + 1] - t0);
+      }
+    }
   }
 }
 
 static int do_stage(drn_ctx* ctx, int psrc, int pbase, int pdst, double factor, int launch) {
   hipEvent_t stop;
   timer_begin(ctx, 0, &stop);
-  if (launch == DRN_LAUNCH_BATCHED) {
-    if (ctx->prec == DRN_F32)
-      launch_stage_dispatch<float>(ctx, ctx->d_tiles, (int)ctx->tiles.size(), (const float*)ctx->buf[psrc],
-                                   (const float*)ctx->buf[pbase], (float*)ctx->buf[pdst], (float)factor);
-    else
-      launch_stage_dispatch<double>(ctx, ctx->d_tiles, (int)ctx->tiles.size(), (const double*)ctx->buf[psrc],
-                                    (const double*)ctx->buf[pbase], (double*)ctx->buf[pdst], factor);
-  } else {
-    const int np = (int)ctx->descs.size();
-    for (int p = 0; p < np; ++p) {
-      const int t0 = ctx->tile_first[p], nt = ctx->tile_first[p + 1] - t0;
-      if (ctx->prec == DRN_F32)
-        launch_stage_dispatch<float>(ctx, ctx->d_tiles + t0, nt, (const float*)ctx->buf[psrc],
-                                     (const float*)ctx->buf[pbase], (float*)ctx->buf[pdst], (float)factor);
-      else
-        launch_stage_dispatch<double>(ctx, ctx->d_tiles + t0, nt, (const double*)ctx->buf[psrc],
-                                      (const double*)ctx->buf[pbase], (double*)ctx->buf[pdst], factor);
-    }
-  }
+  if (ctx->prec == DRN_F32)
+    stage_t<float>(ctx, psrc, pbase, pdst, (float)factor, launch);
+  else
+    stage_t<double>(ctx, psrc, pbase, pdst, factor, launch);
   timer_end(ctx, stop);
   HIPCHK(ctx, hipGetLastError());
+  return DRN_OK;
+}
+
+// build the marching work-lists of the current tiling choice
+static int build_march(drn_ctx* ctx) {
+  const TilingChoice tc = kTilings[ctx->tiling];
+  const int np = (int)ctx->descs.size();
+  ctx->mt_nat.clear();
+  ctx->mt_first.assign(np + 1, 0);
+  ctx->thin.clear();
+  ctx->thin_tiles.clear();
+  int lc = 0, lf = 0;
+  for (int p = 0; p < np; ++p) {
+    const auto& d = ctx->descs[p];
+    ctx->mt_first[p] = (int)ctx->mt_nat.size();
+    const int njk = d.nj * d.nk;
+    int plc = 0, plf = 0;
+    for (int c0 = 0; c0 < njk; c0 += MARCH_BS) {
+      const int cnt = std::min(MARCH_BS, njk - c0);
+      plc = std::max(plc, std::min(njk, c0 + cnt + 2 * d.nk) - std::max(0, c0 - 2 * d.nk));
+      const int rows = (c0 + cnt - 1) / d.nk - c0 / d.nk;
+      plf = std::max(plf, (cnt + d.nk) + (cnt + 1 + rows));
+    }
+    // thin patches (the reference skips the faces of an axis with <= 2 cells) and
+    // patches whose staged plane would not fit the LDS budget use the cell kernel
+    const size_t need = (size_t)5 * (plc + plf) * (ctx->prec == DRN_F32 ? 4 : 8);
+    if (d.ni < 3 || d.nj < 3 || d.nk < 3 || need > 96 * 1024) {
+      ctx->thin.push_back(p);
+      for (int t = ctx->tile_first[p]; t < ctx->tile_first[p + 1]; ++t) ctx->thin_tiles.push_back(ctx->tiles[t]);
+      continue;
+    }
+    lc = std::max(lc, plc);
+    lf = std::max(lf, plf);
+    const int L = tc.irange > 0 ? tc.irange : d.ni;
+    // i ranges outermost within a patch so that the chunks of one plane are adjacent
+    for (int i0 = 0; i0 < d.ni; i0 += L)
+      for (int c0 = 0; c0 < njk; c0 += MARCH_BS) ctx->mt_nat.push_back(TileMarch{p, i0, std::min(d.ni, i0 + L), c0});
+  }
+  ctx->mt_first[np] = (int)ctx->mt_nat.size();
+  ctx->lds_cells = lc;
+  ctx->lds_faces = lf;
+  ctx->n_mt = (int)ctx->mt_nat.size();
+  std::vector<TileMarch> launch = ctx->mt_nat;
+  if (tc.xcd && ctx->n_mt > 0) {
+    // workgroup b runs on XCD b % 8: give each XCD one contiguous segment of the natural list
+    const int n = ctx->n_mt, seg = (n + 7) / 8;
+    std::vector<TileMarch> l;
+    l.reserve(n);
+    for (int r = 0; r < seg; ++r)
+      for (int x = 0; x < 8; ++x)
+        if (x * seg + r < n) l.push_back(ctx->mt_nat[x * seg + r]);
+    launch.swap(l);
+  }
+  dfree(ctx->d_mt_nat);
+  dfree(ctx->d_mt_launch);
+  dfree(ctx->d_thin_tiles);
+  ctx->d_mt_nat = ctx->d_mt_launch = nullptr;
+  ctx->d_thin_tiles = nullptr;
+  HIPCHK(ctx, dalloc(&ctx->d_mt_nat, ctx->mt_nat.size()));
+  HIPCHK(ctx, dalloc(&ctx->d_mt_launch, launch.size()));
+  HIPCHK(ctx, dalloc(&ctx->d_thin_tiles, ctx->thin_tiles.size()));
+  if (!ctx->mt_nat.empty()) {
+    HIPCHK(ctx, hipMemcpy(ctx->d_mt_nat, ctx->mt_nat.data(), ctx->mt_nat.size() * sizeof(TileMarch),
+                          hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->d_mt_launch, launch.data(), launch.size() * sizeof(TileMarch), hipMemcpyHostToDevice));
+  }
+  if (!ctx->thin_tiles.empty())
+    HIPCHK(ctx, hipMemcpy(ctx->d_thin_tiles, ctx->thin_tiles.data(), ctx->thin_tiles.size() * sizeof(TileCell),
+                          hipMemcpyHostToDevice));
   return DRN_OK;
 }
 
@@ -343,6 +477,9 @@ void drn_ctx_destroy(drn_ctx* ctx) {
   for (int b = 0; b < NBUF; ++b) dfree(ctx->buf[b]);
   dfree(ctx->d_pd);
   dfree(ctx->d_tiles);
+  dfree(ctx->d_mt_nat);
+  dfree(ctx->d_mt_launch);
+  dfree(ctx->d_thin_tiles);
   dfree(ctx->d_part);
   dfree(ctx->d_out);
   for (auto& s : ctx->timers) {
@@ -444,6 +581,9 @@ int drn_layout(drn_ctx* ctx) {
   HIPCHK(ctx, hipMemcpy(ctx->d_tiles, ctx->tiles.data(), ctx->tiles.size() * sizeof(TileCell), hipMemcpyHostToDevice));
   HIPCHK(ctx, dalloc(&ctx->d_part, ctx->tiles.size() * 5));
   HIPCHK(ctx, dalloc(&ctx->d_out, 8));
+  ctx->tiling = 1;
+  int rc = build_march(ctx);
+  if (rc) return rc;
   ctx->laid_out = true;
   return DRN_OK;
 }
@@ -849,18 +989,78 @@ int drn_diagnostics(drn_ctx* ctx, int field, double dt, double* out) {
   return ctx->prec == DRN_F32 ? diag_t(ctx, (const float*)U, dt, out) : diag_t(ctx, (const double*)U, dt, out);
 }
 
-int drn_autotune(drn_ctx* ctx, int* choice) {
+int drn_set_tiling(drn_ctx* ctx, int choice) {
   if (!ctx) return DRN_E_ARG;
-  if (choice) *choice = ctx->tiling;
+  if (choice < 0 || choice >= kNumTilings) return fail(ctx, DRN_E_ARG, "unknown tiling choice");
+  if (!ctx->laid_out) return fail(ctx, DRN_E_STATE, "drn_layout not called");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  const int old = ctx->tiling;
+  ctx->tiling = choice;
+  int rc = build_march(ctx);
+  if (rc) {
+    ctx->tiling = old;
+    build_march(ctx);
+    return rc;
+  }
+  if (ctx->gexec) {
+    (void)hipGraphExecDestroy(ctx->gexec);
+    ctx->gexec = nullptr;
+  }
   return DRN_OK;
 }
 
-int drn_set_tiling(drn_ctx* ctx, int choice) {
+int drn_num_tilings(void) { return kNumTilings; }
+
+// Times every tiling choice on the batched stage (field 0 -> field 2 with field 1
+// as base, which leaves the state untouched) and keeps the fastest.
+int drn_autotune(drn_ctx* ctx, int* choice) {
   if (!ctx) return DRN_E_ARG;
-  if (choice != 0) return fail(ctx, DRN_E_ARG, "unknown tiling");
-  ctx->tiling = choice;
+  if (!ctx->laid_out) return fail(ctx, DRN_E_STATE, "drn_layout not called");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const bool was_timing = ctx->timing;
+  ctx->timing = false;
+  hipEvent_t e0, e1;
+  HIPCHK(ctx, hipEventCreate(&e0));
+  HIPCHK(ctx, hipEventCreate(&e1));
+  int best = ctx->tiling;
+  float best_ms = 1e30f;
+  for (int c = 0; c < kNumTilings; ++c) {
+    if (drn_set_tiling(ctx, c) != DRN_OK) continue;
+    int rc = do_stage(ctx, ctx->map[0], ctx->map[1], ctx->map[2], 0.0, DRN_LAUNCH_BATCHED);  // warm-up
+    if (rc) break;
+    HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
+    const int reps = 5;
+    for (int r = 0; r < reps; ++r) do_stage(ctx, ctx->map[0], ctx->map[1], ctx->map[2], 0.0, DRN_LAUNCH_BATCHED);
+    HIPCHK(ctx, hipEventRecord(e1, ctx->stream));
+    HIPCHK(ctx, hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ctx->timing = was_timing;
+  int rc = drn_set_tiling(ctx, best);
+  if (choice) *choice = best;
+  return rc;
+}
+
+int drn_set_arith(drn_ctx* ctx, int mode) {
+  if (!ctx) return DRN_E_ARG;
+  if (mode != DRN_ARITH_IEEE && mode != DRN_ARITH_FAST && mode != -1) return fail(ctx, DRN_E_ARG, "unknown arithmetic mode");
+  ctx->arith = mode;
+  if (ctx->gexec) {
+    (void)hipGraphExecDestroy(ctx->gexec);
+    ctx->gexec = nullptr;
+  }
   return DRN_OK;
 }
+
+int drn_get_tiling(const drn_ctx* ctx) { return ctx ? ctx->tiling : DRN_E_ARG; }
 
 int drn_timing_enable(drn_ctx* ctx, int on) {
   if (!ctx) return DRN_E_ARG;

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libdrnhip.so")
 
 DRN_F32, DRN_F64 = 0, 1
 DRN_LAUNCH_BATCHED, DRN_LAUNCH_PER_PATCH = 0, 1
+DRN_ARITH_IEEE, DRN_ARITH_FAST = 0, 1
 NPAR = 16
 
 
@@ -58,6 +59,9 @@ _SIGS = {
     "drn_diagnostics": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double)]),
     "drn_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "drn_set_tiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "drn_get_tiling": (C.c_int, [C.c_void_p]),
+    "drn_num_tilings": (C.c_int, []),
+    "drn_set_arith": (C.c_int, [C.c_void_p, C.c_int]),
     "drn_timing": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "drn_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "drn_sync": (C.c_int, [C.c_void_p]),
@@ -225,6 +229,15 @@ class Context:
 
     def set_tiling(self, choice):
         self._chk(self.L.drn_set_tiling(self.h, choice), "drn_set_tiling")
+
+    def get_tiling(self):
+        return self.L.drn_get_tiling(self.h)
+
+    def num_tilings(self):
+        return self.L.drn_num_tilings()
+
+    def set_arith(self, mode):
+        self._chk(self.L.drn_set_arith(self.h, mode), "drn_set_arith")
 
     def timing_enable(self, on=True):
         self._chk(self.L.drn_timing_enable(self.h, int(on)), "drn_timing_enable")

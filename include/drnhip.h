@@ -168,11 +168,22 @@ int drn_core_sums(drn_ctx* ctx, int field, double* out);
 int drn_diagnostics(drn_ctx* ctx, int field, double dt, double* out);
 
 /* ---- tuning, timing, sync ------------------------------------------------ */
-/* Time the candidate tile shapes / work-list orders of the batched stage kernel
- * and keep the fastest (writes the choice to *choice if not NULL). */
-int drn_autotune(drn_ctx* ctx, int* choice);
-/* Select a tile shape / work-list order explicitly (see DRN_TILE_*). */
+/* Stage-kernel work-list choices: the marching kernel with an i-range of the
+ * whole patch / 32 / 16 / 8 planes per work item, each in natural (patch, plane
+ * range, chunk) order or segmented per XCD (workgroup b runs on XCD b % 8, so
+ * each XCD gets one contiguous run of neighbouring chunks and shares their
+ * halos in its L2), and last the cell-per-thread kernel.  Default: 1. */
+int drn_num_tilings(void);
 int drn_set_tiling(drn_ctx* ctx, int choice);
+int drn_get_tiling(const drn_ctx* ctx);
+/* Time every choice on the batched stage of the current state (the state is
+ * not modified) and keep the fastest; writes it to *choice if not NULL. */
+int drn_autotune(drn_ctx* ctx, int* choice);
+/* Arithmetic of the flux kernels: IEEE division and square root, or the fast
+ * form (hardware reciprocal, single-division van Albada, exact 0/1 for the
+ * supersonic pressure split).  Default: FAST for DRN_F32, IEEE for DRN_F64. */
+enum { DRN_ARITH_IEEE = 0, DRN_ARITH_FAST = 1 };
+int drn_set_arith(drn_ctx* ctx, int mode);
 /* Average device time (ms) of the launches of one kernel class recorded since
  * the previous call, measured with HIP events on the context stream while
  * timing is enabled: which = 0 stage, 1 exchange, 2 level-set fill, 3 field copy. */

@@ -158,3 +158,57 @@ def test_core_sums(gpu, dtype):
     # momentum sums cancel to ~1e-6 of the mass: compare against the largest sum
     tol = (1e-12 if dtype == np.float64 else 2e-5) * np.abs(b).max()
     assert np.all(np.abs(a - b) <= tol), (a, b)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_every_tiling(gpu, dtype):
+    """Every stage-kernel work-list choice (marching i-ranges, XCD order, cell kernel)
+    on 2x2x2 patches of 16^3 core cells (planes of 324-400 cells: several chunks)."""
+    w = configs.jet(core=16, blocks=2)
+    og = O.OracleGrid(w, dtype, threads=8)
+    og.init_state()
+    og.compute_dependencies()
+    for s in range(4):
+        og.rk_step()
+    tables = tables_of(og)
+    n = solver.GpuGrid(w, dtype).ctx.num_tilings()
+    for choice in range(n):
+        g2 = solver.GpuGrid(w, dtype)
+        g2.upload_initial()
+        g2.set_tables(tables)
+        g2.ctx.set_tiling(choice)
+        for s in range(4):
+            g2.rk_step()
+        err = rel_linf(g2.state(0), og)
+        print("tiling", choice, np.dtype(dtype).name, err, flush=True)
+        assert err <= TOL[dtype], (choice, err)
+        g2.close()
+
+
+@pytest.mark.parametrize("arith", [0, 1])
+def test_arith_modes_fp32(gpu, arith):
+    """IEEE and fast arithmetic of the FP32 flux kernels against the FP32 oracle."""
+    w = configs.sod(n=16)
+    og, gg = make(w, np.float32)
+    gg.ctx.set_arith(arith)
+    for s in range(10):
+        og.rk_step()
+        gg.rk_step()
+    err = rel_linf(gg.state(0), og)
+    print("arith", arith, err, flush=True)
+    assert err <= TOL[np.float32]
+
+
+def test_autotune_keeps_state(gpu):
+    w = configs.jet(core=8, blocks=2)
+    og, gg = make(w, np.float32)
+    before = gg.state(0)
+    choice = gg.ctx.autotune()
+    assert 0 <= choice < gg.ctx.num_tilings()
+    after = gg.state(0)
+    for a, b in zip(before, after):
+        assert np.array_equal(a, b)
+    for s in range(3):
+        og.rk_step()
+        gg.rk_step()
+    assert rel_linf(gg.state(0), og) <= TOL[np.float32]
