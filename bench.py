@@ -67,6 +67,9 @@ def run_form(w, dtype, steps, warmup, launch, label):
     cells = w.total_cells
     esize = np.dtype(dtype).itemsize
     S = len(w.alpha)
+    # the library's autotuner picks the stage work-list (i-range, XCD order, kernel) for this form
+    tiling = ctx.autotune(launch)
+    log("%s: autotuned tiling %d" % (label, tiling))
     for _ in range(warmup):
         ctx.rk_step(w.dt, w.alpha, launch)
     ctx.sync()
@@ -88,7 +91,7 @@ def run_form(w, dtype, steps, warmup, launch, label):
     if diag["nonfinite"] > 0:
         raise RuntimeError("%s: non-finite values after the timed steps" % label)
     res = dict(wall=wall, ms_per_step=1e3 * wall / steps, mcups=cells * S * steps / wall / 1e6,
-               stage_ms=st_ms, stage_n=st_n, exch_ms=ex_ms, exch_n=ex_n)
+               stage_ms=st_ms, stage_n=st_n, exch_ms=ex_ms, exch_n=ex_n, tiling=tiling)
     # per launch the stage kernel advances all stored cells once
     bytes_step = stage_bytes_per_step(cells, esize, S)
     res["stage_bytes_per_launch"] = bytes_step / S
@@ -192,6 +195,7 @@ def main():
             "cells_core": w.core_cells,
             "stages": len(w.alpha),
             "form": "fp32 batched (one stage launch over all patches)",
+            "tiling": f32["tiling"],
         },
         "roofline": {
             "bound": "hbm",
@@ -205,6 +209,7 @@ def main():
             "value": round(f64["mcups"], 2),
             "ms_per_step": round(f64["ms_per_step"], 4),
             "stage_kernel_gbs": round(f64["stage_gbs"], 1),
+            "tiling": f64["tiling"],
             "fp32_batched_over_fp64_plain": round(f32["mcups"] / f64["mcups"], 3),
         },
         "graph": {

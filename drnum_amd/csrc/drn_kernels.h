@@ -401,6 +401,578 @@ __global__ __launch_bounds__(BS) void k_stage_march(const PatchDev* __restrict__
   }
 }
 
+// ---------------------------------------------------------------------------
+// slope-form marching stage kernel (fast arithmetic, van Albada / minmod)
+// ---------------------------------------------------------------------------
+
+// Limited MUSCL slope of a cell from its backward and forward differences.
+// For van Albada and minmod the reference's left-face state
+//   U_i + 1/2 lim(dL, dR) dL   (upwind2.h:44 with d01 = dL, d12 = dR)
+// and right-face state U_i + 1/2 lim(-dR, -dL)(-dR) are U_i + s/2 and U_i - s/2
+// with the symmetric slope s below (van Albada: lim(r)*d01 = dL*dR*(dL+dR)/(dL^2+dR^2);
+// minmod: lim(r)*d01 = minmod(dL, dR)).  The reference's nonZero(d01, 1e-10)
+// guard is replaced by a 1e-30 term in the denominator; the two differ only for
+// |dL| < 1e-10, where the slope itself is below 1e-10.
+template <typename T, int LIM>
+__device__ __forceinline__ T sym_slope(T dl, T dr) {
+  if constexpr (LIM == LIM_VANALBADA) {
+    const T num = dl * dr * (dl + dr);
+    const T den = dl * dl + dr * dr + T(1e-30);
+    return ddiv<T, true>(num, den);
+  } else {
+    // minmod
+    const T al = dabs(dl), ar = dabs(dr);
+    const T m = al < ar ? dl : dr;
+    return (dl * dr > T(0)) ? m : T(0);
+  }
+}
+
+__device__ __forceinline__ int fdiv_idx(int a, float rinv) {
+  // exact floor(a / n) for 0 <= a < 2^20 and n < 2^10, with rinv = 1/n
+  return (int)(((float)a + 0.5f) * rinv);
+}
+
+template <typename T, int LIM, int BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_stage_slope(const PatchDev* __restrict__ pds,
+                                                    const TileMarch* __restrict__ tiles, const T* __restrict__ src,
+                                                    const T* base, T* dst, uint64_t vs, T factor, int lds_cells,
+                                                    int lds_faces, int lds_sy, int lds_sz) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char drn_smem[];
+  T* sU0 = reinterpret_cast<T*>(drn_smem);  // [2][5][lds_cells]
+  T* sSy = sU0 + 10 * lds_cells;            // [5][lds_sy]
+  T* sSz = sSy + 5 * lds_sy;                // [5][lds_sz]
+  T* sF = sSz + 5 * lds_sz;                 // [5][lds_faces]
+  const TileMarch t = tiles[blockIdx.x];
+  const PatchDev& P = pds[t.patch];
+  const int ni = P.ni, nj = P.nj, nk = P.nk, njk = P.njk;
+  const float rnk = 1.0f / (float)nk, rnk1 = 1.0f / (float)(nk + 1);
+  const int c0 = t.c0;
+  const int cnt = min(BS, njk - c0);
+  const int tid = threadIdx.x;
+  const bool own = tid < cnt;
+  const int c = c0 + tid;
+  const int j = own ? fdiv_idx(c, rnk) : 0;
+  const int k = own ? c - j * nk : 0;
+  const int h0 = max(0, c0 - 2 * nk), h1 = min(njk, c0 + cnt + 2 * nk), hn = h1 - h0;
+  const int sy0 = max(0, c0 - nk), sy1 = min(njk, c0 + cnt + nk), nsy = sy1 - sy0;
+  const int sz0 = max(0, c0 - 1), sz1 = min(njk, c0 + cnt + 1), nsz = sz1 - sz0;
+  const int nY = cnt + nk;
+  const int jfirst = fdiv_idx(c0, rnk), jlast = fdiv_idx(c0 + cnt - 1, rnk);
+  const int zf0 = c0 + jfirst;
+  const int nZ = cnt + 1 + (jlast - jfirst);
+  const T dx = T(P.dx), dy = T(P.dy), dz = T(P.dz);
+  const T Ax = dy * dz, Ay = dx * dz, Az = dx * dy;
+  const T pf = factor / (dx * dy * dz);
+  const uint64_t pb = (uint64_t)P.off;
+  const uint64_t col = pb + (uint64_t)c;
+  const uint64_t sx = (uint64_t)njk;
+  const int i0 = t.i0, i1 = t.i1;
+  const T half = T(0.5);
+
+  // x direction in registers: W[q] = U(i-1+q) of the own column, slopes of planes i and i+1
+  T W[4][5], sxc[5], sxn[5], fm[5], fp[5];
+  auto loadW = [&](int q, int ip) {
+    if (own && ip >= 0 && ip < ni) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) W[q][v] = src[v * vs + col + (uint64_t)ip * sx];
+    }
+  };
+  // slope of plane ip from W[a] (ip-1), W[a+1] (ip), W[a+2] (ip+1)
+  auto xslope = [&](int ip, int a, T* sl) {
+    const bool ok = ip >= 1 && ip + 1 < ni;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) sl[v] = ok ? sym_slope<T, LIM>(W[a + 1][v] - W[a][v], W[a + 2][v] - W[a + 1][v]) : T(0);
+  };
+  // x face between planes ia-1 (W[b], slope sl) and ia (W[b+1], slope sr)
+  auto xface = [&](int ia, int b, const T* sl, const T* sr, T* f) {
+    zero5(f);
+    const T yc = half * dy + T(j) * dy, zc = half * dz + T(k) * dz;
+    if (ia == 0) {
+      wall_flux<T>(P.bc[0], P.par[0], W[b + 1], 0, false, yc, zc, j, k, Ax, f);
+    } else if (ia == ni) {
+      wall_flux<T>(P.bc[1], P.par[1], W[b], 0, true, yc, zc, j, k, Ax, f);
+    } else {
+      T vl[5], vr[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        vl[v] = W[b][v] + half * sl[v];
+        vr[v] = W[b + 1][v] - half * sr[v];
+      }
+      ausm_flux<T, true, 0>(vl, vr, Ax, f);
+    }
+  };
+
+  // prologue: x face i0-1/2 from U(i0-2 .. i0+1)
+  loadW(0, i0 - 2);
+  loadW(1, i0 - 1);
+  loadW(2, i0);
+  loadW(3, i0 + 1);
+  if (own) {
+    T sa[5];
+    xslope(i0 - 1, 0, sa);
+    xslope(i0, 1, sxc);
+    xface(i0, 1, sa, sxc, fm);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      W[0][v] = W[1][v];
+      W[1][v] = W[2][v];
+      W[2][v] = W[3][v];
+    }
+  }
+  loadW(3, i0 + 2);
+  // plane i0 straight into LDS buffer 0, plane i0+1 into the prefetch registers
+  constexpr int PRE = 2;  // the first PRE*BS halo cells of a plane go through registers
+  T pre[PRE][5];
+  auto prefetch = [&](int ip) {
+    if (ip < i1) {
+      const uint64_t prow = pb + (uint64_t)ip * sx + (uint64_t)h0;
+#pragma unroll
+      for (int r = 0; r < PRE; ++r) {
+        const int q = tid + r * BS;
+        if (q < hn) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) pre[r][v] = src[v * vs + prow + q];
+        }
+      }
+    }
+  };
+  {
+    const uint64_t prow = pb + (uint64_t)i0 * sx + (uint64_t)h0;
+    for (int q = tid; q < hn; q += BS) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) sU0[v * lds_cells + q] = src[v * vs + prow + q];
+    }
+  }
+  prefetch(i0 + 1);
+  T rx[5];  // x contribution Fx(i-1/2) - Fx(i+1/2) of the plane awaiting its update
+  // residual and update of plane ip from rx and the y/z fluxes in sF
+  auto update = [&](int ip) {
+    if (own) {
+      const int qy = c - c0, qz = nY + (c + j) - zf0;
+      const uint64_t g = col + (uint64_t)ip * sx;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const T r = rx[v] + (sF[v * lds_faces + qy] - sF[v * lds_faces + qy + nk]) +
+                    (sF[v * lds_faces + qz] - sF[v * lds_faces + qz + 1]);
+        dst[v * vs + g] = base[v * vs + g] + pf * r;
+      }
+    }
+  };
+
+  for (int i = i0; i < i1; ++i) {
+    T* sU = sU0 + (size_t)((i - i0) & 1) * 5 * lds_cells;
+    T* sUn = sU0 + (size_t)((i - i0 + 1) & 1) * 5 * lds_cells;
+    __syncthreads();
+    // phase A: y and z slopes of plane i; x slope of plane i+1; update of plane i-1
+    for (int tk = tid; tk < nsy + nsz; tk += BS) {
+      if (tk < nsy) {
+        const int cc = sy0 + tk;
+        const int jj = fdiv_idx(cc, rnk);
+        const int q = cc - h0;
+        const bool ok = jj >= 1 && jj + 1 < nj;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          const T um = sU[v * lds_cells + (ok ? q - nk : q)], u0 = sU[v * lds_cells + q],
+                  up = sU[v * lds_cells + (ok ? q + nk : q)];
+          sSy[v * lds_sy + tk] = ok ? sym_slope<T, LIM>(u0 - um, up - u0) : T(0);
+        }
+      } else {
+        const int r = tk - nsy;
+        const int cc = sz0 + r;
+        const int kk = cc - fdiv_idx(cc, rnk) * nk;
+        const int q = cc - h0;
+        const bool ok = kk >= 1 && kk + 1 < nk;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          const T um = sU[v * lds_cells + (ok ? q - 1 : q)], u0 = sU[v * lds_cells + q],
+                  up = sU[v * lds_cells + (ok ? q + 1 : q)];
+          sSz[v * lds_sz + r] = ok ? sym_slope<T, LIM>(u0 - um, up - u0) : T(0);
+        }
+      }
+    }
+    if (i > i0) update(i - 1);
+    if (own) xslope(i + 1, 1, sxn);
+    __syncthreads();
+    // phase B: y and z face fluxes of plane i, x face i+1/2; stage plane i+1
+    const T xi = half * dx + T(i) * dx;
+    for (int tk = tid; tk < nY + nZ; tk += BS) {
+      T f[5];
+      zero5(f);
+      int cm1, cp1, wall = -1, dir, ja = 0, kb = 0;
+      if (tk < nY) {
+        dir = 1;
+        const int fy = c0 + tk;
+        const int jf = fdiv_idx(fy, rnk);
+        cp1 = fy;
+        cm1 = fy - nk;
+        if (jf == 0) {
+          wall = 2;
+          kb = fy;
+        } else if (jf == nj) {
+          wall = 3;
+          kb = fy - jf * nk;
+        }
+      } else {
+        dir = 2;
+        const int zf = zf0 + (tk - nY);
+        const int jz = fdiv_idx(zf, rnk1);
+        const int kz = zf - jz * (nk + 1);
+        cm1 = jz * nk + kz - 1;
+        cp1 = cm1 + 1;
+        ja = jz;
+        if (kz == 0)
+          wall = 4;
+        else if (kz == nk)
+          wall = 5;
+      }
+      if (wall >= 0) {
+        const bool plus = (wall & 1) != 0;
+        const int cell = (plus ? cm1 : cp1) - h0;
+        T var[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) var[v] = sU[v * lds_cells + cell];
+        if (dir == 1)
+          wall_flux<T>(P.bc[wall], P.par[wall], var, 1, plus, xi, half * dz + T(kb) * dz, i, kb, Ay, f);
+        else
+          wall_flux<T>(P.bc[wall], P.par[wall], var, 2, plus, xi, half * dy + T(ja) * dy, i, ja, Az, f);
+      } else {
+        T vl[5], vr[5];
+        const int qm = cm1 - h0, qp = cp1 - h0;
+        const T* sS = dir == 1 ? sSy : sSz;
+        const int ls = dir == 1 ? lds_sy : lds_sz;
+        const int s0 = dir == 1 ? sy0 : sz0;
+        const int sm = cm1 - s0, sp = cp1 - s0;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          vl[v] = sU[v * lds_cells + qm] + half * sS[v * ls + sm];
+          vr[v] = sU[v * lds_cells + qp] - half * sS[v * ls + sp];
+        }
+        ausm_flux_rt<T, true>(vl, vr, dir, dir == 1 ? Ay : Az, f);
+      }
+#pragma unroll
+      for (int v = 0; v < 5; ++v) sF[v * lds_faces + tk] = f[v];
+    }
+    if (own) {
+      xface(i + 1, 1, sxc, sxn, fp);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        rx[v] = fm[v] - fp[v];
+        fm[v] = fp[v];
+        sxc[v] = sxn[v];
+        W[0][v] = W[1][v];
+        W[1][v] = W[2][v];
+        W[2][v] = W[3][v];
+      }
+    }
+    if (i + 1 < i1) {
+#pragma unroll
+      for (int r = 0; r < PRE; ++r) {
+        const int q = tid + r * BS;
+        if (q < hn) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) sUn[v * lds_cells + q] = pre[r][v];
+        }
+      }
+      // wide planes: the rest of the halo is loaded directly
+      const uint64_t prow = pb + (uint64_t)(i + 1) * sx + (uint64_t)h0;
+      for (int q = PRE * BS + tid; q < hn; q += BS) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) sUn[v * lds_cells + q] = src[v * vs + prow + q];
+      }
+      prefetch(i + 2);
+    }
+    loadW(3, i + 3);
+  }
+  __syncthreads();
+  update(i1 - 1);
+}
+
+// ---------------------------------------------------------------------------
+// column stage kernel (fast arithmetic, van Albada / minmod): no LDS, no barriers
+// ---------------------------------------------------------------------------
+
+// Each wavefront owns 63 consecutive cells of the (j,k) plane of one patch
+// (lane l < 63 owns cell base + l) and marches them along i over [i0, i1).
+//   x: every owned column keeps U(i-1..i+2) and the slope of plane i in
+//      registers, so each x face and each x slope is evaluated once.
+//   z: lane l evaluates the z face between cells base+l-1 and base+l, i.e. the
+//      minus face of its own cell; the plus face is the next lane's value
+//      (lane 63 exists only to evaluate the plus face of cell base+62).  The
+//      patch-face functors at k = 0 / NK-1 are the faces of those cells.
+//   y: both y faces of the owned cell are evaluated from the slopes of cells
+//      j-1, j, j+1 (the reference's two MUSCL states of each face).
+template <typename T, int LIM>
+__global__ __launch_bounds__(256) void k_stage_column(const PatchDev* __restrict__ pds,
+                                                      const TileMarch* __restrict__ tiles,
+                                                      const T* __restrict__ src, const T* base, T* dst, uint64_t vs,
+                                                      T factor) {
+  const TileMarch t = tiles[blockIdx.x];
+  const PatchDev& P = pds[t.patch];
+  const int ni = P.ni, nj = P.nj, nk = P.nk, njk = P.njk;
+  const float rnk = 1.0f / (float)nk;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wbase = t.c0 + 63 * wv;
+  const int c = wbase + lane;                      // cell whose minus z face this lane evaluates
+  const bool in_patch = c < njk;
+  const bool own = lane < 63 && in_patch;          // lane owns cell c
+  const int cc = in_patch ? c : njk - 1;            // clamped for index math of idle lanes
+  const int j = fdiv_idx(cc, rnk);
+  const int k = cc - j * nk;
+  const T dx = T(P.dx), dy = T(P.dy), dz = T(P.dz);
+  const T Ax = dy * dz, Ay = dx * dz, Az = dx * dy;
+  const T pf = factor / (dx * dy * dz);
+  const T half = T(0.5);
+  const uint64_t sx = (uint64_t)njk, sy = (uint64_t)nk;
+  const uint64_t col = (uint64_t)P.off + (uint64_t)cc;
+  const int i0 = t.i0, i1 = t.i1;
+  const T yc = half * dy + T(j) * dy, zc = half * dz + T(k) * dz;
+  // slope validity of the cells j-1, j, j+1 and k-1, k (first order next to the patch faces)
+  const bool okj_m = j - 1 >= 1 && j - 1 + 1 < nj, okj_0 = j >= 1 && j + 1 < nj, okj_p = j + 1 >= 1 && j + 2 < nj;
+  const bool okk_m = k - 1 >= 1 && k < nk, okk_0 = k >= 1 && k + 1 < nk;
+  const bool has_jm2 = j >= 2, has_jp2 = j + 2 < nj, has_km2 = k >= 2, has_kp1 = k + 1 < nk;
+
+  auto ld = [&](int v, uint64_t g) { return src[v * vs + g]; };
+
+  T W[4][5], sxc[5], sxn[5], fm[5], fp[5];
+  auto loadW = [&](int q, int ip) {
+    if (own && ip >= 0 && ip < ni) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) W[q][v] = ld(v, col + (uint64_t)ip * sx);
+    }
+  };
+  auto xslope = [&](int ip, int a, T* sl) {
+    const bool ok = ip >= 1 && ip + 1 < ni;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) sl[v] = ok ? sym_slope<T, LIM>(W[a + 1][v] - W[a][v], W[a + 2][v] - W[a + 1][v]) : T(0);
+  };
+  auto xface = [&](int ia, int b, const T* sl, const T* sr, T* f) {
+    zero5(f);
+    if (ia == 0) {
+      wall_flux<T>(P.bc[0], P.par[0], W[b + 1], 0, false, yc, zc, j, k, Ax, f);
+    } else if (ia == ni) {
+      wall_flux<T>(P.bc[1], P.par[1], W[b], 0, true, yc, zc, j, k, Ax, f);
+    } else {
+      T vl[5], vr[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        vl[v] = W[b][v] + half * sl[v];
+        vr[v] = W[b + 1][v] - half * sr[v];
+      }
+      ausm_flux<T, true, 0>(vl, vr, Ax, f);
+    }
+  };
+
+  loadW(0, i0 - 2);
+  loadW(1, i0 - 1);
+  loadW(2, i0);
+  loadW(3, i0 + 1);
+  if (own) {
+    T sa[5];
+    xslope(i0 - 1, 0, sa);
+    xslope(i0, 1, sxc);
+    xface(i0, 1, sa, sxc, fm);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      W[0][v] = W[1][v];
+      W[1][v] = W[2][v];
+      W[2][v] = W[3][v];
+    }
+  }
+  loadW(3, i0 + 2);
+
+  for (int i = i0; i < i1; ++i) {
+    const uint64_t g = col + (uint64_t)i * sx;
+    const T xi = half * dx + T(i) * dx;
+    T U0[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) U0[v] = own ? W[1][v] : ld(v, g);
+    // ---- z: minus face of cell c (every lane in the patch)
+    T fz[5];
+    zero5(fz);
+    if (in_patch) {
+      if (k == 0) {
+        wall_flux<T>(P.bc[4], P.par[4], U0, 2, false, xi, yc, i, j, Az, fz);
+      } else {
+        T Um[5], Umm[5], Up[5], vl[5], vr[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          Um[v] = ld(v, g - 1);
+          Umm[v] = has_km2 ? ld(v, g - 2) : Um[v];
+          Up[v] = has_kp1 ? ld(v, g + 1) : U0[v];
+        }
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          const T sm = okk_m ? sym_slope<T, LIM>(Um[v] - Umm[v], U0[v] - Um[v]) : T(0);
+          const T s0 = okk_0 ? sym_slope<T, LIM>(U0[v] - Um[v], Up[v] - U0[v]) : T(0);
+          vl[v] = Um[v] + half * sm;
+          vr[v] = U0[v] - half * s0;
+        }
+        ausm_flux<T, true, 2>(vl, vr, Az, fz);
+      }
+    }
+    // plus z face of the owned cell: the next lane's minus face, or the wall at k = NK-1
+    T fzp[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) fzp[v] = __shfl_down(fz[v], 1, 64);
+    if (own && k == nk - 1) {
+      zero5(fzp);
+      wall_flux<T>(P.bc[5], P.par[5], U0, 2, true, xi, yc, i, j, Az, fzp);
+    }
+    if (own) {
+      // ---- y: both faces of the owned cell
+      T fym[5], fyp[5];
+      zero5(fym);
+      zero5(fyp);
+      {
+        T Ujm[5], Ujmm[5], Ujp[5], Ujpp[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          Ujm[v] = j >= 1 ? ld(v, g - sy) : U0[v];
+          Ujmm[v] = has_jm2 ? ld(v, g - 2 * sy) : Ujm[v];
+          Ujp[v] = j + 1 < nj ? ld(v, g + sy) : U0[v];
+          Ujpp[v] = has_jp2 ? ld(v, g + 2 * sy) : Ujp[v];
+        }
+        T sjm[5], sj0[5], sjp[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          sjm[v] = okj_m ? sym_slope<T, LIM>(Ujm[v] - Ujmm[v], U0[v] - Ujm[v]) : T(0);
+          sj0[v] = okj_0 ? sym_slope<T, LIM>(U0[v] - Ujm[v], Ujp[v] - U0[v]) : T(0);
+          sjp[v] = okj_p ? sym_slope<T, LIM>(Ujp[v] - U0[v], Ujpp[v] - Ujp[v]) : T(0);
+        }
+        if (j == 0) {
+          wall_flux<T>(P.bc[2], P.par[2], U0, 1, false, xi, zc, i, k, Ay, fym);
+        } else {
+          T vl[5], vr[5];
+#pragma unroll
+          for (int v = 0; v < 5; ++v) {
+            vl[v] = Ujm[v] + half * sjm[v];
+            vr[v] = U0[v] - half * sj0[v];
+          }
+          ausm_flux<T, true, 1>(vl, vr, Ay, fym);
+        }
+        if (j == nj - 1) {
+          wall_flux<T>(P.bc[3], P.par[3], U0, 1, true, xi, zc, i, k, Ay, fyp);
+        } else {
+          T vl[5], vr[5];
+#pragma unroll
+          for (int v = 0; v < 5; ++v) {
+            vl[v] = U0[v] + half * sj0[v];
+            vr[v] = Ujp[v] - half * sjp[v];
+          }
+          ausm_flux<T, true, 1>(vl, vr, Ay, fyp);
+        }
+      }
+      // ---- x: face i+1/2 from the register window
+      xslope(i + 1, 1, sxn);
+      xface(i + 1, 1, sxc, sxn, fp);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const T r = (fm[v] - fp[v]) + (fym[v] - fyp[v]) + (fz[v] - fzp[v]);
+        dst[v * vs + g] = base[v * vs + g] + pf * r;
+        fm[v] = fp[v];
+        sxc[v] = sxn[v];
+        W[0][v] = W[1][v];
+        W[1][v] = W[2][v];
+        W[2][v] = W[3][v];
+      }
+    }
+    loadW(3, i + 3);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// cell-per-thread stage kernel, slope form (fast arithmetic, van Albada / minmod)
+// ---------------------------------------------------------------------------
+
+// One thread per cell as in k_stage_cell, but along each axis the limited
+// slopes of the cells pos-1, pos, pos+1 are evaluated once (sym_slope) and the
+// four MUSCL states of the two faces are U -/+ s/2; the own cell's slope serves
+// both faces.  Nine slope evaluations per cell instead of twelve reconstructions.
+template <typename T, int LIM, int DIR>
+__device__ __forceinline__ void axis_faces(const PatchDev& P, const T* __restrict__ src, uint64_t vs, uint64_t g,
+                                           uint64_t st, int pos, int n, const T* U0, T A, T ta, T tb, int la, int lb,
+                                           T* res) {
+  const T half = T(0.5);
+  if (n <= 2) return;  // the reference skips an axis with at most two cells
+  T Um[5], Umm[5], Up[5], Upp[5];
+  const bool hm = pos >= 1, hmm = pos >= 2, hp = pos + 1 < n, hpp = pos + 2 < n;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    Um[v] = hm ? src[v * vs + g - st] : U0[v];
+    Umm[v] = hmm ? src[v * vs + g - 2 * st] : Um[v];
+    Up[v] = hp ? src[v * vs + g + st] : U0[v];
+    Upp[v] = hpp ? src[v * vs + g + 2 * st] : Up[v];
+  }
+  T s0[5];
+  const bool ok0 = hm && hp;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) s0[v] = ok0 ? sym_slope<T, LIM>(U0[v] - Um[v], Up[v] - U0[v]) : T(0);
+  T f[5];
+  // minus face
+  zero5(f);
+  if (!hm) {
+    wall_flux<T>(P.bc[2 * DIR], P.par[2 * DIR], U0, DIR, false, ta, tb, la, lb, A, f);
+  } else {
+    T vl[5], vr[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const T sm = hmm ? sym_slope<T, LIM>(Um[v] - Umm[v], U0[v] - Um[v]) : T(0);
+      vl[v] = Um[v] + half * sm;
+      vr[v] = U0[v] - half * s0[v];
+    }
+    ausm_flux<T, true, DIR>(vl, vr, A, f);
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) res[v] += f[v];
+  // plus face
+  zero5(f);
+  if (!hp) {
+    wall_flux<T>(P.bc[2 * DIR + 1], P.par[2 * DIR + 1], U0, DIR, true, ta, tb, la, lb, A, f);
+  } else {
+    T vl[5], vr[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const T sp = hpp ? sym_slope<T, LIM>(Up[v] - U0[v], Upp[v] - Up[v]) : T(0);
+      vl[v] = U0[v] + half * s0[v];
+      vr[v] = Up[v] - half * sp;
+    }
+    ausm_flux<T, true, DIR>(vl, vr, A, f);
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) res[v] -= f[v];
+}
+
+template <typename T, int LIM, int BS>
+__global__ __launch_bounds__(BS) void k_stage_cell_slope(const PatchDev* __restrict__ pds,
+                                                         const TileCell* __restrict__ tiles,
+                                                         const T* __restrict__ src, const T* base, T* dst, uint64_t vs,
+                                                         T factor) {
+  const TileCell t = tiles[blockIdx.x];
+  const PatchDev& P = pds[t.patch];
+  const int nk = P.nk, njk = P.njk;
+  const int c2 = t.c2 + (int)threadIdx.x;
+  if (c2 >= njk) return;
+  const int i = t.i;
+  const int j = fdiv_idx(c2, 1.0f / (float)nk);
+  const int k = c2 - j * nk;
+  const uint64_t g = (uint64_t)P.off + (uint64_t)i * (uint64_t)njk + (uint64_t)c2;
+  const T dx = T(P.dx), dy = T(P.dy), dz = T(P.dz);
+  const T half = T(0.5);
+  const T xc = half * dx + T(i) * dx, yc = half * dy + T(j) * dy, zc = half * dz + T(k) * dz;
+  T U0[5], res[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    U0[v] = src[v * vs + g];
+    res[v] = T(0);
+  }
+  axis_faces<T, LIM, 0>(P, src, vs, g, (uint64_t)njk, i, P.ni, U0, dy * dz, yc, zc, j, k, res);
+  axis_faces<T, LIM, 1>(P, src, vs, g, (uint64_t)nk, j, P.nj, U0, dx * dz, xc, zc, i, k, res);
+  axis_faces<T, LIM, 2>(P, src, vs, g, 1, k, nk, U0, dx * dy, xc, yc, i, j, res);
+  const T pf = factor / (dx * dy * dz);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) dst[v * vs + g] = base[v * vs + g] + pf * res[v];
+}
+
 // plain vectorised copy (PatchIterator::copyField); also the STREAM-style
 // reference kernel whose bytes are known exactly (16 B per lane per iteration)
 __global__ __launch_bounds__(256) void k_copy(uint64_t n16, const uint4* __restrict__ src, uint4* __restrict__ dst) {

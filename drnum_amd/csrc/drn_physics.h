@@ -114,7 +114,10 @@ __device__ __forceinline__ Prim<T> lr_vars(const T* var) {
   T e = var[4] * ir - T(0.5) * (s.u * s.u + s.v * s.v + s.w * s.w);
   s.T_ = FAST ? e * inv_cv : e / Gas<T>::CV;
   s.p = s.r * Gas<T>::R * s.T_;
-  s.a = dsqrt(Gas<T>::GAMMA * Gas<T>::R * s.T_);
+  if constexpr (FAST && sizeof(T) == 4)
+    s.a = __builtin_amdgcn_sqrtf(Gas<T>::GAMMA * Gas<T>::R * s.T_);  // v_sqrt_f32, 1 ulp
+  else
+    s.a = dsqrt(Gas<T>::GAMMA * Gas<T>::R * s.T_);
   s.H = FAST ? (var[4] + s.p) * ir : (var[4] + s.p) / s.r;
   return s;
 }

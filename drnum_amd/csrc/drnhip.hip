@@ -58,15 +58,16 @@ struct drn_ctx {
   TileCell* d_tiles = nullptr;
   double* d_part = nullptr;  // reduction partials (5 per tile)
   double* d_out = nullptr;
-  int tiling = 0;
+  int tiling = 11;     // batched form
+  int tiling_pp = 11;  // per-patch form
   int arith = -1;  // DRN_ARITH_*; -1 = precision default
-  // marching work-lists: natural order (patch-contiguous) and launch order
-  std::vector<TileMarch> mt_nat;
-  std::vector<int> mt_first;  // per patch, size np+1 (into mt_nat)
-  TileMarch* d_mt_nat = nullptr;
+  // marching work-lists: batched (launch order) and per-patch (natural order, patch-contiguous)
+  std::vector<TileMarch> mt_pp;
+  std::vector<int> mt_first;  // per patch, size np+1 (into mt_pp)
+  TileMarch* d_mt_pp = nullptr;
   TileMarch* d_mt_launch = nullptr;
   int n_mt = 0;
-  int lds_cells = 0, lds_faces = 0;
+  int lds_cells = 0, lds_faces = 0, lds_sy = 0, lds_sz = 0;
   std::vector<int> thin;      // patches routed to the cell kernel
   std::vector<TileCell> thin_tiles;
   TileCell* d_thin_tiles = nullptr;
@@ -159,55 +160,100 @@ static void timer_end(drn_ctx* ctx, hipEvent_t stop) {
 
 constexpr int MARCH_BS = 256;
 
-// tiling choices: (i-range length, work-list order); the last entry is the
-// cell-per-thread kernel (every interior face evaluated twice)
+// tiling choices: kernel, i-range length (0 = whole patch) and work-list order
+enum { K_MARCH = 0, K_CELL = 1, K_COLUMN = 2, K_CELL_SLOPE = 3 };
 struct TilingChoice {
   int irange;  // planes per work item, 0 = whole patch
   int xcd;     // 1 = segment the work list so that neighbouring chunks share an XCD
-  int cell;    // 1 = k_stage_cell
+  int kind;    // K_MARCH (LDS planes, or the slope form in fast mode), K_CELL, K_COLUMN
 };
-static const TilingChoice kTilings[] = {{0, 0, 0}, {0, 1, 0}, {32, 0, 0}, {32, 1, 0}, {16, 0, 0},
-                                        {16, 1, 0}, {8, 0, 0},  {8, 1, 0},  {0, 0, 1}};
+static const TilingChoice kTilings[] = {
+    {0, 0, K_MARCH},  {0, 1, K_MARCH},  {32, 0, K_MARCH},  {32, 1, K_MARCH}, {16, 0, K_MARCH},
+    {16, 1, K_MARCH}, {8, 0, K_MARCH},  {8, 1, K_MARCH},   {4, 0, K_MARCH},  {4, 1, K_MARCH},
+    {2, 0, K_MARCH},  {0, 0, K_CELL},   {0, 0, K_COLUMN},  {0, 1, K_COLUMN}, {32, 1, K_COLUMN},
+    {16, 0, K_COLUMN}, {16, 1, K_COLUMN}, {8, 1, K_COLUMN}, {4, 1, K_COLUMN}, {0, 0, K_CELL_SLOPE}};
 static const int kNumTilings = sizeof(kTilings) / sizeof(kTilings[0]);
+
+// the column kernel covers the fast MUSCL forms with a symmetric limited slope
+static bool column_capable(const drn_ctx* ctx);
 
 template <typename T, int RECON, int LIM, bool FAST>
 static void launch_march_t(drn_ctx* ctx, const TileMarch* tiles, int ntiles, const T* src, const T* base, T* dst,
-                           T factor) {
+                           T factor, bool column) {
   if (ntiles <= 0) return;
-  const size_t lds = (size_t)5 * (ctx->lds_cells + ctx->lds_faces) * sizeof(T);
-  hipLaunchKernelGGL((k_stage_march<T, RECON, LIM, FAST, MARCH_BS>), dim3(ntiles), dim3(MARCH_BS), lds, ctx->stream,
-                     ctx->d_pd, tiles, src, base, dst, ctx->vstride, factor, ctx->lds_cells, ctx->lds_faces);
+  if constexpr (FAST && RECON == RECON_UPWIND2 && (LIM == LIM_VANALBADA || LIM == LIM_MINMOD)) {
+    if (column) {
+      hipLaunchKernelGGL((k_stage_column<T, LIM>), dim3(ntiles), dim3(256), 0, ctx->stream, ctx->d_pd, tiles, src,
+                         base, dst, ctx->vstride, factor);
+      return;
+    }
+    // slope form: per-cell limited slopes shared by both faces of the cell
+    const size_t lds = (size_t)(10 * ctx->lds_cells + 5 * (ctx->lds_sy + ctx->lds_sz + ctx->lds_faces)) * sizeof(T);
+    hipLaunchKernelGGL((k_stage_slope<T, LIM, MARCH_BS>), dim3(ntiles), dim3(MARCH_BS), lds, ctx->stream, ctx->d_pd,
+                       tiles, src, base, dst, ctx->vstride, factor, ctx->lds_cells, ctx->lds_faces, ctx->lds_sy,
+                       ctx->lds_sz);
+  } else {
+    const size_t lds = (size_t)5 * (ctx->lds_cells + ctx->lds_faces) * sizeof(T);
+    hipLaunchKernelGGL((k_stage_march<T, RECON, LIM, FAST, MARCH_BS>), dim3(ntiles), dim3(MARCH_BS), lds, ctx->stream,
+                       ctx->d_pd, tiles, src, base, dst, ctx->vstride, factor, ctx->lds_cells, ctx->lds_faces);
+  }
 }
 
 template <typename T, int RECON, int LIM, bool FAST>
 static void launch_cell_t(drn_ctx* ctx, const TileCell* tiles, int ntiles, const T* src, const T* base, T* dst,
-                          T factor) {
+                          T factor, bool slope) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, FAST, 256>), dim3(ntiles), dim3(256), 0, ctx->stream, ctx->d_pd,
-                     tiles, src, base, dst, ctx->vstride, factor);
+  if constexpr (FAST && RECON == RECON_UPWIND2 && (LIM == LIM_VANALBADA || LIM == LIM_MINMOD)) {
+    if (slope) {
+      hipLaunchKernelGGL((k_stage_cell_slope<T, LIM, 256>), dim3(ntiles), dim3(256), 0, ctx->stream, ctx->d_pd, tiles,
+                         src, base, dst, ctx->vstride, factor);
+      return;
+    }
+  }
+  {
+    hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, FAST, 256>), dim3(ntiles), dim3(256), 0, ctx->stream, ctx->d_pd,
+                       tiles, src, base, dst, ctx->vstride, factor);
+  }
 }
 
 // one launch of either kernel over a tile range (march tiles or cell tiles)
+// kind: K_CELL runs k_stage_cell on TileCell items, otherwise the marching
+// kernels (column kernel when kind == K_COLUMN and the operator allows it)
 template <typename T, int RECON, int LIM, bool FAST>
-static void launch_range(drn_ctx* ctx, bool cell, const void* tiles, int ntiles, const T* src, const T* base, T* dst,
+static void launch_range(drn_ctx* ctx, int kind, const void* tiles, int ntiles, const T* src, const T* base, T* dst,
                          T factor) {
-  if (cell)
-    launch_cell_t<T, RECON, LIM, FAST>(ctx, (const TileCell*)tiles, ntiles, src, base, dst, factor);
+  if (kind == K_CELL || kind == K_CELL_SLOPE)
+    launch_cell_t<T, RECON, LIM, FAST>(ctx, (const TileCell*)tiles, ntiles, src, base, dst, factor,
+                                       kind == K_CELL_SLOPE);
   else
-    launch_march_t<T, RECON, LIM, FAST>(ctx, (const TileMarch*)tiles, ntiles, src, base, dst, factor);
+    launch_march_t<T, RECON, LIM, FAST>(ctx, (const TileMarch*)tiles, ntiles, src, base, dst, factor,
+                                        kind == K_COLUMN);
 }
 
 template <typename T, bool FAST>
-static void launch_dispatch(drn_ctx* ctx, bool cell, const void* tiles, int ntiles, const T* src, const T* base, T* dst,
+static void launch_dispatch(drn_ctx* ctx, int kind, const void* tiles, int ntiles, const T* src, const T* base, T* dst,
                             T factor) {
   if (ctx->recon == DRN_RECON_UPWIND1)
-    launch_range<T, RECON_UPWIND1, LIM_SECONDORDER, FAST>(ctx, cell, tiles, ntiles, src, base, dst, factor);
+    launch_range<T, RECON_UPWIND1, LIM_SECONDORDER, FAST>(ctx, kind, tiles, ntiles, src, base, dst, factor);
   else if (ctx->lim == DRN_LIM_VANALBADA)
-    launch_range<T, RECON_UPWIND2, LIM_VANALBADA, FAST>(ctx, cell, tiles, ntiles, src, base, dst, factor);
+    launch_range<T, RECON_UPWIND2, LIM_VANALBADA, FAST>(ctx, kind, tiles, ntiles, src, base, dst, factor);
   else if (ctx->lim == DRN_LIM_MINMOD)
-    launch_range<T, RECON_UPWIND2, LIM_MINMOD, FAST>(ctx, cell, tiles, ntiles, src, base, dst, factor);
+    launch_range<T, RECON_UPWIND2, LIM_MINMOD, FAST>(ctx, kind, tiles, ntiles, src, base, dst, factor);
   else
-    launch_range<T, RECON_UPWIND2, LIM_SECONDORDER, FAST>(ctx, cell, tiles, ntiles, src, base, dst, factor);
+    launch_range<T, RECON_UPWIND2, LIM_SECONDORDER, FAST>(ctx, kind, tiles, ntiles, src, base, dst, factor);
+}
+
+static bool column_capable(const drn_ctx* ctx) {
+  const bool fast = ctx->arith < 0 ? ctx->prec == DRN_F32 : ctx->arith == DRN_ARITH_FAST;
+  return fast && ctx->recon == DRN_RECON_UPWIND2 && (ctx->lim == DRN_LIM_VANALBADA || ctx->lim == DRN_LIM_MINMOD);
+}
+
+// effective kernel kind of a tiling choice in the current arithmetic/operator
+static int effective_kind(const drn_ctx* ctx, int choice) {
+  const int k = kTilings[choice].kind;
+  if (k == K_COLUMN && !column_capable(ctx)) return K_MARCH;
+  if (k == K_CELL_SLOPE && !column_capable(ctx)) return K_CELL;
+  return k;
 }
 
 static bool fast_math(const drn_ctx* ctx) {
@@ -220,30 +266,30 @@ static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int l
   const T* base = (const T*)ctx->buf[pbase];
   T* dst = (T*)ctx->buf[pdst];
   const bool fast = fast_math(ctx);
-  const bool cell = kTilings[ctx->tiling].cell != 0;
-  auto go = [&](bool c, const void* tl, int n) {
+  auto go = [&](int kind, const void* tl, int n) {
     if (fast)
-      launch_dispatch<T, true>(ctx, c, tl, n, src, base, dst, factor);
+      launch_dispatch<T, true>(ctx, kind, tl, n, src, base, dst, factor);
     else
-      launch_dispatch<T, false>(ctx, c, tl, n, src, base, dst, factor);
+      launch_dispatch<T, false>(ctx, kind, tl, n, src, base, dst, factor);
   };
   if (launch == DRN_LAUNCH_BATCHED) {
-    if (cell) {
-      go(true, ctx->d_tiles, (int)ctx->tiles.size());
+    const int kind = effective_kind(ctx, ctx->tiling);
+    if (kind == K_CELL || kind == K_CELL_SLOPE) {
+      go(kind, ctx->d_tiles, (int)ctx->tiles.size());
     } else {
-      go(false, ctx->d_mt_launch, ctx->n_mt);
-      go(true, ctx->d_thin_tiles, (int)ctx->thin_tiles.size());
+      go(kind, ctx->d_mt_launch, ctx->n_mt);
+      go(K_CELL, ctx->d_thin_tiles, (int)ctx->thin_tiles.size());
     }
   } else {
+    const int kind = effective_kind(ctx, ctx->tiling_pp);
     const int np = (int)ctx->descs.size();
     for (int p = 0; p < np; ++p) {
-      const bool thin = std::find(ctx->thin.begin(), ctx->thin.end(), p) != ctx->thin.end();
-      if (cell || thin) {
+      const int m0 = ctx->mt_first[p], mn = ctx->mt_first[p + 1] - m0;
+      if (kind == K_CELL || kind == K_CELL_SLOPE || mn == 0) {
         const int t0 = ctx->tile_first[p];
-        go(true, ctx->d_tiles + t0, ctx->tile_first[p + 1] - t0);
+        go(kind == K_CELL_SLOPE ? K_CELL_SLOPE : K_CELL, ctx->d_tiles + t0, ctx->tile_first[p + 1] - t0);
       } else {
-        const int t0 = ctx->mt_first[p];
-        go(false, ctx->d_mt_nat + t0, ctx->mt_first[p + 1] - t0);
+        go(kind, ctx->d_mt_pp + m0, mn);
       }
     }
   }
@@ -261,69 +307,86 @@ static int do_stage(drn_ctx* ctx, int psrc, int pbase, int pdst, double factor, 
   return DRN_OK;
 }
 
-// build the marching work-lists of the current tiling choice
+// marching work items of one patch for an i-range length L (0 = whole patch);
+// i ranges outermost so that the chunks of one plane range are adjacent
+static void march_items(const drn_patch_desc& d, int p, int L, int chunk, std::vector<TileMarch>& out) {
+  const int njk = d.nj * d.nk;
+  const int Lp = L > 0 ? L : d.ni;
+  for (int i0 = 0; i0 < d.ni; i0 += Lp)
+    for (int c0 = 0; c0 < njk; c0 += chunk) out.push_back(TileMarch{p, i0, std::min(d.ni, i0 + Lp), c0});
+}
+
+// cells of the (j,k) plane per marching work item: the column kernel's four
+// wavefronts own 63 cells each
+static int chunk_of(const drn_ctx* ctx, int choice) { return effective_kind(ctx, choice) == K_COLUMN ? 252 : MARCH_BS; }
+
+// build the marching work-lists of the current tiling choices of both launch forms
 static int build_march(drn_ctx* ctx) {
-  const TilingChoice tc = kTilings[ctx->tiling];
+  const TilingChoice tb = kTilings[ctx->tiling], tp = kTilings[ctx->tiling_pp];
   const int np = (int)ctx->descs.size();
-  ctx->mt_nat.clear();
+  std::vector<TileMarch> nat;
+  ctx->mt_pp.clear();
   ctx->mt_first.assign(np + 1, 0);
   ctx->thin.clear();
   ctx->thin_tiles.clear();
-  int lc = 0, lf = 0;
+  int lc = 0, lf = 0, lsy = 0, lsz = 0;
   for (int p = 0; p < np; ++p) {
     const auto& d = ctx->descs[p];
-    ctx->mt_first[p] = (int)ctx->mt_nat.size();
+    ctx->mt_first[p] = (int)ctx->mt_pp.size();
     const int njk = d.nj * d.nk;
-    int plc = 0, plf = 0;
+    int plc = 0, plf = 0, psy = 0, psz = 0;
     for (int c0 = 0; c0 < njk; c0 += MARCH_BS) {
       const int cnt = std::min(MARCH_BS, njk - c0);
       plc = std::max(plc, std::min(njk, c0 + cnt + 2 * d.nk) - std::max(0, c0 - 2 * d.nk));
+      psy = std::max(psy, std::min(njk, c0 + cnt + d.nk) - std::max(0, c0 - d.nk));
+      psz = std::max(psz, std::min(njk, c0 + cnt + 1) - std::max(0, c0 - 1));
       const int rows = (c0 + cnt - 1) / d.nk - c0 / d.nk;
       plf = std::max(plf, (cnt + d.nk) + (cnt + 1 + rows));
     }
     // thin patches (the reference skips the faces of an axis with <= 2 cells) and
-    // patches whose staged plane would not fit the LDS budget use the cell kernel
-    const size_t need = (size_t)5 * (plc + plf) * (ctx->prec == DRN_F32 ? 4 : 8);
-    if (d.ni < 3 || d.nj < 3 || d.nk < 3 || need > 96 * 1024) {
+    // patches whose staged planes would not fit the LDS budget use the cell kernel
+    const size_t need = (size_t)(10 * plc + 5 * (plf + psy + psz)) * (ctx->prec == DRN_F32 ? 4 : 8);
+    if (d.ni < 3 || d.nj < 3 || d.nk < 3 || d.nk > 1000 || njk > (1 << 20) || need > 120 * 1024) {
       ctx->thin.push_back(p);
       for (int t = ctx->tile_first[p]; t < ctx->tile_first[p + 1]; ++t) ctx->thin_tiles.push_back(ctx->tiles[t]);
       continue;
     }
     lc = std::max(lc, plc);
     lf = std::max(lf, plf);
-    const int L = tc.irange > 0 ? tc.irange : d.ni;
-    // i ranges outermost within a patch so that the chunks of one plane are adjacent
-    for (int i0 = 0; i0 < d.ni; i0 += L)
-      for (int c0 = 0; c0 < njk; c0 += MARCH_BS) ctx->mt_nat.push_back(TileMarch{p, i0, std::min(d.ni, i0 + L), c0});
+    lsy = std::max(lsy, psy);
+    lsz = std::max(lsz, psz);
+    march_items(d, p, tb.irange, chunk_of(ctx, ctx->tiling), nat);
+    march_items(d, p, tp.irange, chunk_of(ctx, ctx->tiling_pp), ctx->mt_pp);
   }
-  ctx->mt_first[np] = (int)ctx->mt_nat.size();
+  ctx->mt_first[np] = (int)ctx->mt_pp.size();
   ctx->lds_cells = lc;
   ctx->lds_faces = lf;
-  ctx->n_mt = (int)ctx->mt_nat.size();
-  std::vector<TileMarch> launch = ctx->mt_nat;
-  if (tc.xcd && ctx->n_mt > 0) {
+  ctx->lds_sy = lsy;
+  ctx->lds_sz = lsz;
+  ctx->n_mt = (int)nat.size();
+  std::vector<TileMarch> launch = nat;
+  if (tb.xcd && ctx->n_mt > 0) {
     // workgroup b runs on XCD b % 8: give each XCD one contiguous segment of the natural list
     const int n = ctx->n_mt, seg = (n + 7) / 8;
     std::vector<TileMarch> l;
     l.reserve(n);
     for (int r = 0; r < seg; ++r)
       for (int x = 0; x < 8; ++x)
-        if (x * seg + r < n) l.push_back(ctx->mt_nat[x * seg + r]);
+        if (x * seg + r < n) l.push_back(nat[x * seg + r]);
     launch.swap(l);
   }
-  dfree(ctx->d_mt_nat);
+  dfree(ctx->d_mt_pp);
   dfree(ctx->d_mt_launch);
   dfree(ctx->d_thin_tiles);
-  ctx->d_mt_nat = ctx->d_mt_launch = nullptr;
+  ctx->d_mt_pp = ctx->d_mt_launch = nullptr;
   ctx->d_thin_tiles = nullptr;
-  HIPCHK(ctx, dalloc(&ctx->d_mt_nat, ctx->mt_nat.size()));
+  HIPCHK(ctx, dalloc(&ctx->d_mt_pp, ctx->mt_pp.size()));
   HIPCHK(ctx, dalloc(&ctx->d_mt_launch, launch.size()));
   HIPCHK(ctx, dalloc(&ctx->d_thin_tiles, ctx->thin_tiles.size()));
-  if (!ctx->mt_nat.empty()) {
-    HIPCHK(ctx, hipMemcpy(ctx->d_mt_nat, ctx->mt_nat.data(), ctx->mt_nat.size() * sizeof(TileMarch),
-                          hipMemcpyHostToDevice));
+  if (!ctx->mt_pp.empty())
+    HIPCHK(ctx, hipMemcpy(ctx->d_mt_pp, ctx->mt_pp.data(), ctx->mt_pp.size() * sizeof(TileMarch), hipMemcpyHostToDevice));
+  if (!launch.empty())
     HIPCHK(ctx, hipMemcpy(ctx->d_mt_launch, launch.data(), launch.size() * sizeof(TileMarch), hipMemcpyHostToDevice));
-  }
   if (!ctx->thin_tiles.empty())
     HIPCHK(ctx, hipMemcpy(ctx->d_thin_tiles, ctx->thin_tiles.data(), ctx->thin_tiles.size() * sizeof(TileCell),
                           hipMemcpyHostToDevice));
@@ -477,7 +540,7 @@ void drn_ctx_destroy(drn_ctx* ctx) {
   for (int b = 0; b < NBUF; ++b) dfree(ctx->buf[b]);
   dfree(ctx->d_pd);
   dfree(ctx->d_tiles);
-  dfree(ctx->d_mt_nat);
+  dfree(ctx->d_mt_pp);
   dfree(ctx->d_mt_launch);
   dfree(ctx->d_thin_tiles);
   dfree(ctx->d_part);
@@ -506,6 +569,15 @@ int drn_set_operator(drn_ctx* ctx, int recon, int limiter) {
   if (limiter < DRN_LIM_VANALBADA || limiter > DRN_LIM_SECONDORDER) return fail(ctx, DRN_E_ARG, "unknown limiter");
   ctx->recon = recon;
   ctx->lim = limiter;
+  if (ctx->laid_out) {
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    int rc = build_march(ctx);
+    if (rc) return rc;
+  }
+  if (ctx->gexec) {
+    (void)hipGraphExecDestroy(ctx->gexec);
+    ctx->gexec = nullptr;
+  }
   return DRN_OK;
 }
 
@@ -581,7 +653,6 @@ int drn_layout(drn_ctx* ctx) {
   HIPCHK(ctx, hipMemcpy(ctx->d_tiles, ctx->tiles.data(), ctx->tiles.size() * sizeof(TileCell), hipMemcpyHostToDevice));
   HIPCHK(ctx, dalloc(&ctx->d_part, ctx->tiles.size() * 5));
   HIPCHK(ctx, dalloc(&ctx->d_out, 8));
-  ctx->tiling = 1;
   int rc = build_march(ctx);
   if (rc) return rc;
   ctx->laid_out = true;
@@ -989,17 +1060,19 @@ int drn_diagnostics(drn_ctx* ctx, int field, double dt, double* out) {
   return ctx->prec == DRN_F32 ? diag_t(ctx, (const float*)U, dt, out) : diag_t(ctx, (const double*)U, dt, out);
 }
 
-int drn_set_tiling(drn_ctx* ctx, int choice) {
+int drn_set_tiling(drn_ctx* ctx, int launch, int choice) {
   if (!ctx) return DRN_E_ARG;
   if (choice < 0 || choice >= kNumTilings) return fail(ctx, DRN_E_ARG, "unknown tiling choice");
+  if (launch != DRN_LAUNCH_BATCHED && launch != DRN_LAUNCH_PER_PATCH) return fail(ctx, DRN_E_ARG, "unknown launch form");
   if (!ctx->laid_out) return fail(ctx, DRN_E_STATE, "drn_layout not called");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  const int old = ctx->tiling;
-  ctx->tiling = choice;
+  int& slot = launch == DRN_LAUNCH_BATCHED ? ctx->tiling : ctx->tiling_pp;
+  const int old = slot;
+  slot = choice;
   int rc = build_march(ctx);
   if (rc) {
-    ctx->tiling = old;
+    slot = old;
     build_march(ctx);
     return rc;
   }
@@ -1010,28 +1083,36 @@ int drn_set_tiling(drn_ctx* ctx, int choice) {
   return DRN_OK;
 }
 
+int drn_get_tiling(const drn_ctx* ctx, int launch) {
+  if (!ctx) return DRN_E_ARG;
+  return launch == DRN_LAUNCH_BATCHED ? ctx->tiling : ctx->tiling_pp;
+}
+
 int drn_num_tilings(void) { return kNumTilings; }
 
-// Times every tiling choice on the batched stage (field 0 -> field 2 with field 1
-// as base, which leaves the state untouched) and keeps the fastest.
-int drn_autotune(drn_ctx* ctx, int* choice) {
+// Times every tiling choice on one stage of the given launch form (field 0 ->
+// field 2 with field 1 as base, which leaves fields 0 and 1 untouched) and keeps
+// the fastest.
+int drn_autotune(drn_ctx* ctx, int launch, int* choice) {
   if (!ctx) return DRN_E_ARG;
   if (!ctx->laid_out) return fail(ctx, DRN_E_STATE, "drn_layout not called");
+  if (launch != DRN_LAUNCH_BATCHED && launch != DRN_LAUNCH_PER_PATCH) return fail(ctx, DRN_E_ARG, "unknown launch form");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const bool was_timing = ctx->timing;
   ctx->timing = false;
   hipEvent_t e0, e1;
   HIPCHK(ctx, hipEventCreate(&e0));
   HIPCHK(ctx, hipEventCreate(&e1));
-  int best = ctx->tiling;
+  int best = drn_get_tiling(ctx, launch);
   float best_ms = 1e30f;
-  for (int c = 0; c < kNumTilings; ++c) {
-    if (drn_set_tiling(ctx, c) != DRN_OK) continue;
-    int rc = do_stage(ctx, ctx->map[0], ctx->map[1], ctx->map[2], 0.0, DRN_LAUNCH_BATCHED);  // warm-up
+  int rc = DRN_OK;
+  for (int c = 0; c < kNumTilings && rc == DRN_OK; ++c) {
+    if (drn_set_tiling(ctx, launch, c) != DRN_OK) continue;
+    rc = do_stage(ctx, ctx->map[0], ctx->map[1], ctx->map[2], 0.0, launch);  // warm-up
     if (rc) break;
     HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
-    const int reps = 5;
-    for (int r = 0; r < reps; ++r) do_stage(ctx, ctx->map[0], ctx->map[1], ctx->map[2], 0.0, DRN_LAUNCH_BATCHED);
+    const int reps = 3;
+    for (int r = 0; r < reps && rc == DRN_OK; ++r) rc = do_stage(ctx, ctx->map[0], ctx->map[1], ctx->map[2], 0.0, launch);
     HIPCHK(ctx, hipEventRecord(e1, ctx->stream));
     HIPCHK(ctx, hipEventSynchronize(e1));
     float ms = 0;
@@ -1044,7 +1125,8 @@ int drn_autotune(drn_ctx* ctx, int* choice) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   ctx->timing = was_timing;
-  int rc = drn_set_tiling(ctx, best);
+  if (rc) return rc;
+  rc = drn_set_tiling(ctx, launch, best);
   if (choice) *choice = best;
   return rc;
 }
@@ -1053,14 +1135,17 @@ int drn_set_arith(drn_ctx* ctx, int mode) {
   if (!ctx) return DRN_E_ARG;
   if (mode != DRN_ARITH_IEEE && mode != DRN_ARITH_FAST && mode != -1) return fail(ctx, DRN_E_ARG, "unknown arithmetic mode");
   ctx->arith = mode;
+  if (ctx->laid_out) {
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    int rc = build_march(ctx);
+    if (rc) return rc;
+  }
   if (ctx->gexec) {
     (void)hipGraphExecDestroy(ctx->gexec);
     ctx->gexec = nullptr;
   }
   return DRN_OK;
 }
-
-int drn_get_tiling(const drn_ctx* ctx) { return ctx ? ctx->tiling : DRN_E_ARG; }
 
 int drn_timing_enable(drn_ctx* ctx, int on) {
   if (!ctx) return DRN_E_ARG;

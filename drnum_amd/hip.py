@@ -57,9 +57,9 @@ _SIGS = {
     "drn_rk_graph_run": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_double), C.c_int]),
     "drn_core_sums": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "drn_diagnostics": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double)]),
-    "drn_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
-    "drn_set_tiling": (C.c_int, [C.c_void_p, C.c_int]),
-    "drn_get_tiling": (C.c_int, [C.c_void_p]),
+    "drn_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    "drn_set_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "drn_get_tiling": (C.c_int, [C.c_void_p, C.c_int]),
     "drn_num_tilings": (C.c_int, []),
     "drn_set_arith": (C.c_int, [C.c_void_p, C.c_int]),
     "drn_timing": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
@@ -222,16 +222,16 @@ class Context:
         self._chk(self.L.drn_diagnostics(self.h, field, float(dt), out), "drn_diagnostics")
         return dict(cfl=out[0], rho_min=out[1], rho_max=out[2], nonfinite=out[3])
 
-    def autotune(self):
+    def autotune(self, launch=DRN_LAUNCH_BATCHED):
         c = C.c_int()
-        self._chk(self.L.drn_autotune(self.h, C.byref(c)), "drn_autotune")
+        self._chk(self.L.drn_autotune(self.h, launch, C.byref(c)), "drn_autotune")
         return c.value
 
-    def set_tiling(self, choice):
-        self._chk(self.L.drn_set_tiling(self.h, choice), "drn_set_tiling")
+    def set_tiling(self, choice, launch=DRN_LAUNCH_BATCHED):
+        self._chk(self.L.drn_set_tiling(self.h, launch, choice), "drn_set_tiling")
 
-    def get_tiling(self):
-        return self.L.drn_get_tiling(self.h)
+    def get_tiling(self, launch=DRN_LAUNCH_BATCHED):
+        return self.L.drn_get_tiling(self.h, launch)
 
     def num_tilings(self):
         return self.L.drn_num_tilings()

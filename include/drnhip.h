@@ -168,17 +168,25 @@ int drn_core_sums(drn_ctx* ctx, int field, double* out);
 int drn_diagnostics(drn_ctx* ctx, int field, double dt, double* out);
 
 /* ---- tuning, timing, sync ------------------------------------------------ */
-/* Stage-kernel work-list choices: the marching kernel with an i-range of the
- * whole patch / 32 / 16 / 8 planes per work item, each in natural (patch, plane
- * range, chunk) order or segmented per XCD (workgroup b runs on XCD b % 8, so
- * each XCD gets one contiguous run of neighbouring chunks and shares their
- * halos in its L2), and last the cell-per-thread kernel.  Default: 1. */
+/* Stage-kernel work-list choices, per launch form (drn_num_tilings() of them):
+ * 0-10  the marching kernel (planes staged in LDS; in fast arithmetic the
+ *       slope form) with an i-range of the whole patch / 32 / 16 / 8 / 4 / 2
+ *       planes per work item, in natural (patch, plane range, chunk) order or
+ *       segmented per XCD (workgroup b runs on XCD b % 8, so each XCD gets one
+ *       contiguous run of neighbouring chunks and shares their halos in L2);
+ * 11    the cell-per-thread kernel;
+ * 12-18 the column kernel (fast MUSCL van Albada / minmod only, otherwise the
+ *       marching kernel): no LDS, z faces shared between lanes, with the
+ *       same i-range / order variants;
+ * 19    the cell-per-thread kernel with per-cell limited slopes (fast MUSCL
+ *       van Albada / minmod only, otherwise 11).
+ * Default for both forms: 11; drn_autotune picks per device. */
 int drn_num_tilings(void);
-int drn_set_tiling(drn_ctx* ctx, int choice);
-int drn_get_tiling(const drn_ctx* ctx);
-/* Time every choice on the batched stage of the current state (the state is
- * not modified) and keep the fastest; writes it to *choice if not NULL. */
-int drn_autotune(drn_ctx* ctx, int* choice);
+int drn_set_tiling(drn_ctx* ctx, int launch, int choice);
+int drn_get_tiling(const drn_ctx* ctx, int launch);
+/* Time every choice on one stage of the launch form on the current state (the
+ * state is not modified) and keep the fastest; writes it to *choice if not NULL. */
+int drn_autotune(drn_ctx* ctx, int launch, int* choice);
 /* Arithmetic of the flux kernels: IEEE division and square root, or the fast
  * form (hardware reciprocal, single-division van Albada, exact 0/1 for the
  * supersonic pressure split).  Default: FAST for DRN_F32, IEEE for DRN_F64. */

@@ -176,9 +176,10 @@ def test_every_tiling(gpu, dtype):
         g2 = solver.GpuGrid(w, dtype)
         g2.upload_initial()
         g2.set_tables(tables)
-        g2.ctx.set_tiling(choice)
+        g2.ctx.set_tiling(choice, hip.DRN_LAUNCH_BATCHED)
+        g2.ctx.set_tiling(choice, hip.DRN_LAUNCH_PER_PATCH)
         for s in range(4):
-            g2.rk_step()
+            g2.rk_step(hip.DRN_LAUNCH_BATCHED if s % 2 == 0 else hip.DRN_LAUNCH_PER_PATCH)
         err = rel_linf(g2.state(0), og)
         print("tiling", choice, np.dtype(dtype).name, err, flush=True)
         assert err <= TOL[dtype], (choice, err)
@@ -203,7 +204,9 @@ def test_autotune_keeps_state(gpu):
     w = configs.jet(core=8, blocks=2)
     og, gg = make(w, np.float32)
     before = gg.state(0)
-    choice = gg.ctx.autotune()
+    choice = gg.ctx.autotune(hip.DRN_LAUNCH_BATCHED)
+    assert 0 <= choice < gg.ctx.num_tilings()
+    choice = gg.ctx.autotune(hip.DRN_LAUNCH_PER_PATCH)
     assert 0 <= choice < gg.ctx.num_tilings()
     after = gg.state(0)
     for a, b in zip(before, after):
