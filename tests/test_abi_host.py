@@ -1,0 +1,146 @@
+"""CPU checks of the C ABI and the host-side logic (no GPU calls).
+
+  * both libraries load and export every function that include/*.h declares,
+    and the Python bindings bind exactly that set;
+  * the host grid reader parses the reference's grid-file format
+    (patchgrid.cpp:479-561, the writer in src/Python/drnum.py:199-244) into
+    the same patches, and the same donor tables, as the programmatic set-up;
+  * error paths return codes and messages instead of stopping the process;
+  * the benchmark workload has the stated size.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from drnum_amd import configs, hip, host
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    return sorted(set(re.findall(r"\b(drnh?_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,lib_path,bound", [
+    ("drnhip.h", hip.LIB_PATH, lambda: hip._SIGS),
+    ("drnum_host.h", host.LIB_PATH, lambda: host._SIGS),
+])
+def test_library_exports_every_declared_symbol(header, lib_path, bound):
+    names = declared(header)
+    assert len(names) > 20
+    L = C.CDLL(lib_path)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert sorted(bound()) == names
+
+
+def test_hip_library_is_a_gfx950_code_object_host():
+    """The HIP library carries device code for gfx950 (offload bundle section)."""
+    data = open(hip.LIB_PATH, "rb").read()
+    assert b"gfx950" in data
+
+
+def grid_file_text(w):
+    """A grid file as the reference's Python writer lays it out (drnum.py:199-244)."""
+    codes = {configs.BC_FARFIELD: "far", configs.BC_SLIP: "slip", configs.BC_TRANSMISSIVE: "trans",
+             configs.BC_JET: "jet", configs.BC_NONE: "none"}
+    out = []
+    for n, P in enumerate(w.patches):
+        txt = "1001 // index=%d name='p%d'\n{\n" % (n, n)
+        txt += "  %r %r %r\n" % tuple(P.origin)
+        txt += "  1 0 0\n  0 1 0\n  1\n"
+        txt += "  %d %d %d\n" % (P.ni, P.nj, P.nk)
+        txt += "  " + " ".join(str(s) for s in P.seek) + "\n"
+        txt += "  %r %r %r\n" % tuple(P.length)
+        txt += "  fx fy fz\n"
+        txt += "  " + " ".join("0" if P.seek[f] else codes[P.bc[f]] for f in range(6)) + "\n"
+        txt += "  0\n}\n"
+        out.append(txt)
+    out.append("0\n")
+    return "".join(out)
+
+
+def test_grid_reader_matches_programmatic_setup(tmp_path):
+    w = configs.jet(core=6, blocks=2)
+    path = tmp_path / "grid"
+    path.write_text(grid_file_text(w))
+    a = host.HostGrid(True)
+    a.read(str(path))
+    b = host.HostGrid.from_workload(w)
+    assert a.num_patches() == len(w.patches)
+    for p, P in enumerate(w.patches):
+        ia, ib = a.patch_info(p), b.patch_info(p)
+        assert ia == ib
+        assert ia["n"] == [P.ni, P.nj, P.nk] and ia["seek"] == list(P.seek)
+    a.compute_dependencies()
+    b.compute_dependencies()
+    for p in range(len(w.patches)):
+        ta, tb = a.tables(p), b.tables(p)
+        for k in ("receivers", "rec_concat", "donor_index", "donor_weight"):
+            np.testing.assert_array_equal(ta[k], tb[k])
+        assert ta["donors"] == tb["donors"]
+
+
+def test_grid_reader_scale(tmp_path):
+    w = configs.jet(core=6, blocks=2)
+    path = tmp_path / "grid"
+    path.write_text(grid_file_text(w))
+    a = host.HostGrid(True)
+    a.read(str(path), 2.0)
+    info = a.patch_info(1)
+    P = w.patches[1]
+    np.testing.assert_allclose(info["origin"], 2.0 * np.array(P.origin), rtol=1e-15)
+    np.testing.assert_allclose(info["length"], 2.0 * np.array(P.length), rtol=1e-15)
+
+
+def test_grid_reader_errors_are_reported(tmp_path):
+    g = host.HostGrid(True)
+    with pytest.raises(RuntimeError, match="cannot open"):
+        g.read(str(tmp_path / "nope"))
+    bad = tmp_path / "rot"
+    bad.write_text("1001 { 0 0 0  0 1 0  1 0 0  1  4 4 4  0 0 0 0 0 0  1 1 1 }\n0\n")
+    with pytest.raises(RuntimeError, match="axis-aligned"):
+        g.read(str(bad))
+    bad2 = tmp_path / "type"
+    bad2.write_text("1002 { }\n0\n")
+    with pytest.raises(RuntimeError, match="unsupported patch type"):
+        g.read(str(bad2))
+
+
+def test_host_rejects_bad_patch_ids():
+    g = host.HostGrid.from_workload(configs.jet(core=6, blocks=2))
+    info = np.zeros(9, np.int32)
+    od = np.zeros(9, np.float64)
+    assert g.L.drnh_grid_patch_info(g.h, 99, info.ctypes.data_as(C.POINTER(C.c_int32)),
+                                    od.ctypes.data_as(C.POINTER(C.c_double))) < 0
+    assert g.L.drnh_grid_patch_info(g.h, -1, info.ctypes.data_as(C.POINTER(C.c_int32)),
+                                    od.ctypes.data_as(C.POINTER(C.c_double))) < 0
+
+
+def test_min_ch_length():
+    w = configs.jet(core=6, blocks=2)
+    g = host.HostGrid.from_workload(w)
+    assert g.min_ch_length() == pytest.approx(1.0 / 12, rel=1e-15)
+
+
+def test_jet_benchmark_workload_sizes():
+    w = configs.jet(core=64, blocks=4)
+    assert len(w.patches) == 64
+    assert w.total_cells == 19_248_832
+    assert w.core_cells == 16_777_216
+    # receivers per exchange: every stored cell outside the cores
+    assert w.total_cells - w.core_cells == 2_471_616
+    dims = sorted({(P.ni, P.nj, P.nk) for P in w.patches})
+    assert {d for dd in dims for d in dd} == {66, 68}
+
+
+def test_periodic_box_every_patch_has_seek_layers():
+    w = configs.kelvin_helmholtz(core=8, blocks=2)
+    for P in w.patches:
+        assert P.seek == [2] * 6 and (P.ni, P.nj, P.nk) == (12, 12, 12)

@@ -1,0 +1,257 @@
+"""Pins the donor tables and the exchange (CPU only).
+
+Oracle tables (cartesianpatch.cpp:1053-1299, patch.cpp:128-486,
+patchgrid.cpp:119-259) against brute force on the geometry:
+
+  * aligned block grids (and periodic ones): every seek cell is a receiver
+    with exactly one donor cell, weight exactly 1, at the same global index;
+  * ratio-2 nesting: dyadic trilinear weights {27,9,3,1}/64 exactly;
+  * ratio-3 nesting: weights sum to 1 and reproduce linear fields;
+  * doubly covered receivers: weights divided by the hit count
+    (intercoeffws.h:233-242);
+  * the exchange reproduces linear fields exactly where the weights are exact.
+
+And the product's host-side table builder (libdrnum.so, C++) against the
+oracle, bit for bit, in FP64 and FP32 geometry.
+"""
+import numpy as np
+import pytest
+
+from drnum_amd import configs
+from oracle import oracle as O
+
+from refsol import core_mask, flat, ijk, seek_cells
+
+FAR = configs.BC_FARFIELD
+
+
+def patch(n, seek, origin, h):
+    return configs.Patch(n[0], n[1], n[2], list(seek), list(origin), [n[a] * h[a] for a in range(3)], [FAR] * 6,
+                         np.zeros((6, configs.NPAR)), [0, 0, 0])
+
+
+def workload(patches, periodic=(0.0, 0.0, 0.0)):
+    def init(x, y, z):
+        one = np.ones_like(x)
+        return np.stack([one, 0.003 * one, 0 * x, 0 * x, 0 * x])
+    return configs.Workload("t", patches, configs.RECON_UPWIND2, configs.LIM_VANALBADA, list(periodic),
+                            [0.25, 0.5, 1.0], 1e-3, init)
+
+
+def nested(ratio, coarse_n=16, lo=4, core=8, seek=2, shift=(0.0, 0.0, 0.0)):
+    """A coarse patch without seek layers and a patch refined by `ratio`
+    covering coarse cells [lo, lo+core) (+ shift coarse cells) with `seek`
+    seek layers all round."""
+    h = 1.0 / coarse_n
+    hf = h / ratio
+    Q = patch((coarse_n,) * 3, [0] * 6, (0.0, 0.0, 0.0), (h,) * 3)
+    nf = core * ratio + 2 * seek
+    org = [(lo + shift[a]) * h - seek * hf for a in range(3)]
+    P = patch((nf,) * 3, [seek] * 6, org, (hf,) * 3)
+    return workload([Q, P])
+
+
+def oracle_tables(w, dtype=np.float64):
+    g = O.OracleGrid(w, dtype)
+    g.compute_dependencies()
+    return g, [g.tables(p) for p in range(len(w.patches))]
+
+
+def receiver_weights(T):
+    """{receiver cell: [(donor patch, donor cell, weight), ...]} from the concatenated lists."""
+    out = {}
+    for (q, nrec, stride, r0, w0) in T["donors"]:
+        for e in range(nrec):
+            rec = int(T["rec_concat"][r0 + e])
+            for s in range(stride):
+                out.setdefault(rec, []).append((q, int(T["donor_index"][w0 + e * stride + s]),
+                                                float(T["donor_weight"][w0 + e * stride + s])))
+    return out
+
+
+def global_index(P, cells, h):
+    i, j, k = ijk(P, cells)
+    g0 = [int(round(P.origin[a] / h)) for a in range(3)]
+    return np.stack([i + g0[0], j + g0[1], k + g0[2]])
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_aligned_blocks_copy_one_donor_cell(periodic):
+    w = (configs.kelvin_helmholtz(core=6, blocks=2) if periodic else configs.jet(core=6, blocks=2))
+    nc = 12
+    h = 1.0 / nc
+    g, tabs = oracle_tables(w)
+    for p, (P, T) in enumerate(zip(w.patches, tabs)):
+        want_rec = seek_cells(P)
+        np.testing.assert_array_equal(T["receivers"], want_rec)
+        assert T["no_donor"] == 0
+        rw = receiver_weights(T)
+        assert sorted(rw) == want_rec.tolist()
+        for rec, lst in rw.items():
+            live = [(q, c, wt) for (q, c, wt) in lst if wt != 0.0]
+            assert len(live) == 1 and live[0][2] == 1.0, (p, rec, lst)
+            q, c, _ = live[0]
+            Q = w.patches[q]
+            assert core_mask(Q)[c], "donor cell must lie in the donor zone"
+            gr = global_index(P, [rec], h)[:, 0]
+            gd = global_index(Q, [c], h)[:, 0]
+            if periodic:
+                np.testing.assert_array_equal(gr % nc, gd % nc)
+            else:
+                np.testing.assert_array_equal(gr, gd)
+
+
+def trilinear_expected(Pf, rec, Q):
+    """Brute-force trilinear weights of a receiver's centre in the coarse patch Q."""
+    i, j, k = ijk(Pf, [rec])
+    x = [Pf.origin[a] + ([i, j, k][a][0] + 0.5) * Pf.dx(a) for a in range(3)]
+    ax = []
+    for a in range(3):
+        t = (x[a] - Q.origin[a]) / Q.dx(a) - 0.5
+        i0 = int(np.floor(t))
+        f = t - i0
+        ax.append(((i0, 1.0 - f), (i0 + 1, f)))
+    out = {}
+    for (ia, wa) in ax[0]:
+        for (jb, wb) in ax[1]:
+            for (kc, wc) in ax[2]:
+                wt = wa * wb * wc
+                if wt != 0.0:
+                    c = int(flat(Q, ia, jb, kc))
+                    out[c] = out.get(c, 0.0) + wt
+    return out
+
+
+def test_ratio2_nesting_gives_dyadic_weights():
+    w = nested(2)
+    g, tabs = oracle_tables(w)
+    Q, Pf = w.patches
+    assert len(tabs[0]["receivers"]) == 0
+    T = tabs[1]
+    np.testing.assert_array_equal(T["receivers"], seek_cells(Pf))
+    rw = receiver_weights(T)
+    assert len(rw) == len(seek_cells(Pf))
+    allowed = {27 / 64, 9 / 64, 3 / 64, 1 / 64}
+    for rec, lst in rw.items():
+        got = {}
+        for (q, c, wt) in lst:
+            assert q == 0
+            if wt != 0.0:
+                got[c] = got.get(c, 0.0) + wt
+        want = trilinear_expected(Pf, rec, Q)
+        assert got == want, (rec, got, want)
+        assert set(got.values()) <= allowed
+
+
+@pytest.mark.parametrize("ratio,shift", [(3, (0.0, 0.0, 0.0)), (2, (0.3, 0.1, 0.45))])
+def test_weights_sum_to_one_and_reproduce_linear_fields(ratio, shift):
+    w = nested(ratio, shift=shift)
+    g, tabs = oracle_tables(w)
+    Q, Pf = w.patches
+    rw = receiver_weights(tabs[1])
+    assert len(rw) == len(seek_cells(Pf))
+    xq = np.stack(w.cell_centres(0))
+    xp = np.stack(w.cell_centres(1))
+    coef = np.array([0.3, -1.7, 2.9])
+    for rec, lst in rw.items():
+        s = sum(wt for (_, _, wt) in lst)
+        assert abs(s - 1.0) <= 8 * np.finfo(np.float64).eps
+        # the brute-force weights (before elimination at 1e-4 none are that small here)
+        want = trilinear_expected(Pf, rec, Q)
+        got = {}
+        for (_, c, wt) in lst:
+            if wt != 0.0:
+                got[c] = got.get(c, 0.0) + wt
+        assert got.keys() == want.keys()
+        for c in want:
+            assert abs(got[c] - want[c]) <= 1e-13
+        lin = sum(wt * (coef @ xq[:, c]) for (_, c, wt) in lst)
+        assert abs(lin - coef @ xp[:, rec]) <= 1e-13
+
+
+def test_exchange_reproduces_linear_fields():
+    w = nested(2)
+    g = O.OracleGrid(w, np.float64)
+    g.compute_dependencies()
+    coef = np.array([[1.0, 0.2, -0.1, 0.3], [0.5, 1.0, 2.0, -1.0], [0.0, -0.5, 0.25, 0.75],
+                     [2.0, 0.1, 0.1, 0.1], [3.0, -2.0, 1.0, 0.5]])
+    for p in range(2):
+        X = np.stack(w.cell_centres(p))
+        g.field(p, 0)[:] = coef[:, :1] + coef[:, 1:] @ X
+    g.field(1, 0)[:, seek_cells(w.patches[1])] = -99.0
+    g.exchange(0)
+    X = np.stack(w.cell_centres(1))
+    want = coef[:, :1] + coef[:, 1:] @ X
+    np.testing.assert_allclose(g.field(1, 0), want, rtol=0, atol=1e-14)
+
+
+def test_double_cover_averages_the_donors():
+    """Two identical donor patches covering the receivers: each receiver has
+    two donors with weight 1/2 (adjust2Average, intercoeffws.h:233-242)."""
+    h = 1.0 / 16
+    Q1 = patch((16, 16, 16), [0] * 6, (0.0, 0.0, 0.0), (h,) * 3)
+    Q2 = patch((16, 16, 16), [0] * 6, (0.0, 0.0, 0.0), (h,) * 3)
+    P = patch((10, 10, 10), [2] * 6, (3 * h, 3 * h, 3 * h), (h,) * 3)
+    w = workload([Q1, Q2, P])
+    g, tabs = oracle_tables(w)
+    T = tabs[2]
+    assert [d[0] for d in T["donors"]] == [0, 1]
+    rw = receiver_weights(T)
+    for rec, lst in rw.items():
+        live = [(q, c, wt) for (q, c, wt) in lst if wt != 0.0]
+        assert sorted(q for q, _, _ in live) == [0, 1]
+        assert all(wt == 0.5 for _, _, wt in live)
+        gr = global_index(P, [rec], h)[:, 0]
+        for q, c, _ in live:
+            np.testing.assert_array_equal(global_index(w.patches[q], [c], h)[:, 0], gr)
+
+
+def test_seek_cells_without_donor_are_dropped():
+    """Seek cells outside every donor zone are removed from the receive list
+    (patch.cpp:319-465) and counted."""
+    h = 1.0 / 16
+    Q = patch((8, 16, 16), [0] * 6, (0.0, 0.0, 0.0), (h,) * 3)
+    P = patch((8, 8, 8), [2] * 6, (6 * h, 4 * h, 4 * h), (h,) * 3)   # x-range half outside Q
+    w = workload([Q, P])
+    g, tabs = oracle_tables(w)
+    T = tabs[1]
+    seek = seek_cells(P)
+    i, _, _ = ijk(P, seek)
+    inside = seek[(i + 6) < 8]
+    np.testing.assert_array_equal(T["receivers"], inside)
+    assert T["no_donor"] == len(seek) - len(inside)
+
+
+# ---------------------------------------------------------------------------
+# the product's host-side builder against the oracle
+# ---------------------------------------------------------------------------
+
+def _host_cases():
+    return {
+        "jet": configs.jet(core=6, blocks=2),
+        "kh": configs.kelvin_helmholtz(core=6, blocks=2),
+        "nest2": nested(2),
+        "nest3": nested(3),
+        "shift": nested(2, shift=(0.3, 0.1, 0.45)),
+        "uneven": configs.uneven_batch(n_patches=12, size_range=(8, 14)),
+    }
+
+
+@pytest.mark.parametrize("case", list(_host_cases()))
+@pytest.mark.parametrize("fp64", [True, False])
+def test_host_tables_match_oracle_bitwise(case, fp64):
+    from drnum_amd import host
+
+    w = _host_cases()[case]
+    dtype = np.float64 if fp64 else np.float32
+    g, tabs = oracle_tables(w, dtype)
+    hg = host.HostGrid.from_workload(w, fp64_geometry=fp64)
+    hg.compute_dependencies(w.periodic, threads=4)
+    for p, T in enumerate(tabs):
+        H = hg.tables(p)
+        np.testing.assert_array_equal(H["receivers"], T["receivers"])
+        assert H["donors"] == T["donors"]
+        assert H["no_donor"] == T["no_donor"]
+        np.testing.assert_array_equal(H["rec_concat"], T["rec_concat"])
+        np.testing.assert_array_equal(H["donor_index"], T["donor_index"])
+        np.testing.assert_array_equal(H["donor_weight"].astype(dtype), T["donor_weight"])
