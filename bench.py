@@ -5,7 +5,8 @@ Workload: 256^3 core cells as 4x4x4 patches of 64^3 core cells with two seek
 layers on every interior patch face (patches stored as 66 or 68 cells per axis,
 19,248,832 stored cells), AUSM+ with MUSCL/van Albada, 3-stage RK
 (alpha = 1/4, 1/2, 1), exchange after every stage, top-hat Mach-0.9 inflow on
-the z=0 face, transmissive elsewhere.  A "step" is one RK step over all patches.
+the z=0 face (Mach-0.1 co-flow around it), transmissive elsewhere.  A "step"
+is one RK step over all patches.
 
   value       FP32 batched form (one launch per stage over the tile work-list of
               all 64 patches), state resident in HBM, timed region = K steps.

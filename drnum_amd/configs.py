@@ -161,12 +161,16 @@ T_AMB = P_AMB / (1.0 * gas.R)   # rho = 1, so a = sqrt(1.4 R T) = 1
 
 
 def jet(core: int = 64, blocks: int = 4, steps: int = 1000, mach: float = 0.9, radius_cells: float = 16.0,
-        noise: float = 0.01, seed: int = 1234) -> Workload:
+        noise: float = 0.01, seed: int = 1234, coflow: float = 0.1) -> Workload:
     """Subsonic top-hat jet entering through the z=0 face of a (core*blocks)^3 box.
     Ambient rho=1, p=1/1.4 (a=1) at rest; inflow |w| = mach inside radius_cells
     of the face centre with seeded +/-noise velocity perturbation (fixed in
-    time per inflow face cell); all other domain faces transmissive; 2 seek
-    layers on interior patch faces."""
+    time per inflow face cell); the rest of the inflow face carries a co-flow
+    of speed `coflow` (the u_far of the reference's JetFlux::xWallM,
+    jetflux.h:180-185); all other domain faces transmissive; 2 seek layers on
+    interior patch faces.  With coflow = 0 the rest of the face is a wall with
+    zero normal velocity, and the cells beside the nozzle lip are drained to
+    negative density within a few dozen steps of the impulsive start."""
     nc = core * blocks
     h = 1.0 / nc
     dom_bc = [BC_TRANSMISSIVE] * 6
@@ -180,7 +184,7 @@ def jet(core: int = 64, blocks: int = 4, steps: int = 1000, mach: float = 0.9, r
             par[2] = radius_cells * h
             par[3] = mach * 1.0                    # a = 1
             par[4] = T_AMB
-            par[5] = 0.0
+            par[5] = coflow
             par[6] = T_AMB
             par[7] = noise
             par[8] = float(seed)

@@ -30,6 +30,28 @@ auto with(const drnh_grid* g, F&& f) {
 
 bool bad_patch(const drnh_grid* g, int p) { return !g || p < 0 || p >= drnh_grid_num_patches(g); }
 
+// PerfectGas::primitiveToConservative (perfectgas.h:56-64) in the state
+// precision T; 0.5 is a double literal there, so the energy term is formed in
+// double and rounded once.
+template <typename T>
+void prim_to_cons(T p, T temp, T u, T v, T w, T* var) {
+  var[0] = p / (T(287.2) * temp);
+  var[1] = var[0] * u;
+  var[2] = var[0] * v;
+  var[3] = var[0] * w;
+  var[4] = T(var[0] * (T(717.5) * temp + 0.5 * (u * u + v * v + w * w)));
+}
+
+// The farfield functor keeps its state in conserved variables
+// (CompressibleFarfieldFlux::setFarfield, compressiblefarfieldflux.h:74-77);
+// the host keeps the primitive p, T, u, v, w the caller gave.
+template <typename T>
+void farfield_to_conserved(const double* prim, double* out) {
+  T var[5];
+  prim_to_cons<T>(T(prim[0]), T(prim[1]), T(prim[2]), T(prim[3]), T(prim[4]), var);
+  for (int q = 0; q < 5; ++q) out[q] = (double)var[q];
+}
+
 }  // namespace
 
 drnh_grid* drnh_grid_create(int fp64_geometry) {
@@ -266,6 +288,12 @@ int drnh_grid_to_device(drnh_grid* g, drn_ctx* ctx) {
       for (int f = 0; f < 6; ++f) {
         d.bc[f] = P.face[f];
         for (int q = 0; q < 16; ++q) d.bc_par[f][q] = P.face_par[f][q];
+        if (P.face[f] == FACE_FARFIELD) {
+          if (prec == DRN_F32)
+            farfield_to_conserved<float>(P.face_par[f], d.bc_par[f]);
+          else
+            farfield_to_conserved<double>(P.face_par[f], d.bc_par[f]);
+        }
       }
       int id = -1;
       int rc = drn_patch_add(ctx, &d, &id);

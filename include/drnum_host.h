@@ -40,7 +40,10 @@ int drnh_grid_read(drnh_grid* g, const char* path, double scale);
 int drnh_grid_num_patches(const drnh_grid* g);
 /* out_i[9] = ni, nj, nk, seek[6]; out_d[9] = origin[3], length[3], dx, dy, dz */
 int drnh_grid_patch_info(const drnh_grid* g, int p, int32_t* out_i, double* out_d);
-/* face functors of a patch (DRN_BC_* and DRN_BC_NPAR parameters per face) */
+/* face functors of a patch (DRN_BC_* and DRN_BC_NPAR parameters per face).
+ * A farfield face takes the primitive state p, T, u, v, w in par[0..4], as
+ * CompressibleFarfieldFlux::setFarfield does (compressiblefarfieldflux.h:74-77);
+ * drnh_grid_to_device converts it to conserved variables in the state precision. */
 int drnh_grid_set_faces(drnh_grid* g, int p, const int32_t* bc6, const double* par6x16);
 
 /* PatchGrid::computeDependencies; period[3] > 0 adds periodic images per axis
