@@ -61,16 +61,20 @@ def stage_bytes_per_step(cells, esize, n_stages):
     return cells * vals * esize
 
 
-def run_form(w, dtype, steps, warmup, launch, label):
+def run_form(w, dtype, steps, warmup, launch, label, tiling=None):
     from drnum_amd import hip
 
     hg, ctx = setup(w, dtype)
     cells = w.total_cells
     esize = np.dtype(dtype).itemsize
     S = len(w.alpha)
-    # the library's autotuner picks the stage work-list (i-range, XCD order, kernel) for this form
-    tiling = ctx.autotune(launch)
-    log("%s: autotuned tiling %d" % (label, tiling))
+    if tiling is None:
+        # the library's autotuner picks the stage work-list (i-range, XCD order, kernel) for this form
+        tiling = ctx.autotune(launch)
+        log("%s: autotuned tiling %d" % (label, tiling))
+    else:
+        ctx.set_tiling(tiling, launch)
+        log("%s: tiling %d" % (label, tiling))
     for _ in range(warmup):
         ctx.rk_step(w.dt, w.alpha, launch)
     ctx.sync()
@@ -125,6 +129,27 @@ def run_form(w, dtype, steps, warmup, launch, label):
     return res
 
 
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "stage_traffic_jet4x64_fp32.md")
+
+
+def measured_traffic(tiling):
+    """HBM bytes per stage launch of the FP32 stage kernel for a work-list choice,
+    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload
+    (counters scaled by the copy-kernel calibration).  None when not measured."""
+    try:
+        lines = open(TRAFFIC_TABLE).read().splitlines()
+    except OSError:
+        return None
+    for ln in lines:
+        cols = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if len(cols) == 8 and cols[0].isdigit() and int(cols[0]) == tiling and cols[1].startswith("k_stage"):
+            try:
+                return float(cols[6])
+            except ValueError:
+                return None
+    return None
+
+
 def ctx_vstride(ctx, w):
     # cells per variable block: patches padded to 64 cells
     return sum((p.cells + 63) // 64 * 64 for p in w.patches)
@@ -154,6 +179,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--blocks", type=int, default=4)
     ap.add_argument("--core", type=int, default=64)
+    ap.add_argument("--tiling", type=int, default=None, help="fixed stage work-list choice (default: autotune)")
+    ap.add_argument("--fp32-only", action="store_true", help="skip the FP64 per-patch form (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (patch sharding across GPUs is "
@@ -165,9 +192,11 @@ def main():
     w = configs.jet(core=args.core, blocks=args.blocks)
     log("workload %s: %d patches, %d stored cells, %d core cells" % (w.name, len(w.patches), w.total_cells,
                                                                       w.core_cells))
-    f32 = run_form(w, np.float32, args.steps, args.warmup, hip.DRN_LAUNCH_BATCHED, "fp32-batched")
-    f64 = run_form(w, np.float64, args.fp64_steps or args.steps, args.warmup, hip.DRN_LAUNCH_PER_PATCH,
-                   "fp64-plain")
+    f32 = run_form(w, np.float32, args.steps, args.warmup, hip.DRN_LAUNCH_BATCHED, "fp32-batched", args.tiling)
+    f64 = None
+    if not args.fp32_only:
+        f64 = run_form(w, np.float64, args.fp64_steps or args.steps, args.warmup, hip.DRN_LAUNCH_PER_PATCH,
+                       "fp64-plain", args.tiling)
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -175,6 +204,8 @@ def main():
         log("cpu baseline", cpu)
 
     ach = f32["stage_gbs"]
+    default_workload = (args.core, args.blocks) == (64, 4)
+    traffic = measured_traffic(f32["tiling"]) if default_workload else None
     out = {
         "metric": METRIC,
         "value": round(f32["mcups"], 2),
@@ -204,9 +235,12 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": ("rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/stage_traffic_jet4x64_fp32.md, tiling %d"
+                               % f32["tiling"]) if traffic else None,
+            "algorithmic_bytes_per_launch": round(f32["stage_bytes_per_launch"]),
         },
-        "fp64_plain": {
+        "fp64_plain": None if f64 is None else {
             "value": round(f64["mcups"], 2),
             "ms_per_step": round(f64["ms_per_step"], 4),
             "stage_kernel_gbs": round(f64["stage_gbs"], 1),

@@ -56,6 +56,8 @@ struct drn_ctx {
   std::vector<TileCell> tiles;
   std::vector<int> tile_first;  // per patch, size np+1
   TileCell* d_tiles = nullptr;
+  TileCell* d_tiles_xcd = nullptr;     // the list in 8 contiguous segments, one per XCD
+  TileCell* d_tiles_xcd_pp = nullptr;  // the same per patch (same tile_first offsets)
   double* d_part = nullptr;  // reduction partials (5 per tile)
   double* d_out = nullptr;
   int tiling = 11;     // batched form
@@ -171,7 +173,8 @@ static const TilingChoice kTilings[] = {
     {0, 0, K_MARCH},  {0, 1, K_MARCH},  {32, 0, K_MARCH},  {32, 1, K_MARCH}, {16, 0, K_MARCH},
     {16, 1, K_MARCH}, {8, 0, K_MARCH},  {8, 1, K_MARCH},   {4, 0, K_MARCH},  {4, 1, K_MARCH},
     {2, 0, K_MARCH},  {0, 0, K_CELL},   {0, 0, K_COLUMN},  {0, 1, K_COLUMN}, {32, 1, K_COLUMN},
-    {16, 0, K_COLUMN}, {16, 1, K_COLUMN}, {8, 1, K_COLUMN}, {4, 1, K_COLUMN}, {0, 0, K_CELL_SLOPE}};
+    {16, 0, K_COLUMN}, {16, 1, K_COLUMN}, {8, 1, K_COLUMN}, {4, 1, K_COLUMN}, {0, 0, K_CELL_SLOPE},
+    {0, 1, K_CELL},    {0, 1, K_CELL_SLOPE}};
 static const int kNumTilings = sizeof(kTilings) / sizeof(kTilings[0]);
 
 // the column kernel covers the fast MUSCL forms with a symmetric limited slope
@@ -275,7 +278,7 @@ static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int l
   if (launch == DRN_LAUNCH_BATCHED) {
     const int kind = effective_kind(ctx, ctx->tiling);
     if (kind == K_CELL || kind == K_CELL_SLOPE) {
-      go(kind, ctx->d_tiles, (int)ctx->tiles.size());
+      go(kind, kTilings[ctx->tiling].xcd ? ctx->d_tiles_xcd : ctx->d_tiles, (int)ctx->tiles.size());
     } else {
       go(kind, ctx->d_mt_launch, ctx->n_mt);
       go(K_CELL, ctx->d_thin_tiles, (int)ctx->thin_tiles.size());
@@ -287,7 +290,8 @@ static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int l
       const int m0 = ctx->mt_first[p], mn = ctx->mt_first[p + 1] - m0;
       if (kind == K_CELL || kind == K_CELL_SLOPE || mn == 0) {
         const int t0 = ctx->tile_first[p];
-        go(kind == K_CELL_SLOPE ? K_CELL_SLOPE : K_CELL, ctx->d_tiles + t0, ctx->tile_first[p + 1] - t0);
+        const TileCell* tl = (kTilings[ctx->tiling_pp].xcd ? ctx->d_tiles_xcd_pp : ctx->d_tiles) + t0;
+        go(kind == K_CELL_SLOPE ? K_CELL_SLOPE : K_CELL, tl, ctx->tile_first[p + 1] - t0);
       } else {
         go(kind, ctx->d_mt_pp + m0, mn);
       }
@@ -540,6 +544,8 @@ void drn_ctx_destroy(drn_ctx* ctx) {
   for (int b = 0; b < NBUF; ++b) dfree(ctx->buf[b]);
   dfree(ctx->d_pd);
   dfree(ctx->d_tiles);
+  dfree(ctx->d_tiles_xcd);
+  dfree(ctx->d_tiles_xcd_pp);
   dfree(ctx->d_mt_pp);
   dfree(ctx->d_mt_launch);
   dfree(ctx->d_thin_tiles);
@@ -651,6 +657,29 @@ int drn_layout(drn_ctx* ctx) {
   HIPCHK(ctx, hipMemcpy(ctx->d_pd, pd.data(), np * sizeof(PatchDev), hipMemcpyHostToDevice));
   HIPCHK(ctx, dalloc(&ctx->d_tiles, ctx->tiles.size()));
   HIPCHK(ctx, hipMemcpy(ctx->d_tiles, ctx->tiles.data(), ctx->tiles.size() * sizeof(TileCell), hipMemcpyHostToDevice));
+  {
+    // XCD-segmented orders of the cell list: workgroup b runs on XCD b % 8, so
+    // launch slot r*8 + x takes item x*seg + r of the natural list and each XCD
+    // walks one contiguous run of planes (its L2 then holds the i+-2 halo planes
+    // its own workgroups need)
+    auto segment = [](const TileCell* nat, int n, TileCell* out) {
+      const int seg = (n + 7) / 8;
+      int m = 0;
+      for (int r = 0; r < seg; ++r)
+        for (int x = 0; x < 8; ++x)
+          if (x * seg + r < n) out[m++] = nat[x * seg + r];
+    };
+    const int nt = (int)ctx->tiles.size();
+    std::vector<TileCell> all(nt), pp(nt);
+    segment(ctx->tiles.data(), nt, all.data());
+    for (int p = 0; p < np; ++p)
+      segment(ctx->tiles.data() + ctx->tile_first[p], ctx->tile_first[p + 1] - ctx->tile_first[p],
+              pp.data() + ctx->tile_first[p]);
+    HIPCHK(ctx, dalloc(&ctx->d_tiles_xcd, nt));
+    HIPCHK(ctx, dalloc(&ctx->d_tiles_xcd_pp, nt));
+    HIPCHK(ctx, hipMemcpy(ctx->d_tiles_xcd, all.data(), nt * sizeof(TileCell), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->d_tiles_xcd_pp, pp.data(), nt * sizeof(TileCell), hipMemcpyHostToDevice));
+  }
   HIPCHK(ctx, dalloc(&ctx->d_part, ctx->tiles.size() * 5));
   HIPCHK(ctx, dalloc(&ctx->d_out, 8));
   int rc = build_march(ctx);

@@ -179,7 +179,9 @@ int drn_diagnostics(drn_ctx* ctx, int field, double dt, double* out);
  *       marching kernel): no LDS, z faces shared between lanes, with the
  *       same i-range / order variants;
  * 19    the cell-per-thread kernel with per-cell limited slopes (fast MUSCL
- *       van Albada / minmod only, otherwise 11).
+ *       van Albada / minmod only, otherwise 11);
+ * 20    as 11, work list in 8 contiguous segments, one per XCD;
+ * 21    as 19, work list in 8 contiguous segments, one per XCD.
  * Default for both forms: 11; drn_autotune picks per device. */
 int drn_num_tilings(void);
 int drn_set_tiling(drn_ctx* ctx, int launch, int choice);

@@ -144,3 +144,14 @@ def test_periodic_box_every_patch_has_seek_layers():
     w = configs.kelvin_helmholtz(core=8, blocks=2)
     for P in w.patches:
         assert P.seek == [2] * 6 and (P.ni, P.nj, P.nk) == (12, 12, 12)
+
+
+def test_bench_reads_the_committed_traffic_figure():
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    b = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(b)
+    t = b.measured_traffic(20)
+    assert t is not None and 0.9e9 < t < 1.2e9
+    assert b.measured_traffic(999) is None
