@@ -1,0 +1,139 @@
+// drnum_run.cpp -- a DrNUM-style application on the HIP drop-in.
+//
+// Written the way the reference's demo mains drive the path
+// (applications/drnumJetDemo/main.h:90-240): read a grid file, compute the
+// donor dependencies, set the initial field, build the iterator for the flux
+// operator AusmPlus<5, Upwind2<5, VanAlbada>, PerfectGas>, add the RK
+// coefficients, run the time steps, copy the result back to the host.
+//
+// usage: drnum_run <grid file> <case file> <out file> <steps> <dt> <mode> [f64] [levelset]
+//   case file (binary, little endian), per patch in grid order:
+//     int32 bc[6], float64 par[6][16], then 5*N initial values in `real`
+//     (variable-major, the layout of field 0 in Patch::m_Data);
+//     with `levelset`, followed by int32 n_spheres, int32 n_inner,
+//     float64 min_inner_rel and n_spheres x (xc, yc, zc, r) in float64
+//   mode: rk     RungeKutta over the virtual iterator interface (reference loop)
+//         hip    HIP_RungeKutta (one library call per step)
+//         graph  HIP_RungeKutta::graph (captured steps)
+//   out file: field 0 of every patch, 5*N values in `real` each
+// Prints one line per phase and exits non-zero on any error.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "../host/iterators.h"
+
+using namespace drnum;
+
+typedef AusmPlus<5, Upwind2<5, VanAlbada>, PerfectGas> euler_t;
+
+template <typename real>
+static int run(const std::string& grid_file, const std::string& case_file, const std::string& out_file, int steps,
+               double dt, const std::string& mode, bool levelset) {
+  PatchGrid<double> grid;
+  std::string err;
+  if (grid.readGrid(grid_file, 1.0, &err)) {
+    fprintf(stderr, "readGrid: %s\n", err.c_str());
+    return 2;
+  }
+  std::ifstream in(case_file, std::ios::binary);
+  if (!in) {
+    fprintf(stderr, "cannot open %s\n", case_file.c_str());
+    return 2;
+  }
+  std::vector<HostPatch<real>> patches;
+  patches.reserve(grid.numPatches());
+  for (size_t p = 0; p < grid.numPatches(); ++p) patches.emplace_back(&grid, p);
+  for (size_t p = 0; p < grid.numPatches(); ++p) {
+    CartesianPatch<double>& P = grid.patch(p);
+    int32_t bc[6];
+    in.read(reinterpret_cast<char*>(bc), sizeof(bc));
+    in.read(reinterpret_cast<char*>(P.face_par), sizeof(P.face_par));
+    for (int f = 0; f < 6; ++f) P.face[f] = bc[f];
+    in.read(reinterpret_cast<char*>(patches[p].getField(0)), (std::streamsize)(5 * P.cells() * sizeof(real)));
+  }
+  int n_spheres = 0, n_inner = 0;
+  double min_rel = 0;
+  std::vector<double> spheres;
+  if (levelset) {
+    int32_t ns = 0, ni = 0;
+    in.read(reinterpret_cast<char*>(&ns), 4);
+    in.read(reinterpret_cast<char*>(&ni), 4);
+    in.read(reinterpret_cast<char*>(&min_rel), 8);
+    n_spheres = ns;
+    n_inner = ni;
+    spheres.resize(4 * (size_t)n_spheres);
+    in.read(reinterpret_cast<char*>(spheres.data()), (std::streamsize)(spheres.size() * sizeof(double)));
+  }
+  if (!in) {
+    fprintf(stderr, "case file %s is short\n", case_file.c_str());
+    return 2;
+  }
+  const double period[3] = {0, 0, 0};
+  if (grid.computeDependencies(period, 8)) {
+    fprintf(stderr, "computeDependencies hit a BUG condition\n");
+    return 2;
+  }
+  if (levelset && grid.levelSetSpheres(spheres, n_inner, 0, min_rel)) {
+    fprintf(stderr, "level-set mirror search hit a BUG condition\n");
+    return 2;
+  }
+  printf("grid: %zu patches read, dependencies computed\n", grid.numPatches());
+
+  HIP_CartesianIterator<5, euler_t, real> iterator(euler_t(), 0);
+  for (auto& P : patches) iterator.addPatch(&P);
+  iterator.updateDevice();
+  HIP_LevelSetBC<5, euler_t, real>* ls = nullptr;
+  if (levelset) ls = new HIP_LevelSetBC<5, euler_t, real>(&iterator);
+
+  RungeKutta<real> rk;
+  HIP_RungeKutta<5, euler_t, real> hrk(&iterator);
+  for (real a : {real(0.25), real(0.5), real(1.0)}) {
+    rk.addAlpha(a);
+    hrk.addAlpha(a);
+  }
+  rk.addIterator(&iterator);
+  if (ls) rk.addPostOperation(ls);
+
+  if (mode == "rk") {
+    for (int s = 0; s < steps; ++s) rk(real(dt));
+  } else if (mode == "hip") {
+    for (int s = 0; s < steps; ++s) hrk(real(dt));
+  } else if (mode == "graph") {
+    hrk.graph(real(dt), steps);
+  } else {
+    fprintf(stderr, "unknown mode %s\n", mode.c_str());
+    return 2;
+  }
+  iterator.sync();
+  printf("ran %d steps (%s)\n", steps, mode.c_str());
+  iterator.updateHost();
+  std::ofstream out(out_file, std::ios::binary);
+  for (auto& P : patches)
+    out.write(reinterpret_cast<const char*>(P.getField(0)), (std::streamsize)(5 * P.variableSize() * sizeof(real)));
+  delete ls;
+  printf("wrote %s\n", out_file.c_str());
+  return out ? 0 : 3;
+}
+
+int main(int argc, char** argv) {
+  if (argc < 7) {
+    fprintf(stderr, "usage: %s <grid> <case> <out> <steps> <dt> <rk|hip|graph> [f64] [levelset]\n", argv[0]);
+    return 2;
+  }
+  bool f64 = false, levelset = false;
+  for (int a = 7; a < argc; ++a) {
+    if (!strcmp(argv[a], "f64")) f64 = true;
+    if (!strcmp(argv[a], "levelset")) levelset = true;
+  }
+  try {
+    if (f64) return run<double>(argv[1], argv[2], argv[3], atoi(argv[4]), atof(argv[5]), argv[6], levelset);
+    return run<float>(argv[1], argv[2], argv[3], atoi(argv[4]), atof(argv[5]), argv[6], levelset);
+  } catch (const DeviceError& e) {
+    fprintf(stderr, "device error: %s\n", e.what());
+    return 4;
+  }
+}

@@ -50,9 +50,26 @@ def build_host():
     return out
 
 
+def build_apps():
+    """C++ applications on the drop-in classes (drnum_amd/apps/*.cpp -> lib/<name>)."""
+    src_dir = os.path.join(HERE, "apps")
+    outs = []
+    for f in sorted(os.listdir(src_dir)):
+        if not f.endswith(".cpp"):
+            continue
+        out = os.path.join(LIB, f[:-4])
+        tmp = out + ".tmp"
+        _run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I" + os.path.join(ROOT, "include"), "-o", tmp,
+              os.path.join(src_dir, f), "-L" + LIB, "-ldrnum", "-ldrnhip", "-Wl,-rpath,$ORIGIN"])
+        os.replace(tmp, out)
+        outs.append(out)
+    return outs
+
+
 def build_all():
     build_hip()
     build_host()
+    build_apps()
 
 
 if __name__ == "__main__":

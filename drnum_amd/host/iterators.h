@@ -1,0 +1,458 @@
+// iterators.h -- C++ drop-in for the reference's GPU iterator path, running on
+// libdrnhip through its C ABI (include/drnhip.h).
+//
+// The reference's path (iterators/patchiterator.h:28-72, tpatchiterator.h:29-61,
+// gpu_cartesianiterator.h:29-304, rungekutta.cpp:27-42, timeintegration.cpp:28-55)
+// is mirrored class by class with the same names and argument meaning:
+//
+//   PatchIterator            compute(factor, patches) / computeAll(factor) /
+//                            copyField(src, dst) / copyDonorData(field) /
+//                            updateHost() / updateDevice() / addPatch(patch)
+//   HIP_CartesianIterator<DIM, OP>(OP op, int device = 0, size_t thread_limit = 0)
+//                            the GPU_CartesianIterator<DIM, OP> replacement
+//   GenericOperation         post operation (genericoperation.h:24-31)
+//   HIP_LevelSetBC           level-set fill as a post operation (the
+//                            CPU_LevelSetObjectBC<LSOBCCompressibleEulerOp> role)
+//   TimeIntegration          addIterator / addPostOperation / copyField /
+//                            computeIterators / copyDonorData / runPostOperations
+//   RungeKutta               addAlpha, operator()(dt): the reference's stage loop
+//   HIP_RungeKutta           the same step through drn_rk_step (no field copy,
+//                            fewer exchanges) or a captured graph (drn_rk_graph_run)
+//
+// Patches are HostPatch<real> objects: the state arrays m_Data laid out
+// [field][var][cell] as Patch::allocateData (patch.cpp:703-715), on top of a
+// drnum::CartesianPatch<double> (geometry, seek layers, face functors) of a
+// drnum::PatchGrid<double> that holds the donor tables.
+//
+// The flux operator OP is a type, as in the reference.  op_traits<OP> maps the
+// supported operator types onto the library's operator enums at compile time;
+// any other OP fails to compile.  Errors of the library raise drnum::DeviceError
+// (the reference stops the process on BUG/ERROR, drnum.h:109-128; an uncaught
+// DeviceError does the same).
+#pragma once
+
+#include <cstddef>
+#include <list>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/drnhip.h"
+#include "grid.h"
+
+namespace drnum {
+
+// ---------------------------------------------------------------------------
+// operator types (names of the reference's templates; no host arithmetic here)
+// ---------------------------------------------------------------------------
+
+struct VanAlbada {};     // reconstruction/vanalbada.h:28-33
+struct MinMod {};        // reconstruction/minmod.h:28-31
+struct SecondOrder {};   // reconstruction/secondorder.h:28-31
+template <unsigned DIM, class TLimiter>
+struct Upwind2 {};       // reconstruction/upwind2.h:29-52
+template <unsigned DIM>
+struct Upwind1 {};       // reconstruction/upwind1.h:29-39
+struct PerfectGas {};    // perfectgas.h:31-34
+template <unsigned DIM, class TReconstruction, class TGas>
+struct AusmPlus {};      // fluxes/ausmplus.h:39-106
+
+template <class L>
+struct limiter_code;
+template <>
+struct limiter_code<VanAlbada> {
+  static constexpr int value = DRN_LIM_VANALBADA;
+};
+template <>
+struct limiter_code<MinMod> {
+  static constexpr int value = DRN_LIM_MINMOD;
+};
+template <>
+struct limiter_code<SecondOrder> {
+  static constexpr int value = DRN_LIM_SECONDORDER;
+};
+
+template <class OP>
+struct op_traits {
+  static_assert(sizeof(OP) == 0, "this flux operator has no HIP implementation");
+};
+template <class L>
+struct op_traits<AusmPlus<5, Upwind2<5, L>, PerfectGas>> {
+  static constexpr int recon = DRN_RECON_UPWIND2;
+  static constexpr int limiter = limiter_code<L>::value;
+};
+template <>
+struct op_traits<AusmPlus<5, Upwind1<5>, PerfectGas>> {
+  static constexpr int recon = DRN_RECON_UPWIND1;
+  static constexpr int limiter = DRN_LIM_VANALBADA;  // unused by first order
+};
+
+class DeviceError : public std::runtime_error {
+ public:
+  using std::runtime_error::runtime_error;
+};
+
+// ---------------------------------------------------------------------------
+// host patch: state arrays + geometry (Patch::m_Data, patch_common.h:26-95)
+// ---------------------------------------------------------------------------
+
+template <typename real>
+class HostPatch {
+ public:
+  HostPatch(const PatchGrid<double>* grid, size_t index, size_t num_fields = 3, size_t num_vars = 5)
+      : grid_(grid), index_(index), nf_(num_fields), nv_(num_vars) {
+    data_.assign(nf_ * nv_ * grid_->patch(index_).cells(), real(0));
+  }
+  const CartesianPatch<double>& geometry() const { return grid_->patch(index_); }
+  const PatchGrid<double>* grid() const { return grid_; }
+  size_t getIndex() const { return index_; }
+  size_t sizeI() const { return (size_t)geometry().ni(); }
+  size_t sizeJ() const { return (size_t)geometry().nj(); }
+  size_t sizeK() const { return (size_t)geometry().nk(); }
+  size_t numFields() const { return nf_; }
+  size_t numVariables() const { return nv_; }
+  size_t variableSize() const { return geometry().cells(); }
+  size_t fieldSize() const { return nv_ * variableSize(); }
+  size_t dataSize() const { return nf_ * fieldSize(); }
+  real* getData() { return data_.data(); }
+  real* getField(size_t i_field) { return data_.data() + i_field * fieldSize(); }
+  real* getVariable(size_t i_field, size_t i_var) { return getField(i_field) + i_var * variableSize(); }
+  // Patch::setFieldToConst
+  void setFieldToConst(size_t i_field, const real* var) {
+    for (size_t v = 0; v < nv_; ++v) {
+      real* p = getVariable(i_field, v);
+      for (size_t l = 0; l < variableSize(); ++l) p[l] = var[v];
+    }
+  }
+
+ private:
+  const PatchGrid<double>* grid_;
+  size_t index_, nf_, nv_;
+  std::vector<real> data_;
+};
+
+// ---------------------------------------------------------------------------
+// iterators
+// ---------------------------------------------------------------------------
+
+template <typename real>
+class PatchIterator {
+ public:
+  virtual ~PatchIterator() {}
+  virtual void updateHost() = 0;
+  virtual void updateDevice() = 0;
+  size_t numPatches() const { return patches_.size(); }
+  HostPatch<real>* getPatch(size_t i) { return patches_[i]; }
+  // PatchIterator::computeAll (patchiterator.h:80-88): every active patch
+  void computeAll(real factor) {
+    std::vector<size_t> ids;
+    for (size_t i = 0; i < patches_.size(); ++i)
+      if (active_[i]) ids.push_back(i);
+    compute(factor, ids);
+  }
+  virtual void addPatch(HostPatch<real>* patch) {
+    patches_.push_back(patch);
+    active_.push_back(true);
+  }
+  virtual void compute(real factor, const std::vector<size_t>& patches) = 0;
+  virtual void copyField(size_t i_src, size_t i_dst) = 0;
+  virtual void copyDonorData(size_t i_field) = 0;
+  void activatePatch(size_t i) { active_[i] = true; }
+  void deactivatePatch(size_t i) { active_[i] = false; }
+  bool patchActive(size_t i) const { return active_[i]; }
+  void setCodeString(const std::string& s) { codes_ = s; }
+  std::string getCodeString() const { return codes_; }
+
+ protected:
+  std::vector<HostPatch<real>*> patches_;
+  std::vector<bool> active_;
+  std::string codes_ = "void";
+};
+
+template <typename real>
+struct real_precision;
+template <>
+struct real_precision<float> {
+  static constexpr int value = DRN_F32;
+};
+template <>
+struct real_precision<double> {
+  static constexpr int value = DRN_F64;
+};
+
+// GPU_CartesianIterator<DIM, OP> (gpu_cartesianiterator.h:29-304) on libdrnhip.
+// One device context holds every patch added before the first device call; a
+// stage is one launch over all of them.  As GPU_CartesianIterator::compute
+// (gpu_cartesianiterator.h:224), compute() advances every patch whatever the
+// `patches` argument holds.  thread_limit is accepted for interface parity
+// and unused (the work list, not a block size, is tuned: drn_autotune).
+template <unsigned DIM, class OP, typename real = float>
+class HIP_CartesianIterator : public PatchIterator<real> {
+  static_assert(DIM == 5, "the HIP path covers the 5-variable compressible system");
+
+ public:
+  explicit HIP_CartesianIterator(OP op = OP(), int device = 0, size_t thread_limit = 0) : op_(op) {
+    (void)thread_limit;
+    check(drn_ctx_create(device, real_precision<real>::value, &ctx_), "drn_ctx_create");
+    check(drn_set_operator(ctx_, op_traits<OP>::recon, op_traits<OP>::limiter), "drn_set_operator");
+  }
+  ~HIP_CartesianIterator() override { drn_ctx_destroy(ctx_); }
+  HIP_CartesianIterator(const HIP_CartesianIterator&) = delete;
+  HIP_CartesianIterator& operator=(const HIP_CartesianIterator&) = delete;
+
+  drn_ctx* context() {
+    setup();
+    return ctx_;
+  }
+
+  void addPatch(HostPatch<real>* patch) override {
+    if (laid_out_) throw DeviceError("HIP_CartesianIterator::addPatch after the first device call");
+    PatchIterator<real>::addPatch(patch);
+  }
+
+  // Patch -> device copy of all fields (GPU_Patch::copyToDevice, gpu_patch.h:130-134)
+  void updateDevice() override {
+    setup();
+    for (size_t p = 0; p < this->patches_.size(); ++p)
+      for (size_t f = 0; f < this->patches_[p]->numFields() && f < 3; ++f)
+        check(drn_patch_upload(ctx_, (int)p, (int)f, this->patches_[p]->getField(f)), "drn_patch_upload");
+  }
+  // device -> Patch copy of all fields (GPU_Patch::copyFromDevice, gpu_patch.h:136-139)
+  void updateHost() override {
+    setup();
+    for (size_t p = 0; p < this->patches_.size(); ++p)
+      for (size_t f = 0; f < this->patches_[p]->numFields() && f < 3; ++f)
+        check(drn_patch_download(ctx_, (int)p, (int)f, this->patches_[p]->getField(f)), "drn_patch_download");
+  }
+
+  // CartesianIterator::compute: field 0 <- field 1 + factor/dV * Res(field 0).  The
+  // kernel writes the scratch field 2, which then becomes logical field 0
+  // (field 2 is the reference's scratch / abuse field).
+  void compute(real factor, const std::vector<size_t>&) override {
+    setup();
+    check(drn_stage(ctx_, 0, 1, 2, (double)factor, launch_), "drn_stage");
+    check(drn_swap_fields(ctx_, 0, 2), "drn_swap_fields");
+  }
+  void copyField(size_t i_src, size_t i_dst) override {
+    setup();
+    check(drn_copy_field(ctx_, (int)i_src, (int)i_dst), "drn_copy_field");
+  }
+  void copyDonorData(size_t i_field) override {
+    setup();
+    check(drn_exchange(ctx_, (int)i_field), "drn_exchange");
+  }
+
+  // DRN_LAUNCH_BATCHED (default: one launch over all patches) or DRN_LAUNCH_PER_PATCH
+  void setLaunchForm(int launch) { launch_ = launch; }
+  int autotune() {
+    setup();
+    int choice = -1;
+    check(drn_autotune(ctx_, launch_, &choice), "drn_autotune");
+    return choice;
+  }
+  void sync() { check(drn_sync(ctx_), "drn_sync"); }
+  int launchForm() const { return launch_; }
+  const OP& op() const { return op_; }
+
+  void check(int rc, const char* what) const {
+    if (rc != DRN_OK) throw DeviceError(std::string(what) + ": " + (ctx_ ? drn_last_error(ctx_) : "no context"));
+  }
+
+ private:
+  // device layout of every added patch: geometry, face functors (farfield state
+  // converted to conserved variables in `real`), donor tables of the grid
+  void setup() {
+    if (laid_out_) return;
+    if (this->patches_.empty()) throw DeviceError("HIP_CartesianIterator: no patches");
+    const PatchGrid<double>* grid = this->patches_[0]->grid();
+    for (size_t p = 0; p < this->patches_.size(); ++p) {
+      const HostPatch<real>& hp = *this->patches_[p];
+      if (hp.grid() != grid) throw DeviceError("HIP_CartesianIterator: patches of different grids");
+      const CartesianPatch<double>& P = hp.geometry();
+      drn_patch_desc d{};
+      d.ni = P.ni();
+      d.nj = P.nj();
+      d.nk = P.nk();
+      for (int f = 0; f < 6; ++f) d.seek[f] = P.seek(f);
+      // CartesianPatch::computeDeltas in `real` (cartesianpatch.cpp:164-168)
+      d.dx = (double)(real(P.length(0)) / real((size_t)P.ni()));
+      d.dy = (double)(real(P.length(1)) / real((size_t)P.nj()));
+      d.dz = (double)(real(P.length(2)) / real((size_t)P.nk()));
+      for (int f = 0; f < 6; ++f) {
+        d.bc[f] = P.face[f];
+        for (int q = 0; q < DRN_BC_NPAR; ++q) d.bc_par[f][q] = P.face_par[f][q];
+        if (P.face[f] == FACE_FARFIELD) {
+          // PerfectGas::primitiveToConservative in `real` (perfectgas.h:56-64)
+          const real pr = real(P.face_par[f][0]), T = real(P.face_par[f][1]), u = real(P.face_par[f][2]),
+                     v = real(P.face_par[f][3]), w = real(P.face_par[f][4]);
+          const real rho = pr / (real(287.2) * T);
+          d.bc_par[f][0] = (double)rho;
+          d.bc_par[f][1] = (double)(real)(rho * u);
+          d.bc_par[f][2] = (double)(real)(rho * v);
+          d.bc_par[f][3] = (double)(real)(rho * w);
+          d.bc_par[f][4] = (double)(real)(rho * (real(717.5) * T + 0.5 * (u * u + v * v + w * w)));
+        }
+      }
+      int id = -1;
+      check(drn_patch_add(ctx_, &d, &id), "drn_patch_add");
+    }
+    check(drn_layout(ctx_), "drn_layout");
+    // donor tables (Patch::buildDonorTransferData, patch.cpp:128-218), for patches of this iterator
+    bool any = false;
+    for (size_t p = 0; p < this->patches_.size(); ++p) {
+      const DonorTables& t = grid->tables(this->patches_[p]->getIndex());
+      if (t.donors.empty()) continue;
+      std::vector<int32_t> dp;
+      std::vector<int64_t> info, ru(t.receivers.begin(), t.receivers.end()),
+          rc(t.rec_concat.begin(), t.rec_concat.end()), di(t.donor_index.begin(), t.donor_index.end());
+      for (const auto& D : t.donors) {
+        int local = -1;
+        for (size_t q = 0; q < this->patches_.size(); ++q)
+          if ((int)this->patches_[q]->getIndex() == D.patch) local = (int)q;
+        if (local < 0) throw DeviceError("HIP_CartesianIterator: a donor patch is not in this iterator");
+        dp.push_back(local);
+        info.push_back((int64_t)D.nrec);
+        info.push_back((int64_t)D.stride);
+        info.push_back((int64_t)D.rec_start);
+        info.push_back((int64_t)D.wi_start);
+      }
+      check(drn_exchange_set(ctx_, (int)p, (int)t.donors.size(), dp.data(), info.data(), (int64_t)ru.size(),
+                             ru.data(), (int64_t)rc.size(), rc.data(), (int64_t)di.size(), di.data(),
+                             t.donor_weight.data()),
+            "drn_exchange_set");
+      any = true;
+    }
+    if (any) check(drn_exchange_commit(ctx_), "drn_exchange_commit");
+    laid_out_ = true;
+  }
+
+  OP op_;
+  drn_ctx* ctx_ = nullptr;
+  bool laid_out_ = false;
+  int launch_ = DRN_LAUNCH_BATCHED;
+};
+
+// ---------------------------------------------------------------------------
+// post operations and time integration
+// ---------------------------------------------------------------------------
+
+class GenericOperation {
+ public:
+  virtual ~GenericOperation() {}
+  virtual void operator()() = 0;
+};
+
+// The slip level-set fill (CPU_LevelSetObjectBC<LSOBCCompressibleEulerOp>,
+// cpu_levelsetobjectbc.h:58-90, relax 0.5) on the inner layers the grid's
+// level-set builder produced, for the patches of one HIP iterator.
+template <unsigned DIM, class OP, typename real>
+class HIP_LevelSetBC : public GenericOperation {
+ public:
+  HIP_LevelSetBC(HIP_CartesianIterator<DIM, OP, real>* it, size_t field = 0, double relax = 0.5)
+      : it_(it), field_(field), relax_(relax) {
+    drn_ctx* ctx = it_->context();
+    const PatchGrid<double>* grid = it_->getPatch(0)->grid();
+    for (size_t p = 0; p < it_->numPatches(); ++p) {
+      std::vector<int64_t> cells, donors;
+      std::vector<double> nrm, w;
+      const size_t gp = it_->getPatch(p)->getIndex();
+      for (int layer = 0; layer < grid->numInnerLayers(); ++layer)
+        for (const LayerCell& c : grid->innerLayer(gp, layer)) {
+          cells.push_back((int64_t)c.cell);
+          nrm.push_back(c.gx);
+          nrm.push_back(c.gy);
+          nrm.push_back(c.gz);
+          for (int s = 0; s < 8; ++s) {
+            donors.push_back((int64_t)c.donor[s]);
+            w.push_back(c.weight[s]);
+          }
+        }
+      if (cells.empty()) continue;
+      it_->check(drn_levelset_set(ctx, (int)p, (int64_t)cells.size(), cells.data(), nrm.data(), donors.data(),
+                                  w.data()),
+                 "drn_levelset_set");
+    }
+    it_->check(drn_levelset_commit(ctx), "drn_levelset_commit");
+  }
+  void operator()() override { it_->check(drn_levelset_fill(it_->context(), (int)field_, relax_), "drn_levelset_fill"); }
+
+ private:
+  HIP_CartesianIterator<DIM, OP, real>* it_;
+  size_t field_;
+  double relax_;
+};
+
+// TimeIntegration (timeintegration.cpp:28-55)
+template <typename real>
+class TimeIntegration {
+ public:
+  virtual ~TimeIntegration() {}
+  void addIterator(PatchIterator<real>* it) { iterators_.push_back(it); }
+  void addPostOperation(GenericOperation* op) { post_.push_back(op); }
+  void copyField(size_t i_src, size_t i_dst) {
+    for (auto* it : iterators_) it->copyField(i_src, i_dst);
+  }
+  void computeIterators(real factor) {
+    for (auto* it : iterators_) it->computeAll(factor);
+  }
+  void copyDonorData(size_t i_field) {
+    for (auto* it : iterators_) it->copyDonorData(i_field);
+  }
+  void runPostOperations() {
+    for (auto* op : post_) (*op)();
+  }
+  virtual void operator()(real dt) = 0;
+
+ protected:
+  std::list<PatchIterator<real>*> iterators_;
+  std::list<GenericOperation*> post_;
+};
+
+// RungeKutta (rungekutta.cpp:27-42): the reference's stage loop over the virtual
+// iterator interface (copy of field 0 into field 1, two exchanges per stage).
+template <typename real>
+class RungeKutta : public TimeIntegration<real> {
+ public:
+  void addAlpha(real a) { alpha_.push_back(a); }
+  void operator()(real dt) override {
+    bool first_step = true;
+    this->copyDonorData(0);
+    this->copyField(0, 1);
+    for (real a : alpha_) {
+      if (!first_step) this->copyDonorData(0);
+      first_step = false;
+      this->computeIterators(a * dt);
+      this->copyDonorData(0);
+      this->runPostOperations();
+    }
+  }
+  const std::list<real>& alphas() const { return alpha_; }
+
+ protected:
+  std::list<real> alpha_;
+};
+
+// The same step for one HIP iterator in a single library call: the stages
+// alternate between the two spare state buffers instead of copying field 0 into
+// field 1, and the exchange before a stage is skipped when nothing changed the
+// field since the last one.  A level-set fill set on the iterator's context runs
+// after every stage's exchange, as the post operation does.  graph(dt, n) runs
+// n steps from a captured HIP graph.
+template <unsigned DIM, class OP, typename real>
+class HIP_RungeKutta : public RungeKutta<real> {
+ public:
+  explicit HIP_RungeKutta(HIP_CartesianIterator<DIM, OP, real>* it) : it_(it) { this->addIterator(it); }
+  void operator()(real dt) override {
+    std::vector<double> a(this->alpha_.begin(), this->alpha_.end());
+    it_->check(drn_rk_step(it_->context(), (double)dt, (int)a.size(), a.data(), it_->launchForm()), "drn_rk_step");
+  }
+  void graph(real dt, int n_steps) {
+    std::vector<double> a(this->alpha_.begin(), this->alpha_.end());
+    it_->check(drn_rk_graph_run(it_->context(), (double)dt, (int)a.size(), a.data(), n_steps), "drn_rk_graph_run");
+  }
+
+ private:
+  HIP_CartesianIterator<DIM, OP, real>* it_;
+};
+
+}  // namespace drnum

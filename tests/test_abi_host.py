@@ -17,6 +17,8 @@ import pytest
 
 from drnum_amd import configs, hip, host
 
+from casefiles import grid_file_text
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -44,26 +46,6 @@ def test_hip_library_is_a_gfx950_code_object_host():
     """The HIP library carries device code for gfx950 (offload bundle section)."""
     data = open(hip.LIB_PATH, "rb").read()
     assert b"gfx950" in data
-
-
-def grid_file_text(w):
-    """A grid file as the reference's Python writer lays it out (drnum.py:199-244)."""
-    codes = {configs.BC_FARFIELD: "far", configs.BC_SLIP: "slip", configs.BC_TRANSMISSIVE: "trans",
-             configs.BC_JET: "jet", configs.BC_NONE: "none"}
-    out = []
-    for n, P in enumerate(w.patches):
-        txt = "1001 // index=%d name='p%d'\n{\n" % (n, n)
-        txt += "  %r %r %r\n" % tuple(P.origin)
-        txt += "  1 0 0\n  0 1 0\n  1\n"
-        txt += "  %d %d %d\n" % (P.ni, P.nj, P.nk)
-        txt += "  " + " ".join(str(s) for s in P.seek) + "\n"
-        txt += "  %r %r %r\n" % tuple(P.length)
-        txt += "  fx fy fz\n"
-        txt += "  " + " ".join("0" if P.seek[f] else codes[P.bc[f]] for f in range(6)) + "\n"
-        txt += "  0\n}\n"
-        out.append(txt)
-    out.append("0\n")
-    return "".join(out)
 
 
 def test_grid_reader_matches_programmatic_setup(tmp_path):
@@ -155,3 +137,23 @@ def test_bench_reads_the_committed_traffic_figure():
     t = b.measured_traffic(20)
     assert t is not None and 0.9e9 < t < 1.2e9
     assert b.measured_traffic(999) is None
+
+
+def test_dropin_app_reports_missing_device(tmp_path):
+    """Without a GPU the C++ application stops with the device error of
+    drn_ctx_create (the C ABI's return code, turned into drnum::DeviceError)."""
+    import subprocess
+
+    from casefiles import case_file_bytes
+
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is visible")
+    w = configs.jet(core=6, blocks=2)
+    (tmp_path / "grid").write_text(grid_file_text(w))
+    (tmp_path / "case").write_bytes(case_file_bytes(w, np.float32))
+    app = os.path.join(ROOT, "drnum_amd", "lib", "drnum_run")
+    r = subprocess.run([app, str(tmp_path / "grid"), str(tmp_path / "case"), str(tmp_path / "out"), "1", "1e-4", "rk"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 4, r.stdout + r.stderr
+    assert "drn_ctx_create" in r.stderr
+    assert "grid: 8 patches read" in r.stdout
