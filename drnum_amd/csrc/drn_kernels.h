@@ -36,6 +36,14 @@ struct TileMarch {
   int32_t patch, i0, i1, c0;
 };
 
+// work item of the wave-tile stage kernel: patch, i range [i0, i1), and the
+// first row / column (j0, k0) of the wavefront's 8x8 lane tile
+struct TileWave {
+  int32_t patch;
+  int16_t i0, i1, j0, k0;
+  int32_t pad;
+};
+
 template <typename T>
 __device__ __forceinline__ void zero5(T* f) {
 #pragma unroll
@@ -44,27 +52,34 @@ __device__ __forceinline__ void zero5(T* f) {
 
 // face between cells cm1 (minus side) and cp1 (plus side) along DIR; cm2 and cp2
 // are the next cells outwards, valid only when has_m2 / has_p2 (upwind2.h:38-50).
+// The cells are 32-bit byte offsets into the variable arrays sv[0..4] (uniform
+// bases), so every load is a scalar-base + 32-bit vector-offset access.
+template <typename T>
+__device__ __forceinline__ T ldv(const char* base, uint32_t boff) {
+  return *reinterpret_cast<const T*>(base + boff);
+}
+
 template <typename T, int RECON, int LIM, bool FAST, int DIR>
-__device__ __forceinline__ void face_flux_g(const T* __restrict__ s, uint64_t vs, uint64_t cm2, uint64_t cm1,
-                                            uint64_t cp1, uint64_t cp2, bool has_m2, bool has_p2, T A, T* f) {
+__device__ __forceinline__ void face_flux_g(const char* const* sv, uint32_t cm2, uint32_t cm1, uint32_t cp1,
+                                            uint32_t cp2, bool has_m2, bool has_p2, T A, T* f) {
   T vl[5], vr[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
-    vl[v] = s[v * vs + cm1];
-    vr[v] = s[v * vs + cp1];
+    vl[v] = ldv<T>(sv[v], cm1);
+    vr[v] = ldv<T>(sv[v], cp1);
   }
   if constexpr (RECON == RECON_UPWIND2) {
     if (has_m2) {
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        T a = vl[v], b = vr[v], m = s[v * vs + cm2];
+        T a = vl[v], b = vr[v], m = ldv<T>(sv[v], cm2);
         vl[v] = a + T(0.5) * limited<T, LIM, FAST>(a - m, b - a);
       }
     }
     if (has_p2) {
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        T a = s[v * vs + cm1], b = s[v * vs + cp1], q = s[v * vs + cp2];
+        T a = ldv<T>(sv[v], cm1), b = ldv<T>(sv[v], cp1), q = ldv<T>(sv[v], cp2);
         vr[v] = b + T(0.5) * limited<T, LIM, FAST>(b - q, a - b);
       }
     }
@@ -84,8 +99,13 @@ __global__ __launch_bounds__(BS) void k_stage_cell(const PatchDev* __restrict__ 
   const int i = t.i;
   const int j = c2 / nk;
   const int k = c2 - j * nk;
-  const uint64_t c = (uint64_t)P.off + (uint64_t)i * (uint64_t)njk + (uint64_t)c2;
-  const uint64_t sx = (uint64_t)njk, sy = (uint64_t)nk;
+  // byte offsets within one variable array (the layout keeps them below 2^32)
+  constexpr uint32_t W = sizeof(T);
+  const uint32_t c = (P.off + (uint32_t)i * (uint32_t)njk + (uint32_t)c2) * W;
+  const uint32_t sx = (uint32_t)njk * W, sy = (uint32_t)nk * W;
+  const char* sv[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) sv[v] = reinterpret_cast<const char*>(src + (uint64_t)v * vs);
   const T dx = T(P.dx), dy = T(P.dy), dz = T(P.dz);
   const T Ax = dy * dz, Ay = dx * dz, Az = dx * dy;
   T res[5], f[5];
@@ -94,37 +114,37 @@ __global__ __launch_bounds__(BS) void k_stage_cell(const PatchDev* __restrict__ 
   // order the later cells of the reference loop subtract them
   if (i > 0 && ni > 2) {
     zero5(f);
-    face_flux_g<T, RECON, LIM, FAST, 0>(src, vs, c - 2 * sx, c - sx, c, c + sx, i >= 2, i + 1 < ni, Ax, f);
+    face_flux_g<T, RECON, LIM, FAST, 0>(sv, c - 2 * sx, c - sx, c, c + sx, i >= 2, i + 1 < ni, Ax, f);
 #pragma unroll
     for (int v = 0; v < 5; ++v) res[v] += f[v];
   }
   if (j > 0 && nj > 2) {
     zero5(f);
-    face_flux_g<T, RECON, LIM, FAST, 1>(src, vs, c - 2 * sy, c - sy, c, c + sy, j >= 2, j + 1 < nj, Ay, f);
+    face_flux_g<T, RECON, LIM, FAST, 1>(sv, c - 2 * sy, c - sy, c, c + sy, j >= 2, j + 1 < nj, Ay, f);
 #pragma unroll
     for (int v = 0; v < 5; ++v) res[v] += f[v];
   }
   if (k > 0 && nk > 2) {
     zero5(f);
-    face_flux_g<T, RECON, LIM, FAST, 2>(src, vs, c - 2, c - 1, c, c + 1, k >= 2, k + 1 < nk, Az, f);
+    face_flux_g<T, RECON, LIM, FAST, 2>(sv, c - 2 * W, c - W, c, c + W, k >= 2, k + 1 < nk, Az, f);
 #pragma unroll
     for (int v = 0; v < 5; ++v) res[v] += f[v];
   }
   if (k < nk - 1 && nk > 2) {
     zero5(f);
-    face_flux_g<T, RECON, LIM, FAST, 2>(src, vs, c - 1, c, c + 1, c + 2, k >= 1, k + 2 < nk, Az, f);
+    face_flux_g<T, RECON, LIM, FAST, 2>(sv, c - W, c, c + W, c + 2 * W, k >= 1, k + 2 < nk, Az, f);
 #pragma unroll
     for (int v = 0; v < 5; ++v) res[v] -= f[v];
   }
   if (j < nj - 1 && nj > 2) {
     zero5(f);
-    face_flux_g<T, RECON, LIM, FAST, 1>(src, vs, c - sy, c, c + sy, c + 2 * sy, j >= 1, j + 2 < nj, Ay, f);
+    face_flux_g<T, RECON, LIM, FAST, 1>(sv, c - sy, c, c + sy, c + 2 * sy, j >= 1, j + 2 < nj, Ay, f);
 #pragma unroll
     for (int v = 0; v < 5; ++v) res[v] -= f[v];
   }
   if (i < ni - 1 && ni > 2) {
     zero5(f);
-    face_flux_g<T, RECON, LIM, FAST, 0>(src, vs, c - sx, c, c + sx, c + 2 * sx, i >= 1, i + 2 < ni, Ax, f);
+    face_flux_g<T, RECON, LIM, FAST, 0>(sv, c - sx, c, c + sx, c + 2 * sx, i >= 1, i + 2 < ni, Ax, f);
 #pragma unroll
     for (int v = 0; v < 5; ++v) res[v] -= f[v];
   }
@@ -135,7 +155,7 @@ __global__ __launch_bounds__(BS) void k_stage_cell(const PatchDev* __restrict__ 
   if (wx || wy || wz) {
     T var[5];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) var[v] = src[v * vs + c];
+    for (int v = 0; v < 5; ++v) var[v] = ldv<T>(sv[v], c);
     const T xc = T(0.5) * dx + T(i) * dx, yc = T(0.5) * dy + T(j) * dy, zc = T(0.5) * dz + T(k) * dz;
     if (ni > 2) {
       if (i == 0) {
@@ -183,7 +203,11 @@ __global__ __launch_bounds__(BS) void k_stage_cell(const PatchDev* __restrict__ 
   // cartesianiterator.h:310-322
   const T pf = factor / (dx * dy * dz);
 #pragma unroll
-  for (int v = 0; v < 5; ++v) dst[v * vs + c] = base[v * vs + c] + pf * res[v];
+  for (int v = 0; v < 5; ++v) {
+    const char* bv = reinterpret_cast<const char*>(base + (uint64_t)v * vs);
+    char* dv = reinterpret_cast<char*>(dst + (uint64_t)v * vs);
+    *reinterpret_cast<T*>(dv + c) = ldv<T>(bv, c) + pf * res[v];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -971,6 +995,235 @@ __global__ __launch_bounds__(BS) void k_stage_cell_slope(const PatchDev* __restr
   const T pf = factor / (dx * dy * dz);
 #pragma unroll
   for (int v = 0; v < 5; ++v) dst[v * vs + g] = base[v * vs + g] + pf * res[v];
+}
+
+// ---------------------------------------------------------------------------
+// wave-tile marching stage kernel
+// ---------------------------------------------------------------------------
+
+// Every wavefront of a 256-thread block works alone (no block barrier): its 64
+// lanes form an 8x8 tile of (j, k) columns starting at (j0, k0) and march over
+// the planes [i0, i1).  Lane (jl, kl) owns column (j0 + jl, k0 + kl) and per
+// plane computes the three faces on the minus side of its cell:
+//   x face i+1/2   from a register window of its own column (the i-1/2 face is
+//                  carried from the previous plane),
+//   y face j-1/2   and z face k-1/2 from the plane's 11x11 halo tile, staged in
+//                  a wave-private, double-buffered LDS tile,
+// where a face on a patch boundary (j == 0 or NJ, k == 0 or NK, i == 0 or NI)
+// is the face functor of that boundary.  The plus-side y and z faces of a cell
+// are the minus-side faces of the lanes (jl+1, kl) and (jl, kl+1), fetched by
+// lane shuffles; lanes with jl == 7 or kl == 7 only supply faces, so one tile
+// advances 7x7 columns.  Each face is evaluated once (plus the seam faces of
+// the tile and of the i range), 3 face evaluations per lane per plane.
+// The residual of a cell is summed as Fx- + Fy- + Fz- - Fz+ - Fy+ - Fx+
+// (cartesianiterator.h:132-308); a boundary face functor takes the place of its
+// face in that sum.  Requires NI, NJ, NK >= 3 (thinner patches use k_stage_cell).
+template <typename T, int RECON, int LIM, bool FAST>
+__global__ __launch_bounds__(256) void k_stage_wtile(const PatchDev* __restrict__ pds,
+                                                     const TileWave* __restrict__ items, int n_items,
+                                                     const T* __restrict__ src, const T* base, T* dst, uint64_t vs,
+                                                     T factor) {
+  constexpr int TR = 11;      // tile rows / columns staged: j0-2 .. j0+8
+  constexpr int RS = 12;      // LDS row stride
+  constexpr int TS = TR * RS; // values per variable and buffer
+  __shared__ T sm[4][2][5][TS];
+  const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  const int item = (int)blockIdx.x * 4 + wave;
+  if (item >= n_items) return;  // the whole wavefront leaves; no block-level barrier follows
+  const TileWave t = items[item];
+  const PatchDev& P = pds[t.patch];
+  const int ni = P.ni, nj = P.nj, nk = P.nk, njk = P.njk;
+  const int i0 = t.i0, i1 = t.i1, j0 = t.j0, k0 = t.k0;
+  const int jl = lane >> 3, kl = lane & 7;
+  const int j = j0 + jl, k = k0 + kl;
+  const bool in_col = j < nj && k < nk;
+  const bool upd = in_col && jl < 7 && kl < 7;
+  const bool has_y = j <= nj && k < nk;  // a y face at j-1/2 (functor face when j == 0 or j == nj)
+  const bool has_z = j < nj && k <= nk;  // a z face at k-1/2
+  const T dx = T(P.dx), dy = T(P.dy), dz = T(P.dz);
+  const T Ax = dy * dz, Ay = dx * dz, Az = dx * dy;
+  const T pf = factor / (dx * dy * dz);
+  constexpr uint32_t W = sizeof(T);
+  const char* sv[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) sv[v] = reinterpret_cast<const char*>(src + (uint64_t)v * vs);
+  const uint32_t pb = P.off;  // first cell of the patch
+  const uint32_t col = in_col ? (uint32_t)(j * nk + k) : 0u;
+
+  // staging of plane ip: lane loads tile entries q = lane and lane + 64 (q < 121)
+  T stage[2][5];
+  auto load_plane = [&](int ip) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int q = lane + 64 * h;
+      const int r = q / TR, c = q - r * TR;
+      const int jj = j0 - 2 + r, kk = k0 - 2 + c;
+      if (q < TR * TR && jj >= 0 && jj < nj && kk >= 0 && kk < nk) {
+        const uint32_t off = (pb + (uint32_t)ip * (uint32_t)njk + (uint32_t)(jj * nk + kk)) * W;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) stage[h][v] = ldv<T>(sv[v], off);
+      }
+    }
+  };
+  auto store_plane = [&](int buf) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int q = lane + 64 * h;
+      const int r = q / TR, c = q - r * TR;
+      const int jj = j0 - 2 + r, kk = k0 - 2 + c;
+      if (q < TR * TR && jj >= 0 && jj < nj && kk >= 0 && kk < nk) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) sm[wave][buf][v][r * RS + c] = stage[h][v];
+      }
+    }
+    // the wavefront's own LDS writes complete before its reads of them
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+
+  // register window of the own column: Wn[q] = U(i - 1 + q)
+  T Wn[4][5];
+  auto load_col = [&](int q, int ip) {
+    if (in_col && ip >= 0 && ip < ni) {
+      const uint32_t off = (pb + (uint32_t)ip * (uint32_t)njk + col) * W;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) Wn[q][v] = ldv<T>(sv[v], off);
+    }
+  };
+  // x face between planes ia-1 and ia, window holding planes ia-2 .. ia+1
+  auto xface = [&](int ia, T* f) {
+    zero5(f);
+    if (!in_col) return;
+    const T yc = T(0.5) * dy + T(j) * dy, zc = T(0.5) * dz + T(k) * dz;
+    if (ia == 0) {
+      wall_flux<T>(P.bc[0], P.par[0], Wn[2], 0, false, yc, zc, j, k, Ax, f);
+    } else if (ia == ni) {
+      wall_flux<T>(P.bc[1], P.par[1], Wn[1], 0, true, yc, zc, j, k, Ax, f);
+    } else {
+      const bool hm = ia >= 2, hp = ia + 1 < ni;
+      T vl[5], vr[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        vl[v] = recon1<T, RECON, LIM, FAST>(Wn[1][v], Wn[0][v], Wn[2][v], hm);
+        vr[v] = recon1<T, RECON, LIM, FAST>(Wn[2][v], Wn[3][v], Wn[1][v], hp);
+      }
+      ausm_flux<T, FAST, 0>(vl, vr, Ax, f);
+    }
+  };
+
+  T fxm[5], fxp[5];
+  load_col(0, i0 - 2);
+  load_col(1, i0 - 1);
+  load_col(2, i0);
+  load_col(3, i0 + 1);
+  xface(i0, fxm);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    Wn[0][v] = Wn[1][v];
+    Wn[1][v] = Wn[2][v];
+    Wn[2][v] = Wn[3][v];
+  }
+  load_col(3, i0 + 2);
+  load_plane(i0);
+  store_plane(0);
+
+  for (int i = i0; i < i1; ++i) {
+    const int cur = (i - i0) & 1;
+    if (i + 1 < i1) load_plane(i + 1);  // in flight while this plane is computed
+    const T xc = T(0.5) * dx + T(i) * dx;
+    const T(*S)[TS] = sm[wave][cur];
+    // y face j-1/2
+    T fym[5];
+    zero5(fym);
+    if (has_y) {
+      const int cc = kl + 2;
+      if (j == 0) {
+        T var[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) var[v] = S[v][(jl + 2) * RS + cc];
+        wall_flux<T>(P.bc[2], P.par[2], var, 1, false, xc, T(0.5) * dz + T(k) * dz, i, k, Ay, fym);
+      } else if (j == nj) {
+        T var[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) var[v] = S[v][(jl + 1) * RS + cc];
+        wall_flux<T>(P.bc[3], P.par[3], var, 1, true, xc, T(0.5) * dz + T(k) * dz, i, k, Ay, fym);
+      } else {
+        const bool hm = j >= 2, hp = j + 1 < nj;
+        T vl[5], vr[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          const T a = S[v][(jl + 1) * RS + cc], b = S[v][(jl + 2) * RS + cc];
+          const T m = S[v][(hm ? jl : jl + 1) * RS + cc], q = S[v][(hp ? jl + 3 : jl + 2) * RS + cc];
+          vl[v] = recon1<T, RECON, LIM, FAST>(a, m, b, hm);
+          vr[v] = recon1<T, RECON, LIM, FAST>(b, q, a, hp);
+        }
+        ausm_flux<T, FAST, 1>(vl, vr, Ay, fym);
+      }
+    }
+    // z face k-1/2
+    T fzm[5];
+    zero5(fzm);
+    if (has_z) {
+      const int rr = (jl + 2) * RS;
+      if (k == 0) {
+        T var[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) var[v] = S[v][rr + kl + 2];
+        wall_flux<T>(P.bc[4], P.par[4], var, 2, false, xc, T(0.5) * dy + T(j) * dy, i, j, Az, fzm);
+      } else if (k == nk) {
+        T var[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) var[v] = S[v][rr + kl + 1];
+        wall_flux<T>(P.bc[5], P.par[5], var, 2, true, xc, T(0.5) * dy + T(j) * dy, i, j, Az, fzm);
+      } else {
+        const bool hm = k >= 2, hp = k + 1 < nk;
+        T vl[5], vr[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          const T a = S[v][rr + kl + 1], b = S[v][rr + kl + 2];
+          const T m = S[v][rr + (hm ? kl : kl + 1)], q = S[v][rr + (hp ? kl + 3 : kl + 2)];
+          vl[v] = recon1<T, RECON, LIM, FAST>(a, m, b, hm);
+          vr[v] = recon1<T, RECON, LIM, FAST>(b, q, a, hp);
+        }
+        ausm_flux<T, FAST, 2>(vl, vr, Az, fzm);
+      }
+    }
+    // x face i+1/2 of the own column
+    xface(i + 1, fxp);
+    // plus-side y and z faces from the neighbouring lanes (all lanes take part)
+    T fyp[5], fzp[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      fzp[v] = __shfl_down(fzm[v], 1, 64);
+      fyp[v] = __shfl_down(fym[v], 8, 64);
+    }
+    if (upd) {
+      const uint32_t g = (pb + (uint32_t)i * (uint32_t)njk + col) * W;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        T r = T(0);
+        r += fxm[v];
+        r += fym[v];
+        r += fzm[v];
+        r -= fzp[v];
+        r -= fyp[v];
+        r -= fxp[v];
+        const char* bv = reinterpret_cast<const char*>(base + (uint64_t)v * vs);
+        char* dv = reinterpret_cast<char*>(dst + (uint64_t)v * vs);
+        *reinterpret_cast<T*>(dv + g) = ldv<T>(bv, g) + pf * r;
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      fxm[v] = fxp[v];
+      Wn[0][v] = Wn[1][v];
+      Wn[1][v] = Wn[2][v];
+      Wn[2][v] = Wn[3][v];
+    }
+    load_col(3, i + 3);
+    if (i + 1 < i1) store_plane(cur ^ 1);
+  }
 }
 
 // plain vectorised copy (PatchIterator::copyField); also the STREAM-style

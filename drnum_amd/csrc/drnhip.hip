@@ -70,6 +70,11 @@ struct drn_ctx {
   TileMarch* d_mt_launch = nullptr;
   int n_mt = 0;
   int lds_cells = 0, lds_faces = 0, lds_sy = 0, lds_sz = 0;
+  std::vector<TileWave> wt_pp;  // wave-tile items per patch (per-patch form)
+  std::vector<int> wt_first;    // per patch, size np+1 (into wt_pp)
+  TileWave* d_wt_pp = nullptr;
+  TileWave* d_wt_launch = nullptr;  // batched form, in launch order
+  int n_wt = 0;
   std::vector<int> thin;      // patches routed to the cell kernel
   std::vector<TileCell> thin_tiles;
   TileCell* d_thin_tiles = nullptr;
@@ -163,7 +168,7 @@ static void timer_end(drn_ctx* ctx, hipEvent_t stop) {
 constexpr int MARCH_BS = 256;
 
 // tiling choices: kernel, i-range length (0 = whole patch) and work-list order
-enum { K_MARCH = 0, K_CELL = 1, K_COLUMN = 2, K_CELL_SLOPE = 3 };
+enum { K_MARCH = 0, K_CELL = 1, K_COLUMN = 2, K_CELL_SLOPE = 3, K_WTILE = 4 };
 struct TilingChoice {
   int irange;  // planes per work item, 0 = whole patch
   int xcd;     // 1 = segment the work list so that neighbouring chunks share an XCD
@@ -174,7 +179,8 @@ static const TilingChoice kTilings[] = {
     {16, 1, K_MARCH}, {8, 0, K_MARCH},  {8, 1, K_MARCH},   {4, 0, K_MARCH},  {4, 1, K_MARCH},
     {2, 0, K_MARCH},  {0, 0, K_CELL},   {0, 0, K_COLUMN},  {0, 1, K_COLUMN}, {32, 1, K_COLUMN},
     {16, 0, K_COLUMN}, {16, 1, K_COLUMN}, {8, 1, K_COLUMN}, {4, 1, K_COLUMN}, {0, 0, K_CELL_SLOPE},
-    {0, 1, K_CELL},    {0, 1, K_CELL_SLOPE}};
+    {0, 1, K_CELL},    {0, 1, K_CELL_SLOPE}, {0, 1, K_WTILE}, {16, 1, K_WTILE}, {8, 1, K_WTILE},
+    {4, 1, K_WTILE},   {8, 0, K_WTILE}};
 static const int kNumTilings = sizeof(kTilings) / sizeof(kTilings[0]);
 
 // the column kernel covers the fast MUSCL forms with a symmetric limited slope
@@ -219,6 +225,13 @@ static void launch_cell_t(drn_ctx* ctx, const TileCell* tiles, int ntiles, const
   }
 }
 
+template <typename T, int RECON, int LIM, bool FAST>
+static void launch_wtile_t(drn_ctx* ctx, const TileWave* items, int n, const T* src, const T* base, T* dst, T factor) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_stage_wtile<T, RECON, LIM, FAST>), dim3((n + 3) / 4), dim3(256), 0, ctx->stream, ctx->d_pd,
+                     items, n, src, base, dst, ctx->vstride, factor);
+}
+
 // one launch of either kernel over a tile range (march tiles or cell tiles)
 // kind: K_CELL runs k_stage_cell on TileCell items, otherwise the marching
 // kernels (column kernel when kind == K_COLUMN and the operator allows it)
@@ -228,6 +241,8 @@ static void launch_range(drn_ctx* ctx, int kind, const void* tiles, int ntiles, 
   if (kind == K_CELL || kind == K_CELL_SLOPE)
     launch_cell_t<T, RECON, LIM, FAST>(ctx, (const TileCell*)tiles, ntiles, src, base, dst, factor,
                                        kind == K_CELL_SLOPE);
+  else if (kind == K_WTILE)
+    launch_wtile_t<T, RECON, LIM, FAST>(ctx, (const TileWave*)tiles, ntiles, src, base, dst, factor);
   else
     launch_march_t<T, RECON, LIM, FAST>(ctx, (const TileMarch*)tiles, ntiles, src, base, dst, factor,
                                         kind == K_COLUMN);
@@ -279,6 +294,9 @@ static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int l
     const int kind = effective_kind(ctx, ctx->tiling);
     if (kind == K_CELL || kind == K_CELL_SLOPE) {
       go(kind, kTilings[ctx->tiling].xcd ? ctx->d_tiles_xcd : ctx->d_tiles, (int)ctx->tiles.size());
+    } else if (kind == K_WTILE) {
+      go(K_WTILE, ctx->d_wt_launch, ctx->n_wt);
+      go(K_CELL, ctx->d_thin_tiles, (int)ctx->thin_tiles.size());
     } else {
       go(kind, ctx->d_mt_launch, ctx->n_mt);
       go(K_CELL, ctx->d_thin_tiles, (int)ctx->thin_tiles.size());
@@ -288,7 +306,10 @@ static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int l
     const int np = (int)ctx->descs.size();
     for (int p = 0; p < np; ++p) {
       const int m0 = ctx->mt_first[p], mn = ctx->mt_first[p + 1] - m0;
-      if (kind == K_CELL || kind == K_CELL_SLOPE || mn == 0) {
+      const int w0 = ctx->wt_first[p], wn = ctx->wt_first[p + 1] - w0;
+      if (kind == K_WTILE && wn > 0) {
+        go(K_WTILE, ctx->d_wt_pp + w0, wn);
+      } else if (kind == K_CELL || kind == K_CELL_SLOPE || kind == K_WTILE || mn == 0) {
         const int t0 = ctx->tile_first[p];
         const TileCell* tl = (kTilings[ctx->tiling_pp].xcd ? ctx->d_tiles_xcd_pp : ctx->d_tiles) + t0;
         go(kind == K_CELL_SLOPE ? K_CELL_SLOPE : K_CELL, tl, ctx->tile_first[p + 1] - t0);
@@ -320,6 +341,16 @@ static void march_items(const drn_patch_desc& d, int p, int L, int chunk, std::v
     for (int c0 = 0; c0 < njk; c0 += chunk) out.push_back(TileMarch{p, i0, std::min(d.ni, i0 + Lp), c0});
 }
 
+// wave-tile items of one patch for an i-range length L (0 = whole patch):
+// 8x8-lane tiles advance 7x7 columns; k tiles innermost
+static void wtile_items(const drn_patch_desc& d, int p, int L, std::vector<TileWave>& out) {
+  const int Lp = L > 0 ? L : d.ni;
+  for (int i0 = 0; i0 < d.ni; i0 += Lp)
+    for (int j0 = 0; j0 < d.nj; j0 += 7)
+      for (int k0 = 0; k0 < d.nk; k0 += 7)
+        out.push_back(TileWave{p, (int16_t)i0, (int16_t)std::min(d.ni, i0 + Lp), (int16_t)j0, (int16_t)k0, 0});
+}
+
 // cells of the (j,k) plane per marching work item: the column kernel's four
 // wavefronts own 63 cells each
 static int chunk_of(const drn_ctx* ctx, int choice) { return effective_kind(ctx, choice) == K_COLUMN ? 252 : MARCH_BS; }
@@ -329,8 +360,11 @@ static int build_march(drn_ctx* ctx) {
   const TilingChoice tb = kTilings[ctx->tiling], tp = kTilings[ctx->tiling_pp];
   const int np = (int)ctx->descs.size();
   std::vector<TileMarch> nat;
+  std::vector<TileWave> wnat;
   ctx->mt_pp.clear();
   ctx->mt_first.assign(np + 1, 0);
+  ctx->wt_pp.clear();
+  ctx->wt_first.assign(np + 1, 0);
   ctx->thin.clear();
   ctx->thin_tiles.clear();
   int lc = 0, lf = 0, lsy = 0, lsz = 0;
@@ -353,6 +387,7 @@ static int build_march(drn_ctx* ctx) {
     if (d.ni < 3 || d.nj < 3 || d.nk < 3 || d.nk > 1000 || njk > (1 << 20) || need > 120 * 1024) {
       ctx->thin.push_back(p);
       for (int t = ctx->tile_first[p]; t < ctx->tile_first[p + 1]; ++t) ctx->thin_tiles.push_back(ctx->tiles[t]);
+      ctx->wt_first[p] = (int)ctx->wt_pp.size();
       continue;
     }
     lc = std::max(lc, plc);
@@ -361,8 +396,12 @@ static int build_march(drn_ctx* ctx) {
     lsz = std::max(lsz, psz);
     march_items(d, p, tb.irange, chunk_of(ctx, ctx->tiling), nat);
     march_items(d, p, tp.irange, chunk_of(ctx, ctx->tiling_pp), ctx->mt_pp);
+    ctx->wt_first[p] = (int)ctx->wt_pp.size();
+    if (tb.kind == K_WTILE) wtile_items(d, p, tb.irange, wnat);
+    if (tp.kind == K_WTILE) wtile_items(d, p, tp.irange, ctx->wt_pp);
   }
   ctx->mt_first[np] = (int)ctx->mt_pp.size();
+  ctx->wt_first[np] = (int)ctx->wt_pp.size();
   ctx->lds_cells = lc;
   ctx->lds_faces = lf;
   ctx->lds_sy = lsy;
@@ -379,6 +418,30 @@ static int build_march(drn_ctx* ctx) {
         if (x * seg + r < n) l.push_back(nat[x * seg + r]);
     launch.swap(l);
   }
+  // wave-tile launch list: blocks of 4 items, segmented per XCD like the march list
+  ctx->n_wt = (int)wnat.size();
+  std::vector<TileWave> wlaunch = wnat;
+  if (tb.kind == K_WTILE && tb.xcd && ctx->n_wt > 0) {
+    const int nb = (ctx->n_wt + 3) / 4, seg = (nb + 7) / 8;
+    std::vector<TileWave> l;
+    l.reserve(ctx->n_wt);
+    for (int r = 0; r < seg; ++r)
+      for (int x = 0; x < 8; ++x) {
+        const int b = x * seg + r;
+        if (b < nb)
+          for (int q = 4 * b; q < std::min(4 * b + 4, ctx->n_wt); ++q) l.push_back(wnat[q]);
+      }
+    wlaunch.swap(l);
+  }
+  dfree(ctx->d_wt_pp);
+  dfree(ctx->d_wt_launch);
+  ctx->d_wt_pp = ctx->d_wt_launch = nullptr;
+  HIPCHK(ctx, dalloc(&ctx->d_wt_pp, ctx->wt_pp.size()));
+  HIPCHK(ctx, dalloc(&ctx->d_wt_launch, wlaunch.size()));
+  if (!ctx->wt_pp.empty())
+    HIPCHK(ctx, hipMemcpy(ctx->d_wt_pp, ctx->wt_pp.data(), ctx->wt_pp.size() * sizeof(TileWave), hipMemcpyHostToDevice));
+  if (!wlaunch.empty())
+    HIPCHK(ctx, hipMemcpy(ctx->d_wt_launch, wlaunch.data(), wlaunch.size() * sizeof(TileWave), hipMemcpyHostToDevice));
   dfree(ctx->d_mt_pp);
   dfree(ctx->d_mt_launch);
   dfree(ctx->d_thin_tiles);
@@ -549,6 +612,8 @@ void drn_ctx_destroy(drn_ctx* ctx) {
   dfree(ctx->d_mt_pp);
   dfree(ctx->d_mt_launch);
   dfree(ctx->d_thin_tiles);
+  dfree(ctx->d_wt_pp);
+  dfree(ctx->d_wt_launch);
   dfree(ctx->d_part);
   dfree(ctx->d_out);
   for (auto& s : ctx->timers) {
@@ -591,6 +656,7 @@ int drn_patch_add(drn_ctx* ctx, const drn_patch_desc* d, int* id) {
   if (!ctx || !d) return DRN_E_ARG;
   if (ctx->laid_out) return fail(ctx, DRN_E_STATE, "patches must be added before drn_layout");
   if (d->ni < 1 || d->nj < 1 || d->nk < 1) return fail(ctx, DRN_E_ARG, "patch sizes must be positive");
+  if (d->ni > 32767 || d->nj > 32767 || d->nk > 32767) return fail(ctx, DRN_E_ARG, "patch side above 32767 cells");
   if ((uint64_t)d->ni * d->nj * d->nk > (1ull << 31)) return fail(ctx, DRN_E_ARG, "patch too large");
   for (int f = 0; f < 6; ++f) {
     if (d->bc[f] < DRN_BC_SLIP || d->bc[f] > DRN_BC_NONE) return fail(ctx, DRN_E_ARG, "unknown face functor");
@@ -647,7 +713,9 @@ int drn_layout(drn_ctx* ctx) {
       for (int c2 = 0; c2 < njk; c2 += 256) ctx->tiles.push_back(TileCell{p, i, c2, 0});
   }
   ctx->tile_first[np] = (int)ctx->tiles.size();
-  if (off >= (1ull << 32)) return fail(ctx, DRN_E_ARG, "grid exceeds 2^32 cells");
+  // the cell kernel addresses a variable array with 32-bit byte offsets
+  if (off * ctx->esize() >= (1ull << 32))
+    return fail(ctx, DRN_E_ARG, "grid exceeds 2^32 bytes per variable array (2^30 FP32 / 2^29 FP64 cells)");
   ctx->vstride = off;
   for (int b = 0; b < NBUF; ++b) {
     if (hipMalloc(&ctx->buf[b], 5 * off * ctx->esize()) != hipSuccess) return fail(ctx, DRN_E_NOMEM, "state buffer allocation failed");

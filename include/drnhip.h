@@ -94,7 +94,9 @@ int drn_set_operator(drn_ctx* ctx, int recon, int limiter);
 /* Replaces PatchIterator::addPatch (patchiterator.h:124-128) + the lazy device
  * mirror GPU_Patch (gpu_patch.h:43-129).  Patches are added before drn_layout. */
 int drn_patch_add(drn_ctx* ctx, const drn_patch_desc* desc, int* id);
-/* Allocate the three state buffers for all added patches (gpu_patch.h:57-128). */
+/* Allocate the three state buffers for all added patches (gpu_patch.h:57-128).
+ * One variable array of the grid (all patches, each padded to 64 cells) must
+ * stay below 2^32 bytes: 2^30 stored cells in FP32, 2^29 in FP64. */
 int drn_layout(drn_ctx* ctx);
 int drn_num_patches(const drn_ctx* ctx);
 size_t drn_patch_cells(const drn_ctx* ctx, int id);
@@ -181,7 +183,12 @@ int drn_diagnostics(drn_ctx* ctx, int field, double dt, double* out);
  * 19    the cell-per-thread kernel with per-cell limited slopes (fast MUSCL
  *       van Albada / minmod only, otherwise 11);
  * 20    as 11, work list in 8 contiguous segments, one per XCD;
- * 21    as 19, work list in 8 contiguous segments, one per XCD.
+ * 21    as 19, work list in 8 contiguous segments, one per XCD;
+ * 22-26 the wave-tile kernel: each wavefront owns an 8x8 (j,k) lane tile and
+ *       marches over an i-range of the whole patch / 16 / 8 / 4 planes (XCD
+ *       segmented) or 8 planes (natural order); every face is evaluated once
+ *       per tile, y/z neighbours come from a wave-private LDS plane tile and
+ *       lane shuffles, with no block-level barrier.
  * Default for both forms: 11; drn_autotune picks per device. */
 int drn_num_tilings(void);
 int drn_set_tiling(drn_ctx* ctx, int launch, int choice);
