@@ -50,6 +50,10 @@ def setup(w, dtype, launch_tables=True):
     hg.to_device(ctx)
     if launch_tables:
         hg.upload_exchange(ctx)
+    if w.spheres is not None:
+        # slip level-set fill after every stage (the post operation of the RK step)
+        hg.levelset_spheres(w.spheres, w.ls_inner_layers, 0, w.ls_min_inner_rel)
+        hg.upload_levelset(ctx)
     for p in range(len(w.patches)):
         ctx.upload(p, 0, w.initial_state(p, dtype))
     return hg, ctx
@@ -129,6 +133,23 @@ def run_form(w, dtype, steps, warmup, launch, label, tiling=None):
     return res
 
 
+def _workloads():
+    from drnum_amd import configs
+
+    return {
+        "jet": (configs.jet, "subsonic jet, 256^3 core as 4x4x4 patches of 64^3 core + 2 seek layers on interior "
+                             "faces; AUSM+ MUSCL van Albada, 3-stage RK, exchange every stage"),
+        "uneven": (configs.uneven_batch, "uneven batch: 200 patches of 16-96 cells per axis with 2-4 cell overlaps, "
+                                         "20% nested at ratio 2; Mach 0.5 + density pulse"),
+        "spheres": (configs.levelset_spheres, "level set: 192x128x128 core as 12 patches, 8 spheres of radius 6-14 "
+                                              "cells, Mach 0.3 farfield, slip level-set fill every stage"),
+        "kh": (configs.kelvin_helmholtz, "periodic Kelvin-Helmholtz box, 512^3 core as 8x8x8 patches of 64^3 + 2 "
+                                         "seek layers on every face"),
+    }
+
+
+WORKLOADS = _workloads()
+
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "stage_traffic_jet4x64_fp32.md")
 
 
@@ -180,6 +201,8 @@ def main():
     ap.add_argument("--blocks", type=int, default=4)
     ap.add_argument("--core", type=int, default=64)
     ap.add_argument("--tiling", type=int, default=None, help="fixed stage work-list choice (default: autotune)")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="jet",
+                    help="jet (the headline case) or one of the other configurations")
     ap.add_argument("--fp32-only", action="store_true", help="skip the FP64 per-patch form (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -189,7 +212,7 @@ def main():
 
     from drnum_amd import configs, hip
 
-    w = configs.jet(core=args.core, blocks=args.blocks)
+    w = configs.jet(core=args.core, blocks=args.blocks) if args.workload == "jet" else WORKLOADS[args.workload][0]()
     log("workload %s: %d patches, %d stored cells, %d core cells" % (w.name, len(w.patches), w.total_cells,
                                                                       w.core_cells))
     f32 = run_form(w, np.float32, args.steps, args.warmup, hip.DRN_LAUNCH_BATCHED, "fp32-batched", args.tiling)
@@ -198,13 +221,13 @@ def main():
         f64 = run_form(w, np.float64, args.fp64_steps or args.steps, args.warmup, hip.DRN_LAUNCH_PER_PATCH,
                        "fp64-plain", args.tiling)
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload == "jet":
         threads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(w, np.float32, threads)
         log("cpu baseline", cpu)
 
     ach = f32["stage_gbs"]
-    default_workload = (args.core, args.blocks) == (64, 4)
+    default_workload = args.workload == "jet" and (args.core, args.blocks) == (64, 4)
     traffic = measured_traffic(f32["tiling"]) if default_workload else None
     out = {
         "metric": METRIC,
@@ -220,8 +243,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "subsonic jet, 256^3 core as 4x4x4 patches of 64^3 core + 2 seek layers on interior "
-                        "faces; AUSM+ MUSCL van Albada, 3-stage RK, exchange every stage",
+            "workload": WORKLOADS[args.workload][1] if (args.workload != "jet" or default_workload) else w.name,
             "patches": len(w.patches),
             "cells_stored": w.total_cells,
             "cells_core": w.core_cells,

@@ -127,6 +127,25 @@ std::vector<size_t> CartesianPatch<G>::seekCells() const {
 
 // linearCCInterpolCoeffs (cartesianpatch.cpp:1053-1162)
 template <typename G>
+bool CartesianPatch<G>::donorZoneMayContain(const double* lo, const double* hi) const {
+  for (int a = 0; a < 3; ++a) {
+    const double d = (double)d_[a];
+    const double z0 = (double)dz_first_[a] * d, z1 = (double)dz_after_[a] * d;
+    const double tol = 1e-6 * (std::fabs(z0) + std::fabs(z1) + std::fabs(lo[a]) + std::fabs(hi[a]) + d);
+    if (hi[a] < z0 - tol || lo[a] > z1 + tol) return false;
+  }
+  return true;
+}
+
+template <typename G>
+void CartesianPatch<G>::cellCentreBox(double* lo, double* hi) const {
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = (double)ccmin_[a];
+    hi[a] = (double)ccmin_[a] + (double)(n_[a] - 1) * (double)d_[a];
+  }
+}
+
+template <typename G>
 int CartesianPatch<G>::lin(G s_test, int axis, size_t& ref, G& coeff) const {
   const G m_eps = 1.e-5;  // cartesianpatch.cpp:40
   const size_t first = dz_first_[axis], afterlast = dz_after_[axis], num = (size_t)n_[axis];
@@ -329,6 +348,15 @@ int PatchGrid<G>::computeDependencies(const double* period, int threads) {
         // translation receiver -> donor frame (coordtransform.cpp concatenate + transform)
         G b[3];
         for (int a = 0; a < 3; ++a) b[a] = P.origin(a) - oq[a];
+        {
+          double lo[3], hi[3];
+          P.cellCentreBox(lo, hi);
+          for (int a = 0; a < 3; ++a) {
+            lo[a] += (double)b[a];
+            hi[a] += (double)b[a];
+          }
+          if (!Q.donorZoneMayContain(lo, hi)) continue;
+        }
         std::vector<Contribution<G>> list;
         for (size_t ll = 0; ll < recv.size(); ++ll) {
           G x, y, z;

@@ -64,6 +64,12 @@ class CartesianPatch {
   // seek cells, sorted and unique (cartesianpatch.cpp:393-491, patch.cpp:468-486)
   std::vector<size_t> seekCells() const;
   bool isCore(size_t i, size_t j, size_t k) const;
+  // false when no point of the box [lo, hi] (patch coordinates) can fall into
+  // the donor zone, with a margin far above rounding: a candidate-pair prune
+  // that changes no table (points outside the zone get no coefficients)
+  bool donorZoneMayContain(const double* lo, const double* hi) const;
+  // cell-centre bounding box of the whole patch in its own coordinates
+  void cellCentreBox(double* lo, double* hi) const;
 
   // face functors and their parameters (drn_patch_desc)
   int face[6] = {FACE_SLIP, FACE_SLIP, FACE_SLIP, FACE_SLIP, FACE_SLIP, FACE_SLIP};

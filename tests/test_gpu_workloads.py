@@ -109,6 +109,32 @@ def test_uneven_nested_batch(gpu, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_every_tiling_on_uneven_patches(gpu, dtype):
+    """Every stage-kernel work-list choice on patches of uneven sizes (10-20 cells
+    per axis, nested ratio-2 patches): tile seams that do not divide the patch."""
+    w = configs.uneven_batch(n_patches=12, size_range=(10, 20))
+    og = O.OracleGrid(w, dtype, threads=8)
+    og.init_state()
+    og.compute_dependencies()
+    for _ in range(3):
+        og.rk_step()
+    tables = tables_of(og)
+    n = hip.lib().drn_num_tilings()
+    for choice in range(n):
+        g2 = solver.GpuGrid(w, dtype)
+        g2.upload_initial()
+        g2.set_tables(tables)
+        g2.ctx.set_tiling(choice, hip.DRN_LAUNCH_BATCHED)
+        g2.ctx.set_tiling(choice, hip.DRN_LAUNCH_PER_PATCH)
+        for s in range(3):
+            g2.rk_step(hip.DRN_LAUNCH_BATCHED if s % 2 == 0 else hip.DRN_LAUNCH_PER_PATCH)
+        err = rel_linf(g2.state(0), og)
+        print("uneven tiling", choice, np.dtype(dtype).name, err, flush=True)
+        g2.close()
+        assert err <= TOL[dtype], (choice, err)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_host_tables_drive_the_device(gpu, dtype):
     """Geometry, donor tables and level-set tables built by libdrnum.so and handed
     to the device library directly (drnh_grid_to_device / upload_*)."""
