@@ -62,29 +62,94 @@ __device__ __forceinline__ T ldv(const char* base, uint32_t boff) {
 template <typename T, int RECON, int LIM, bool FAST, int DIR>
 __device__ __forceinline__ void face_flux_g(const char* const* sv, uint32_t cm2, uint32_t cm1, uint32_t cp1,
                                             uint32_t cp2, bool has_m2, bool has_p2, T A, T* f) {
-  T vl[5], vr[5];
+  T ua[5], ub[5], vl[5], vr[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
-    vl[v] = ldv<T>(sv[v], cm1);
-    vr[v] = ldv<T>(sv[v], cp1);
+    ua[v] = ldv<T>(sv[v], cm1);
+    ub[v] = ldv<T>(sv[v], cp1);
+    vl[v] = ua[v];
+    vr[v] = ub[v];
   }
   if constexpr (RECON == RECON_UPWIND2) {
     if (has_m2) {
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        T a = vl[v], b = vr[v], m = ldv<T>(sv[v], cm2);
-        vl[v] = a + T(0.5) * limited<T, LIM, FAST>(a - m, b - a);
+        const T m = ldv<T>(sv[v], cm2);
+        vl[v] = ua[v] + T(0.5) * limited<T, LIM, FAST>(ua[v] - m, ub[v] - ua[v]);
       }
     }
     if (has_p2) {
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        T a = ldv<T>(sv[v], cm1), b = ldv<T>(sv[v], cp1), q = ldv<T>(sv[v], cp2);
-        vr[v] = b + T(0.5) * limited<T, LIM, FAST>(b - q, a - b);
+        const T q = ldv<T>(sv[v], cp2);
+        vr[v] = ub[v] + T(0.5) * limited<T, LIM, FAST>(ub[v] - q, ua[v] - ub[v]);
       }
     }
   }
   ausm_flux<T, FAST, DIR>(vl, vr, A, f);
+}
+
+// the same face from values already in registers: a = cell on the minus side,
+// b = cell on the plus side, m / q the next cells outwards (valid when hm / hp)
+template <typename T, int RECON, int LIM, bool FAST, int DIR>
+__device__ __forceinline__ void face_flux_r(const T* m, const T* a, const T* b, const T* q, bool hm, bool hp, T A,
+                                            T* f) {
+  T vl[5], vr[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    vl[v] = a[v];
+    vr[v] = b[v];
+  }
+  if constexpr (RECON == RECON_UPWIND2) {
+    if (hm) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) vl[v] = a[v] + T(0.5) * limited<T, LIM, FAST>(a[v] - m[v], b[v] - a[v]);
+    }
+    if (hp) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) vr[v] = b[v] + T(0.5) * limited<T, LIM, FAST>(b[v] - q[v], a[v] - b[v]);
+    }
+  }
+  ausm_flux<T, FAST, DIR>(vl, vr, A, f);
+}
+
+// the two faces of a cell along DIR (index idx of n along that axis, byte
+// stride st): loads the four neighbours once, adds the minus face to res and
+// returns the plus face in fp (zero when the face does not exist)
+template <typename T, int RECON, int LIM, bool FAST, int DIR>
+__device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, uint32_t st, int idx, int n,
+                                             const T* u0, T A, T* res, T* fp) {
+  T um2[5], um1[5], up1[5], up2[5];
+  const bool have = n > 2;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    um2[v] = um1[v] = up1[v] = up2[v] = u0[v];
+  }
+  if (have && idx >= 2) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) um2[v] = ldv<T>(sv[v], c - 2 * st);
+  }
+  if (have && idx >= 1) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) um1[v] = ldv<T>(sv[v], c - st);
+  }
+  if (have && idx + 1 < n) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) up1[v] = ldv<T>(sv[v], c + st);
+  }
+  if (have && idx + 2 < n) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) up2[v] = ldv<T>(sv[v], c + 2 * st);
+  }
+  zero5(fp);
+  if (have && idx > 0) {
+    T f[5];
+    zero5(f);
+    face_flux_r<T, RECON, LIM, FAST, DIR>(um2, um1, u0, up1, idx >= 2, idx + 1 < n, A, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] += f[v];
+  }
+  if (have && idx < n - 1) face_flux_r<T, RECON, LIM, FAST, DIR>(um1, u0, up1, up2, idx >= 1, idx + 2 < n, A, fp);
 }
 
 template <typename T, int RECON, int LIM, bool FAST, int BS>
@@ -110,52 +175,27 @@ __global__ __launch_bounds__(BS) void k_stage_cell(const PatchDev* __restrict__ 
   const T Ax = dy * dz, Ay = dx * dz, Az = dx * dy;
   T res[5], f[5];
   zero5(res);
-  // minus faces (visited at this cell's own iteration), then plus faces in the
-  // order the later cells of the reference loop subtract them
-  if (i > 0 && ni > 2) {
-    zero5(f);
-    face_flux_g<T, RECON, LIM, FAST, 0>(sv, c - 2 * sx, c - sx, c, c + sx, i >= 2, i + 1 < ni, Ax, f);
+  // Residual in the reference's order Fx- + Fy- + Fz- - Fz+ - Fy+ - Fx+: each
+  // axis loads its four neighbours once, adds its minus face and keeps its
+  // plus face until the end.
+  T u0[5], fxp[5], fyp[5], fzp[5];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) res[v] += f[v];
-  }
-  if (j > 0 && nj > 2) {
-    zero5(f);
-    face_flux_g<T, RECON, LIM, FAST, 1>(sv, c - 2 * sy, c - sy, c, c + sy, j >= 2, j + 1 < nj, Ay, f);
+  for (int v = 0; v < 5; ++v) u0[v] = ldv<T>(sv[v], c);
+  axis_faces_r<T, RECON, LIM, FAST, 0>(sv, c, sx, i, ni, u0, Ax, res, fxp);
+  axis_faces_r<T, RECON, LIM, FAST, 1>(sv, c, sy, j, nj, u0, Ay, res, fyp);
+  axis_faces_r<T, RECON, LIM, FAST, 2>(sv, c, W, k, nk, u0, Az, res, fzp);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) res[v] += f[v];
-  }
-  if (k > 0 && nk > 2) {
-    zero5(f);
-    face_flux_g<T, RECON, LIM, FAST, 2>(sv, c - 2 * W, c - W, c, c + W, k >= 2, k + 1 < nk, Az, f);
-#pragma unroll
-    for (int v = 0; v < 5; ++v) res[v] += f[v];
-  }
-  if (k < nk - 1 && nk > 2) {
-    zero5(f);
-    face_flux_g<T, RECON, LIM, FAST, 2>(sv, c - W, c, c + W, c + 2 * W, k >= 1, k + 2 < nk, Az, f);
-#pragma unroll
-    for (int v = 0; v < 5; ++v) res[v] -= f[v];
-  }
-  if (j < nj - 1 && nj > 2) {
-    zero5(f);
-    face_flux_g<T, RECON, LIM, FAST, 1>(sv, c - sy, c, c + sy, c + 2 * sy, j >= 1, j + 2 < nj, Ay, f);
-#pragma unroll
-    for (int v = 0; v < 5; ++v) res[v] -= f[v];
-  }
-  if (i < ni - 1 && ni > 2) {
-    zero5(f);
-    face_flux_g<T, RECON, LIM, FAST, 0>(sv, c - sx, c, c + sx, c + 2 * sx, i >= 1, i + 2 < ni, Ax, f);
-#pragma unroll
-    for (int v = 0; v < 5; ++v) res[v] -= f[v];
+  for (int v = 0; v < 5; ++v) {
+    res[v] -= fzp[v];
+    res[v] -= fyp[v];
+    res[v] -= fxp[v];
   }
   // patch-face functors (cartesianiterator.h:190-308)
   const bool wx = ni > 2 && (i == 0 || i == ni - 1);
   const bool wy = nj > 2 && (j == 0 || j == nj - 1);
   const bool wz = nk > 2 && (k == 0 || k == nk - 1);
   if (wx || wy || wz) {
-    T var[5];
-#pragma unroll
-    for (int v = 0; v < 5; ++v) var[v] = ldv<T>(sv[v], c);
+    const T* var = u0;
     const T xc = T(0.5) * dx + T(i) * dx, yc = T(0.5) * dy + T(j) * dy, zc = T(0.5) * dz + T(k) * dz;
     if (ni > 2) {
       if (i == 0) {
@@ -913,19 +953,18 @@ __global__ __launch_bounds__(256) void k_stage_column(const PatchDev* __restrict
 // four MUSCL states of the two faces are U -/+ s/2; the own cell's slope serves
 // both faces.  Nine slope evaluations per cell instead of twelve reconstructions.
 template <typename T, int LIM, int DIR>
-__device__ __forceinline__ void axis_faces(const PatchDev& P, const T* __restrict__ src, uint64_t vs, uint64_t g,
-                                           uint64_t st, int pos, int n, const T* U0, T A, T ta, T tb, int la, int lb,
-                                           T* res) {
+__device__ __forceinline__ void axis_faces(const PatchDev& P, const char* const* sv, uint32_t g, uint32_t st, int pos,
+                                           int n, const T* U0, T A, T ta, T tb, int la, int lb, T* res) {
   const T half = T(0.5);
   if (n <= 2) return;  // the reference skips an axis with at most two cells
   T Um[5], Umm[5], Up[5], Upp[5];
   const bool hm = pos >= 1, hmm = pos >= 2, hp = pos + 1 < n, hpp = pos + 2 < n;
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
-    Um[v] = hm ? src[v * vs + g - st] : U0[v];
-    Umm[v] = hmm ? src[v * vs + g - 2 * st] : Um[v];
-    Up[v] = hp ? src[v * vs + g + st] : U0[v];
-    Upp[v] = hpp ? src[v * vs + g + 2 * st] : Up[v];
+    Um[v] = hm ? ldv<T>(sv[v], g - st) : U0[v];
+    Umm[v] = hmm ? ldv<T>(sv[v], g - 2 * st) : Um[v];
+    Up[v] = hp ? ldv<T>(sv[v], g + st) : U0[v];
+    Upp[v] = hpp ? ldv<T>(sv[v], g + 2 * st) : Up[v];
   }
   T s0[5];
   const bool ok0 = hm && hp;
@@ -979,22 +1018,30 @@ __global__ __launch_bounds__(BS) void k_stage_cell_slope(const PatchDev* __restr
   const int i = t.i;
   const int j = fdiv_idx(c2, 1.0f / (float)nk);
   const int k = c2 - j * nk;
-  const uint64_t g = (uint64_t)P.off + (uint64_t)i * (uint64_t)njk + (uint64_t)c2;
+  constexpr uint32_t W = sizeof(T);
+  const uint32_t g = (P.off + (uint32_t)i * (uint32_t)njk + (uint32_t)c2) * W;
+  const char* sv[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) sv[v] = reinterpret_cast<const char*>(src + (uint64_t)v * vs);
   const T dx = T(P.dx), dy = T(P.dy), dz = T(P.dz);
   const T half = T(0.5);
   const T xc = half * dx + T(i) * dx, yc = half * dy + T(j) * dy, zc = half * dz + T(k) * dz;
   T U0[5], res[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
-    U0[v] = src[v * vs + g];
+    U0[v] = ldv<T>(sv[v], g);
     res[v] = T(0);
   }
-  axis_faces<T, LIM, 0>(P, src, vs, g, (uint64_t)njk, i, P.ni, U0, dy * dz, yc, zc, j, k, res);
-  axis_faces<T, LIM, 1>(P, src, vs, g, (uint64_t)nk, j, P.nj, U0, dx * dz, xc, zc, i, k, res);
-  axis_faces<T, LIM, 2>(P, src, vs, g, 1, k, nk, U0, dx * dy, xc, yc, i, j, res);
+  axis_faces<T, LIM, 0>(P, sv, g, (uint32_t)njk * W, i, P.ni, U0, dy * dz, yc, zc, j, k, res);
+  axis_faces<T, LIM, 1>(P, sv, g, (uint32_t)nk * W, j, P.nj, U0, dx * dz, xc, zc, i, k, res);
+  axis_faces<T, LIM, 2>(P, sv, g, W, k, nk, U0, dx * dy, xc, yc, i, j, res);
   const T pf = factor / (dx * dy * dz);
 #pragma unroll
-  for (int v = 0; v < 5; ++v) dst[v * vs + g] = base[v * vs + g] + pf * res[v];
+  for (int v = 0; v < 5; ++v) {
+    const char* bv = reinterpret_cast<const char*>(base + (uint64_t)v * vs);
+    char* dv = reinterpret_cast<char*>(dst + (uint64_t)v * vs);
+    *reinterpret_cast<T*>(dv + g) = ldv<T>(bv, g) + pf * res[v];
+  }
 }
 
 // ---------------------------------------------------------------------------

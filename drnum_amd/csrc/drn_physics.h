@@ -77,20 +77,27 @@ __device__ __forceinline__ T limited(T d01, T d12) {
   const T eps = T(1e-10);  // GLOBAL_EPS, drnum.h:70
   if constexpr (LIM == LIM_VANALBADA) {
     if constexpr (FAST) {
-      // (r^2 + r)/(r^2 + 1) * d01 with r = d12/e, e = nonZero(d01):
-      // multiply through by e^2 -> one division
-      T e = non_zero(d01, eps);
-      T num = d12 * d12 + d12 * e;
-      T den = d12 * d12 + e * e;
-      return ddiv<T, FAST>(num, den) * d01;
+      // (r^2 + r)/(r^2 + 1) * d01 with r = d12/d01, multiplied through by d01^2:
+      // one reciprocal.  The reference's nonZero(d01, 1e-10) guard is replaced by
+      // 1e-30 in the denominator; both give 0 for d01 = 0 and differ only where
+      // |d01| or |d12| is below 1e-15, far under the FP32 resolution of the state.
+      const T num = d12 * d12 + d12 * d01;
+      const T den = d12 * d12 + d01 * d01 + T(1e-30);
+      return d01 * num * drcp<T, FAST>(den);
     } else {
       T r = d12 / non_zero(d01, eps);
       return ((r * r + r) / (r * r + T(1))) * d01;
     }
   } else if constexpr (LIM == LIM_MINMOD) {
-    T t = ddiv<T, FAST>(d12, non_zero(d01, eps));
-    T m = (t < T(1)) ? t : T(1);
-    return ((T(0) < m) ? m : T(0)) * d01;
+    if constexpr (FAST) {
+      // max(0, min(1, d12/d01)) * d01 without the division: the smaller of the
+      // two differences when they have the same sign, else 0
+      return (d01 * d12 > T(0)) ? (dabs(d01) < dabs(d12) ? d01 : d12) : T(0);
+    } else {
+      T t = d12 / non_zero(d01, eps);
+      T m = (t < T(1)) ? t : T(1);
+      return ((T(0) < m) ? m : T(0)) * d01;
+    }
   } else {
     return d01;
   }
