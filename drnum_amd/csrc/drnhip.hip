@@ -58,6 +58,12 @@ struct drn_ctx {
   TileCell* d_tiles = nullptr;
   TileCell* d_tiles_xcd = nullptr;     // the list in 8 contiguous segments, one per XCD
   TileCell* d_tiles_xcd_pp = nullptr;  // the same per patch (same tile_first offsets)
+  // 252-cell items of the z-shared cell kernel (4 wavefronts x 63 cells)
+  std::vector<TileCell> tz;
+  std::vector<int> tz_first;
+  TileCell* d_tz = nullptr;
+  TileCell* d_tz_xcd = nullptr;
+  TileCell* d_tz_xcd_pp = nullptr;
   double* d_part = nullptr;  // reduction partials (5 per tile)
   double* d_out = nullptr;
   int tiling = 11;     // batched form
@@ -168,7 +174,7 @@ static void timer_end(drn_ctx* ctx, hipEvent_t stop) {
 constexpr int MARCH_BS = 256;
 
 // tiling choices: kernel, i-range length (0 = whole patch) and work-list order
-enum { K_MARCH = 0, K_CELL = 1, K_COLUMN = 2, K_CELL_SLOPE = 3, K_WTILE = 4 };
+enum { K_MARCH = 0, K_CELL = 1, K_COLUMN = 2, K_CELL_SLOPE = 3, K_WTILE = 4, K_CELLZ = 5 };
 struct TilingChoice {
   int irange;  // planes per work item, 0 = whole patch
   int xcd;     // 1 = segment the work list so that neighbouring chunks share an XCD
@@ -180,7 +186,7 @@ static const TilingChoice kTilings[] = {
     {2, 0, K_MARCH},  {0, 0, K_CELL},   {0, 0, K_COLUMN},  {0, 1, K_COLUMN}, {32, 1, K_COLUMN},
     {16, 0, K_COLUMN}, {16, 1, K_COLUMN}, {8, 1, K_COLUMN}, {4, 1, K_COLUMN}, {0, 0, K_CELL_SLOPE},
     {0, 1, K_CELL},    {0, 1, K_CELL_SLOPE}, {0, 1, K_WTILE}, {16, 1, K_WTILE}, {8, 1, K_WTILE},
-    {4, 1, K_WTILE},   {8, 0, K_WTILE}};
+    {4, 1, K_WTILE},   {8, 0, K_WTILE},      {0, 1, K_CELLZ},  {0, 0, K_CELLZ}};
 static const int kNumTilings = sizeof(kTilings) / sizeof(kTilings[0]);
 
 // the column kernel covers the fast MUSCL forms with a symmetric limited slope
@@ -243,6 +249,11 @@ static void launch_range(drn_ctx* ctx, int kind, const void* tiles, int ntiles, 
                                        kind == K_CELL_SLOPE);
   else if (kind == K_WTILE)
     launch_wtile_t<T, RECON, LIM, FAST>(ctx, (const TileWave*)tiles, ntiles, src, base, dst, factor);
+  else if (kind == K_CELLZ) {
+    if (ntiles > 0)
+      hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, FAST, 256, true>), dim3(ntiles), dim3(256), 0, ctx->stream,
+                         ctx->d_pd, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
+  }
   else
     launch_march_t<T, RECON, LIM, FAST>(ctx, (const TileMarch*)tiles, ntiles, src, base, dst, factor,
                                         kind == K_COLUMN);
@@ -292,7 +303,9 @@ static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int l
   };
   if (launch == DRN_LAUNCH_BATCHED) {
     const int kind = effective_kind(ctx, ctx->tiling);
-    if (kind == K_CELL || kind == K_CELL_SLOPE) {
+    if (kind == K_CELLZ) {
+      go(K_CELLZ, kTilings[ctx->tiling].xcd ? ctx->d_tz_xcd : ctx->d_tz, (int)ctx->tz.size());
+    } else if (kind == K_CELL || kind == K_CELL_SLOPE) {
       go(kind, kTilings[ctx->tiling].xcd ? ctx->d_tiles_xcd : ctx->d_tiles, (int)ctx->tiles.size());
     } else if (kind == K_WTILE) {
       go(K_WTILE, ctx->d_wt_launch, ctx->n_wt);
@@ -307,7 +320,11 @@ static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int l
     for (int p = 0; p < np; ++p) {
       const int m0 = ctx->mt_first[p], mn = ctx->mt_first[p + 1] - m0;
       const int w0 = ctx->wt_first[p], wn = ctx->wt_first[p + 1] - w0;
-      if (kind == K_WTILE && wn > 0) {
+      if (kind == K_CELLZ) {
+        const int z0 = ctx->tz_first[p];
+        const TileCell* tl = (kTilings[ctx->tiling_pp].xcd ? ctx->d_tz_xcd_pp : ctx->d_tz) + z0;
+        go(K_CELLZ, tl, ctx->tz_first[p + 1] - z0);
+      } else if (kind == K_WTILE && wn > 0) {
         go(K_WTILE, ctx->d_wt_pp + w0, wn);
       } else if (kind == K_CELL || kind == K_CELL_SLOPE || kind == K_WTILE || mn == 0) {
         const int t0 = ctx->tile_first[p];
@@ -609,6 +626,9 @@ void drn_ctx_destroy(drn_ctx* ctx) {
   dfree(ctx->d_tiles);
   dfree(ctx->d_tiles_xcd);
   dfree(ctx->d_tiles_xcd_pp);
+  dfree(ctx->d_tz);
+  dfree(ctx->d_tz_xcd);
+  dfree(ctx->d_tz_xcd_pp);
   dfree(ctx->d_mt_pp);
   dfree(ctx->d_mt_launch);
   dfree(ctx->d_thin_tiles);
@@ -713,6 +733,16 @@ int drn_layout(drn_ctx* ctx) {
       for (int c2 = 0; c2 < njk; c2 += 256) ctx->tiles.push_back(TileCell{p, i, c2, 0});
   }
   ctx->tile_first[np] = (int)ctx->tiles.size();
+  ctx->tz.clear();
+  ctx->tz_first.assign(np + 1, 0);
+  for (int p = 0; p < np; ++p) {
+    const auto& d = ctx->descs[p];
+    ctx->tz_first[p] = (int)ctx->tz.size();
+    const int njk = d.nj * d.nk;
+    for (int i = 0; i < d.ni; ++i)
+      for (int c2 = 0; c2 < njk; c2 += 252) ctx->tz.push_back(TileCell{p, i, c2, 0});
+  }
+  ctx->tz_first[np] = (int)ctx->tz.size();
   // the cell kernel addresses a variable array with 32-bit byte offsets
   if (off * ctx->esize() >= (1ull << 32))
     return fail(ctx, DRN_E_ARG, "grid exceeds 2^32 bytes per variable array (2^30 FP32 / 2^29 FP64 cells)");
@@ -747,6 +777,17 @@ int drn_layout(drn_ctx* ctx) {
     HIPCHK(ctx, dalloc(&ctx->d_tiles_xcd_pp, nt));
     HIPCHK(ctx, hipMemcpy(ctx->d_tiles_xcd, all.data(), nt * sizeof(TileCell), hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(ctx->d_tiles_xcd_pp, pp.data(), nt * sizeof(TileCell), hipMemcpyHostToDevice));
+    const int nz = (int)ctx->tz.size();
+    std::vector<TileCell> zall(nz), zpp(nz);
+    segment(ctx->tz.data(), nz, zall.data());
+    for (int p = 0; p < np; ++p)
+      segment(ctx->tz.data() + ctx->tz_first[p], ctx->tz_first[p + 1] - ctx->tz_first[p], zpp.data() + ctx->tz_first[p]);
+    HIPCHK(ctx, dalloc(&ctx->d_tz, nz));
+    HIPCHK(ctx, dalloc(&ctx->d_tz_xcd, nz));
+    HIPCHK(ctx, dalloc(&ctx->d_tz_xcd_pp, nz));
+    HIPCHK(ctx, hipMemcpy(ctx->d_tz, ctx->tz.data(), nz * sizeof(TileCell), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->d_tz_xcd, zall.data(), nz * sizeof(TileCell), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->d_tz_xcd_pp, zpp.data(), nz * sizeof(TileCell), hipMemcpyHostToDevice));
   }
   HIPCHK(ctx, dalloc(&ctx->d_part, ctx->tiles.size() * 5));
   HIPCHK(ctx, dalloc(&ctx->d_out, 8));

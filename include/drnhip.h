@@ -188,7 +188,10 @@ int drn_diagnostics(drn_ctx* ctx, int field, double dt, double* out);
  *       marches over an i-range of the whole patch / 16 / 8 / 4 planes (XCD
  *       segmented) or 8 planes (natural order); every face is evaluated once
  *       per tile, y/z neighbours come from a wave-private LDS plane tile and
- *       lane shuffles, with no block-level barrier.
+ *       lane shuffles, with no block-level barrier;
+ * 27, 28 the cell-per-thread kernel with 63 cells per wavefront whose z faces
+ *       are evaluated once and handed to the neighbouring lane (XCD segmented /
+ *       natural order).
  * Default for both forms: 11; drn_autotune picks per device. */
 int drn_num_tilings(void);
 int drn_set_tiling(drn_ctx* ctx, int launch, int choice);
