@@ -142,14 +142,52 @@ __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, 
     for (int v = 0; v < 5; ++v) up2[v] = ldv<T>(sv[v], c + 2 * st);
   }
   zero5(fp);
-  if (have && idx > 0) {
-    T f[5];
-    zero5(f);
-    face_flux_r<T, RECON, LIM, FAST, DIR>(um2, um1, u0, up1, idx >= 2, idx + 1 < n, A, f);
+  if constexpr (FAST && RECON == RECON_UPWIND2 && (LIM == LIM_VANALBADA || LIM == LIM_MINMOD)) {
+    // Fast arithmetic: the limited slope term of a cell, lim(dL, dR)*dL with
+    // dL = U_c - U_c-1, dR = U_c+1 - U_c, is symmetric in (dL, dR) for van
+    // Albada and minmod, and the plus-side state of a face uses its negative
+    // (upwind2.h:44 with the cell as i1 and the face neighbour as i2).  So the
+    // three slopes of the cells c-1, c, c+1 serve both faces of the axis.
+    if (!have) return;
+    T sm[5], s0[5], sp[5];
+    const bool okm = idx >= 2, ok0 = idx >= 1 && idx + 1 < n, okp = idx + 2 < n;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) res[v] += f[v];
+    for (int v = 0; v < 5; ++v) {
+      sm[v] = okm ? limited<T, LIM, true>(um1[v] - um2[v], u0[v] - um1[v]) : T(0);
+      s0[v] = ok0 ? limited<T, LIM, true>(u0[v] - um1[v], up1[v] - u0[v]) : T(0);
+      sp[v] = okp ? limited<T, LIM, true>(up1[v] - u0[v], up2[v] - up1[v]) : T(0);
+    }
+    if (idx > 0) {
+      T vl[5], vr[5], f[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        vl[v] = um1[v] + T(0.5) * sm[v];
+        vr[v] = u0[v] - T(0.5) * s0[v];
+      }
+      zero5(f);
+      ausm_flux<T, FAST, DIR>(vl, vr, A, f);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) res[v] += f[v];
+    }
+    if (idx < n - 1) {
+      T vl[5], vr[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        vl[v] = u0[v] + T(0.5) * s0[v];
+        vr[v] = up1[v] - T(0.5) * sp[v];
+      }
+      ausm_flux<T, FAST, DIR>(vl, vr, A, fp);
+    }
+  } else {
+    if (have && idx > 0) {
+      T f[5];
+      zero5(f);
+      face_flux_r<T, RECON, LIM, FAST, DIR>(um2, um1, u0, up1, idx >= 2, idx + 1 < n, A, f);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) res[v] += f[v];
+    }
+    if (have && idx < n - 1) face_flux_r<T, RECON, LIM, FAST, DIR>(um1, u0, up1, up2, idx >= 1, idx + 2 < n, A, fp);
   }
-  if (have && idx < n - 1) face_flux_r<T, RECON, LIM, FAST, DIR>(um1, u0, up1, up2, idx >= 1, idx + 2 < n, A, fp);
 }
 
 // ZS = false: one thread per cell, BS cells per work item.
