@@ -327,6 +327,162 @@ __global__ __launch_bounds__(BS) void k_stage_cell(const PatchDev* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// cell kernel with y and z faces shared inside the workgroup
+// ---------------------------------------------------------------------------
+
+// 8 wavefronts own 8 x 63 = 504 consecutive cells of one (j,k) plane.  As in
+// k_stage_cell<..., ZS = true> the z faces are evaluated once and handed to the
+// neighbouring lane.  Each cell also evaluates its y face j-1/2 and publishes
+// it in LDS; after one barrier a cell takes its plus y face j+1/2 from the cell
+// NK further on when that cell belongs to the workgroup, and evaluates it only
+// otherwise (the last NK cells of the item).  About 4.1 face evaluations per
+// cell.  The residual keeps the order Fx- + Fy- + Fz- - Fz+ - Fy+ - Fx+.
+template <typename T, int RECON, int LIM, bool FAST>
+__global__ __launch_bounds__(512) void k_stage_cell_yz(const PatchDev* __restrict__ pds,
+                                                       const TileCell* __restrict__ tiles, const T* __restrict__ src,
+                                                       const T* base, T* dst, uint64_t vs, T factor) {
+  constexpr int NC = 63 * 8;
+  __shared__ T sfy[5][NC];
+  const TileCell t = tiles[blockIdx.x];
+  const PatchDev& P = pds[t.patch];
+  const int ni = P.ni, nj = P.nj, nk = P.nk, njk = P.njk;
+  const int lane = (int)(threadIdx.x & 63);
+  const int q = 63 * (int)(threadIdx.x >> 6) + lane;  // cell of the item; lane 63 shadows the next wave's first
+  const int c2 = t.c2 + q;
+  const bool in_plane = c2 < njk;
+  const bool own = in_plane && lane < 63;
+  const int i = t.i;
+  const int cc2 = in_plane ? c2 : njk - 1;
+  const int j = cc2 / nk;
+  const int k = cc2 - j * nk;
+  constexpr uint32_t W = sizeof(T);
+  const uint32_t c = (P.off + (uint32_t)i * (uint32_t)njk + (uint32_t)cc2) * W;
+  const uint32_t sx = (uint32_t)njk * W, sy = (uint32_t)nk * W;
+  const char* sv[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) sv[v] = reinterpret_cast<const char*>(src + (uint64_t)v * vs);
+  const T dx = T(P.dx), dy = T(P.dy), dz = T(P.dz);
+  const T Ax = dy * dz, Ay = dx * dz, Az = dx * dy;
+  T res[5], f[5], u0[5], fxp[5];
+  zero5(res);
+  zero5(fxp);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) u0[v] = ldv<T>(sv[v], c);
+  if (own) axis_faces_r<T, RECON, LIM, FAST, 0>(sv, c, sx, i, ni, u0, Ax, res, fxp);
+  // y face j-1/2, published for the cell below
+  T fym[5];
+  zero5(fym);
+  if (own && nj > 2 && j >= 1) {
+    T um2[5], um1[5], up1[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      um1[v] = ldv<T>(sv[v], c - sy);
+      um2[v] = j >= 2 ? ldv<T>(sv[v], c - 2 * sy) : um1[v];
+      up1[v] = j + 1 < nj ? ldv<T>(sv[v], c + sy) : u0[v];
+    }
+    face_flux_r<T, RECON, LIM, FAST, 1>(um2, um1, u0, up1, j >= 2, j + 1 < nj, Ay, fym);
+  }
+  if (own) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      res[v] += fym[v];
+      sfy[v][q] = fym[v];
+    }
+  }
+  // z face k-1/2 of every lane, the plus z face from the lane above
+  T fzm[5], fzp[5];
+  zero5(fzm);
+  if (in_plane && nk > 2 && k >= 1) {
+    T um2[5], um1[5], up1[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      um1[v] = ldv<T>(sv[v], c - W);
+      um2[v] = k >= 2 ? ldv<T>(sv[v], c - 2 * W) : um1[v];
+      up1[v] = k + 1 < nk ? ldv<T>(sv[v], c + W) : u0[v];
+    }
+    face_flux_r<T, RECON, LIM, FAST, 2>(um2, um1, u0, up1, k >= 2, k + 1 < nk, Az, fzm);
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    res[v] += fzm[v];
+    const T up = __shfl_down(fzm[v], 1, 64);
+    fzp[v] = (nk > 2 && k + 1 < nk) ? up : T(0);
+  }
+  __syncthreads();
+  if (!own) return;
+  // plus y face j+1/2: from the cell NK further on, or evaluated here
+  T fyp[5];
+  zero5(fyp);
+  if (nj > 2 && j + 1 < nj) {
+    if (q + nk < NC) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) fyp[v] = sfy[v][q + nk];
+    } else {
+      T um1[5], up1[5], up2[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        up1[v] = ldv<T>(sv[v], c + sy);
+        um1[v] = j >= 1 ? ldv<T>(sv[v], c - sy) : u0[v];
+        up2[v] = j + 2 < nj ? ldv<T>(sv[v], c + 2 * sy) : up1[v];
+      }
+      face_flux_r<T, RECON, LIM, FAST, 1>(um1, u0, up1, up2, j >= 1, j + 2 < nj, Ay, fyp);
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    res[v] -= fzp[v];
+    res[v] -= fyp[v];
+    res[v] -= fxp[v];
+  }
+  // patch-face functors (cartesianiterator.h:190-308)
+  const T xc = T(0.5) * dx + T(i) * dx, yc = T(0.5) * dy + T(j) * dy, zc = T(0.5) * dz + T(k) * dz;
+  if (ni > 2 && i == 0) {
+    zero5(f);
+    wall_flux<T>(P.bc[0], P.par[0], u0, 0, false, yc, zc, j, k, Ax, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] += f[v];
+  }
+  if (ni > 2 && i == ni - 1) {
+    zero5(f);
+    wall_flux<T>(P.bc[1], P.par[1], u0, 0, true, yc, zc, j, k, Ax, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] -= f[v];
+  }
+  if (nj > 2 && j == 0) {
+    zero5(f);
+    wall_flux<T>(P.bc[2], P.par[2], u0, 1, false, xc, zc, i, k, Ay, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] += f[v];
+  }
+  if (nj > 2 && j == nj - 1) {
+    zero5(f);
+    wall_flux<T>(P.bc[3], P.par[3], u0, 1, true, xc, zc, i, k, Ay, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] -= f[v];
+  }
+  if (nk > 2 && k == 0) {
+    zero5(f);
+    wall_flux<T>(P.bc[4], P.par[4], u0, 2, false, xc, yc, i, j, Az, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] += f[v];
+  }
+  if (nk > 2 && k == nk - 1) {
+    zero5(f);
+    wall_flux<T>(P.bc[5], P.par[5], u0, 2, true, xc, yc, i, j, Az, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] -= f[v];
+  }
+  // cartesianiterator.h:310-322
+  const T pf = factor / (dx * dy * dz);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const char* bv = reinterpret_cast<const char*>(base + (uint64_t)v * vs);
+    char* dv = reinterpret_cast<char*>(dst + (uint64_t)v * vs);
+    *reinterpret_cast<T*>(dv + c) = ldv<T>(bv, c) + pf * res[v];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // marching stage kernel
 // ---------------------------------------------------------------------------
 

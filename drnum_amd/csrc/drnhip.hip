@@ -64,6 +64,12 @@ struct drn_ctx {
   TileCell* d_tz = nullptr;
   TileCell* d_tz_xcd = nullptr;
   TileCell* d_tz_xcd_pp = nullptr;
+  // 504-cell items of the y/z-shared cell kernel (8 wavefronts x 63 cells)
+  std::vector<TileCell> tyz;
+  std::vector<int> tyz_first;
+  TileCell* d_tyz = nullptr;
+  TileCell* d_tyz_xcd = nullptr;
+  TileCell* d_tyz_xcd_pp = nullptr;
   double* d_part = nullptr;  // reduction partials (5 per tile)
   double* d_out = nullptr;
   int tiling = 11;     // batched form
@@ -174,7 +180,7 @@ static void timer_end(drn_ctx* ctx, hipEvent_t stop) {
 constexpr int MARCH_BS = 256;
 
 // tiling choices: kernel, i-range length (0 = whole patch) and work-list order
-enum { K_MARCH = 0, K_CELL = 1, K_COLUMN = 2, K_CELL_SLOPE = 3, K_WTILE = 4, K_CELLZ = 5 };
+enum { K_MARCH = 0, K_CELL = 1, K_COLUMN = 2, K_CELL_SLOPE = 3, K_WTILE = 4, K_CELLZ = 5, K_CELLYZ = 6 };
 struct TilingChoice {
   int irange;  // planes per work item, 0 = whole patch
   int xcd;     // 1 = segment the work list so that neighbouring chunks share an XCD
@@ -186,7 +192,8 @@ static const TilingChoice kTilings[] = {
     {2, 0, K_MARCH},  {0, 0, K_CELL},   {0, 0, K_COLUMN},  {0, 1, K_COLUMN}, {32, 1, K_COLUMN},
     {16, 0, K_COLUMN}, {16, 1, K_COLUMN}, {8, 1, K_COLUMN}, {4, 1, K_COLUMN}, {0, 0, K_CELL_SLOPE},
     {0, 1, K_CELL},    {0, 1, K_CELL_SLOPE}, {0, 1, K_WTILE}, {16, 1, K_WTILE}, {8, 1, K_WTILE},
-    {4, 1, K_WTILE},   {8, 0, K_WTILE},      {0, 1, K_CELLZ},  {0, 0, K_CELLZ}};
+    {4, 1, K_WTILE},   {8, 0, K_WTILE},      {0, 1, K_CELLZ},  {0, 0, K_CELLZ},  {0, 1, K_CELLYZ},
+    {0, 0, K_CELLYZ}};
 static const int kNumTilings = sizeof(kTilings) / sizeof(kTilings[0]);
 
 // the column kernel covers the fast MUSCL forms with a symmetric limited slope
@@ -249,7 +256,11 @@ static void launch_range(drn_ctx* ctx, int kind, const void* tiles, int ntiles, 
                                        kind == K_CELL_SLOPE);
   else if (kind == K_WTILE)
     launch_wtile_t<T, RECON, LIM, FAST>(ctx, (const TileWave*)tiles, ntiles, src, base, dst, factor);
-  else if (kind == K_CELLZ) {
+  else if (kind == K_CELLYZ) {
+    if (ntiles > 0)
+      hipLaunchKernelGGL((k_stage_cell_yz<T, RECON, LIM, FAST>), dim3(ntiles), dim3(512), 0, ctx->stream, ctx->d_pd,
+                         (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
+  } else if (kind == K_CELLZ) {
     if (ntiles > 0)
       hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, FAST, 256, true>), dim3(ntiles), dim3(256), 0, ctx->stream,
                          ctx->d_pd, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
@@ -303,7 +314,9 @@ static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int l
   };
   if (launch == DRN_LAUNCH_BATCHED) {
     const int kind = effective_kind(ctx, ctx->tiling);
-    if (kind == K_CELLZ) {
+    if (kind == K_CELLYZ) {
+      go(K_CELLYZ, kTilings[ctx->tiling].xcd ? ctx->d_tyz_xcd : ctx->d_tyz, (int)ctx->tyz.size());
+    } else if (kind == K_CELLZ) {
       go(K_CELLZ, kTilings[ctx->tiling].xcd ? ctx->d_tz_xcd : ctx->d_tz, (int)ctx->tz.size());
     } else if (kind == K_CELL || kind == K_CELL_SLOPE) {
       go(kind, kTilings[ctx->tiling].xcd ? ctx->d_tiles_xcd : ctx->d_tiles, (int)ctx->tiles.size());
@@ -320,7 +333,11 @@ static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int l
     for (int p = 0; p < np; ++p) {
       const int m0 = ctx->mt_first[p], mn = ctx->mt_first[p + 1] - m0;
       const int w0 = ctx->wt_first[p], wn = ctx->wt_first[p + 1] - w0;
-      if (kind == K_CELLZ) {
+      if (kind == K_CELLYZ) {
+        const int z0 = ctx->tyz_first[p];
+        const TileCell* tl = (kTilings[ctx->tiling_pp].xcd ? ctx->d_tyz_xcd_pp : ctx->d_tyz) + z0;
+        go(K_CELLYZ, tl, ctx->tyz_first[p + 1] - z0);
+      } else if (kind == K_CELLZ) {
         const int z0 = ctx->tz_first[p];
         const TileCell* tl = (kTilings[ctx->tiling_pp].xcd ? ctx->d_tz_xcd_pp : ctx->d_tz) + z0;
         go(K_CELLZ, tl, ctx->tz_first[p + 1] - z0);
@@ -629,6 +646,9 @@ void drn_ctx_destroy(drn_ctx* ctx) {
   dfree(ctx->d_tz);
   dfree(ctx->d_tz_xcd);
   dfree(ctx->d_tz_xcd_pp);
+  dfree(ctx->d_tyz);
+  dfree(ctx->d_tyz_xcd);
+  dfree(ctx->d_tyz_xcd_pp);
   dfree(ctx->d_mt_pp);
   dfree(ctx->d_mt_launch);
   dfree(ctx->d_thin_tiles);
@@ -743,6 +763,16 @@ int drn_layout(drn_ctx* ctx) {
       for (int c2 = 0; c2 < njk; c2 += 252) ctx->tz.push_back(TileCell{p, i, c2, 0});
   }
   ctx->tz_first[np] = (int)ctx->tz.size();
+  ctx->tyz.clear();
+  ctx->tyz_first.assign(np + 1, 0);
+  for (int p = 0; p < np; ++p) {
+    const auto& d = ctx->descs[p];
+    ctx->tyz_first[p] = (int)ctx->tyz.size();
+    const int njk = d.nj * d.nk;
+    for (int i = 0; i < d.ni; ++i)
+      for (int c2 = 0; c2 < njk; c2 += 504) ctx->tyz.push_back(TileCell{p, i, c2, 0});
+  }
+  ctx->tyz_first[np] = (int)ctx->tyz.size();
   // the cell kernel addresses a variable array with 32-bit byte offsets
   if (off * ctx->esize() >= (1ull << 32))
     return fail(ctx, DRN_E_ARG, "grid exceeds 2^32 bytes per variable array (2^30 FP32 / 2^29 FP64 cells)");
@@ -788,6 +818,18 @@ int drn_layout(drn_ctx* ctx) {
     HIPCHK(ctx, hipMemcpy(ctx->d_tz, ctx->tz.data(), nz * sizeof(TileCell), hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(ctx->d_tz_xcd, zall.data(), nz * sizeof(TileCell), hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(ctx->d_tz_xcd_pp, zpp.data(), nz * sizeof(TileCell), hipMemcpyHostToDevice));
+    const int ny = (int)ctx->tyz.size();
+    std::vector<TileCell> yall(ny), ypp(ny);
+    segment(ctx->tyz.data(), ny, yall.data());
+    for (int p = 0; p < np; ++p)
+      segment(ctx->tyz.data() + ctx->tyz_first[p], ctx->tyz_first[p + 1] - ctx->tyz_first[p],
+              ypp.data() + ctx->tyz_first[p]);
+    HIPCHK(ctx, dalloc(&ctx->d_tyz, ny));
+    HIPCHK(ctx, dalloc(&ctx->d_tyz_xcd, ny));
+    HIPCHK(ctx, dalloc(&ctx->d_tyz_xcd_pp, ny));
+    HIPCHK(ctx, hipMemcpy(ctx->d_tyz, ctx->tyz.data(), ny * sizeof(TileCell), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->d_tyz_xcd, yall.data(), ny * sizeof(TileCell), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->d_tyz_xcd_pp, ypp.data(), ny * sizeof(TileCell), hipMemcpyHostToDevice));
   }
   HIPCHK(ctx, dalloc(&ctx->d_part, ctx->tiles.size() * 5));
   HIPCHK(ctx, dalloc(&ctx->d_out, 8));

@@ -191,7 +191,9 @@ int drn_diagnostics(drn_ctx* ctx, int field, double dt, double* out);
  *       lane shuffles, with no block-level barrier;
  * 27, 28 the cell-per-thread kernel with 63 cells per wavefront whose z faces
  *       are evaluated once and handed to the neighbouring lane (XCD segmented /
- *       natural order).
+ *       natural order);
+ * 29, 30 as 27 / 28 with 8 wavefronts per workgroup sharing their y faces
+ *       through LDS (one barrier).
  * Default for both forms: 11; drn_autotune picks per device. */
 int drn_num_tilings(void);
 int drn_set_tiling(drn_ctx* ctx, int launch, int choice);
