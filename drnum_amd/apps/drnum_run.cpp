@@ -7,6 +7,7 @@
 // coefficients, run the time steps, copy the result back to the host.
 //
 // usage: drnum_run <grid file> <case file> <out file> <steps> <dt> <mode> [f64] [levelset]
+//                  [restart=<file.dnd>] [dnd=<base name>]
 //   case file (binary, little endian), per patch in grid order:
 //     int32 bc[6], float64 par[6][16], then 5*N initial values in `real`
 //     (variable-major, the layout of field 0 in Patch::m_Data);
@@ -16,6 +17,10 @@
 //         hip    HIP_RungeKutta (one library call per step)
 //         graph  HIP_RungeKutta::graph (captured steps)
 //   out file: field 0 of every patch, 5*N values in `real` each
+//   restart=F: field 0 is read from the DrNUM restart file F after the case
+//              file (PatchGrid::readData), the time from its header
+//   dnd=B:     field 0 is also written to B_<steps, 6 digits>.dnd at the end
+//              (PatchGrid::writeData), with time = restart time + steps*dt
 // Prints one line per phase and exits non-zero on any error.
 #include <cstdio>
 #include <cstdlib>
@@ -32,7 +37,7 @@ typedef AusmPlus<5, Upwind2<5, VanAlbada>, PerfectGas> euler_t;
 
 template <typename real>
 static int run(const std::string& grid_file, const std::string& case_file, const std::string& out_file, int steps,
-               double dt, const std::string& mode, bool levelset) {
+               double dt, const std::string& mode, bool levelset, const std::string& restart, const std::string& dnd) {
   PatchGrid<double> grid;
   std::string err;
   if (grid.readGrid(grid_file, 1.0, &err)) {
@@ -82,6 +87,13 @@ static int run(const std::string& grid_file, const std::string& case_file, const
     return 2;
   }
   printf("grid: %zu patches read, dependencies computed\n", grid.numPatches());
+  std::vector<HostPatch<real>*> pp;
+  for (auto& P : patches) pp.push_back(&P);
+  double t0 = 0;
+  if (!restart.empty()) {
+    t0 = (double)readData<real>(pp, 0, restart);
+    printf("restart: t = %g from %s\n", t0, restart.c_str());
+  }
 
   HIP_CartesianIterator<5, euler_t, real> iterator(euler_t(), 0);
   for (auto& P : patches) iterator.addPatch(&P);
@@ -116,6 +128,10 @@ static int run(const std::string& grid_file, const std::string& case_file, const
     out.write(reinterpret_cast<const char*>(P.getField(0)), (std::streamsize)(5 * P.variableSize() * sizeof(real)));
   delete ls;
   printf("wrote %s\n", out_file.c_str());
+  if (!dnd.empty()) {
+    writeData<real>(pp, 0, dnd, real(t0 + steps * dt), steps);
+    printf("wrote restart file %s_%06d.dnd\n", dnd.c_str(), steps);
+  }
   return out ? 0 : 3;
 }
 
@@ -125,13 +141,17 @@ int main(int argc, char** argv) {
     return 2;
   }
   bool f64 = false, levelset = false;
+  std::string restart, dnd;
   for (int a = 7; a < argc; ++a) {
     if (!strcmp(argv[a], "f64")) f64 = true;
     if (!strcmp(argv[a], "levelset")) levelset = true;
+    if (!strncmp(argv[a], "restart=", 8)) restart = argv[a] + 8;
+    if (!strncmp(argv[a], "dnd=", 4)) dnd = argv[a] + 4;
   }
   try {
-    if (f64) return run<double>(argv[1], argv[2], argv[3], atoi(argv[4]), atof(argv[5]), argv[6], levelset);
-    return run<float>(argv[1], argv[2], argv[3], atoi(argv[4]), atof(argv[5]), argv[6], levelset);
+    if (f64)
+      return run<double>(argv[1], argv[2], argv[3], atoi(argv[4]), atof(argv[5]), argv[6], levelset, restart, dnd);
+    return run<float>(argv[1], argv[2], argv[3], atoi(argv[4]), atof(argv[5]), argv[6], levelset, restart, dnd);
   } catch (const DeviceError& e) {
     fprintf(stderr, "device error: %s\n", e.what());
     return 4;

@@ -38,6 +38,9 @@ _SIGS = {
     "drnh_grid_ls_num_cells": (C.c_int64, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "drnh_grid_ls_cells": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _I64P, _DP, _I64P, _DP]),
     "drnh_grid_min_ch_length": (C.c_double, [C.c_void_p]),
+    "drnh_dnd_write": (C.c_int, [C.c_char_p, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_void_p), _I64P]),
+    "drnh_dnd_read": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_void_p), _I64P,
+                                C.POINTER(C.c_int)]),
     "drnh_grid_to_device": (C.c_int, [C.c_void_p, C.c_void_p]),
     "drnh_grid_upload_exchange": (C.c_int, [C.c_void_p, C.c_void_p]),
     "drnh_grid_upload_levelset": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -179,3 +182,32 @@ class HostGrid:
 
     def upload_levelset(self, ctx: hip.Context):
         self._chk(self.L.drnh_grid_upload_levelset(self.h, ctx.h), "drnh_grid_upload_levelset")
+
+
+# ---- restart files ------------------------------------------------------------
+
+def dnd_write(path, fields, time, precision=None):
+    """Write one field of every patch (list of (5, N) arrays) as a DrNUM .dnd file."""
+    prec = precision or (4 if np.asarray(fields[0]).dtype == np.float32 else 8)
+    dt = np.float32 if prec == 4 else np.float64
+    arrs = [np.ascontiguousarray(f, dtype=dt).reshape(-1) for f in fields]
+    ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    counts = np.array([a.size for a in arrs], dtype=np.int64)
+    rc = lib().drnh_dnd_write(path.encode(), prec, float(time), len(arrs), ptrs, _ptr(counts, C.c_int64))
+    if rc != 0:
+        raise RuntimeError("drnh_dnd_write failed (%d)" % rc)
+
+
+def dnd_read(path, shapes, dtype=np.float64):
+    """Read a .dnd file into arrays of the given (5, N) shapes; returns (fields, time, file_precision)."""
+    dt = np.dtype(dtype)
+    prec = 4 if dt == np.float32 else 8
+    arrs = [np.zeros(s, dtype=dt) for s in shapes]
+    ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    counts = np.array([a.size for a in arrs], dtype=np.int64)
+    t = C.c_double()
+    fp = C.c_int()
+    rc = lib().drnh_dnd_read(path.encode(), prec, C.byref(t), len(arrs), ptrs, _ptr(counts, C.c_int64), C.byref(fp))
+    if rc != 0:
+        raise RuntimeError("drnh_dnd_read failed (%d)" % rc)
+    return arrs, t.value, fp.value

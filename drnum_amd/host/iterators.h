@@ -32,12 +32,14 @@
 #pragma once
 
 #include <cstddef>
+#include <cstdio>
 #include <list>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../include/drnhip.h"
+#include "../../include/drnum_host.h"
 #include "grid.h"
 
 namespace drnum {
@@ -130,6 +132,40 @@ class HostPatch {
   size_t index_, nf_, nv_;
   std::vector<real> data_;
 };
+
+// PatchGrid::writeData / readData (patchgrid.cpp:563-610): field i_field of
+// every patch to / from "<base>_<count, 6 digits>.dnd" (write) or the named
+// file (read, ".dnd" appended when missing); readData returns the time.
+template <typename real>
+void writeData(std::vector<HostPatch<real>*>& patches, size_t i_field, const std::string& base_file_name, real time,
+               int count) {
+  char num[16];
+  snprintf(num, sizeof(num), "%06d", count);
+  const std::string name = base_file_name + "_" + num + ".dnd";
+  std::vector<const void*> ptr;
+  std::vector<int64_t> cnt;
+  for (auto* P : patches) {
+    ptr.push_back(P->getField(i_field));
+    cnt.push_back((int64_t)P->fieldSize());
+  }
+  if (drnh_dnd_write(name.c_str(), (int)sizeof(real), (double)time, (int)patches.size(), ptr.data(), cnt.data()))
+    throw DeviceError("writeData: cannot write " + name);
+}
+
+template <typename real>
+real readData(std::vector<HostPatch<real>*>& patches, size_t i_field, const std::string& file_name) {
+  std::vector<void*> ptr;
+  std::vector<int64_t> cnt;
+  for (auto* P : patches) {
+    ptr.push_back(P->getField(i_field));
+    cnt.push_back((int64_t)P->fieldSize());
+  }
+  double t = 0;
+  int prec = 0;
+  if (drnh_dnd_read(file_name.c_str(), (int)sizeof(real), &t, (int)patches.size(), ptr.data(), cnt.data(), &prec))
+    throw DeviceError("readData: cannot read " + file_name + " for this grid");
+  return (real)t;
+}
 
 // ---------------------------------------------------------------------------
 // iterators

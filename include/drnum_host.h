@@ -71,6 +71,18 @@ int64_t drnh_grid_ls_num_cells(const drnh_grid* g, int p, int layer, int outer);
 int drnh_grid_ls_cells(const drnh_grid* g, int p, int layer, int outer, int64_t* cells, double* gvals,
                        int64_t* donors, double* weights);
 
+/* Restart files (.dnd): PatchGrid::writeData / readData (patchgrid.cpp:563-610,
+ * patch.cpp:776-790).  Big-endian int32 precision (4 | 8), the time, then for
+ * each patch in grid order counts[p] values of one field ([var][cell]).
+ * `precision` is that of the caller's buffers (fields[p] are float* for 4,
+ * double* for 8); a file of the other precision is converted on reading.
+ * drnh_dnd_read appends ".dnd" when the name lacks it, and fails (-3) on a
+ * short or over-long file.  0 on success, -1 bad argument, -2 I/O error. */
+int drnh_dnd_write(const char* path, int precision, double time, int n_patches, const void* const* fields,
+                   const int64_t* counts);
+int drnh_dnd_read(const char* path, int precision, double* time, int n_patches, void* const* fields,
+                  const int64_t* counts, int* file_precision);
+
 /* PatchGrid::computeMinChLength (patchgrid.cpp:659-671) */
 double drnh_grid_min_ch_length(const drnh_grid* g);
 

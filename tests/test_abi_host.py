@@ -157,3 +157,48 @@ def test_dropin_app_reports_missing_device(tmp_path):
     assert r.returncode == 4, r.stdout + r.stderr
     assert "drn_ctx_create" in r.stderr
     assert "grid: 8 patches read" in r.stdout
+
+
+# ---------------------------------------------------------------------------
+# restart files (.dnd, patchgrid.cpp:563-610)
+# ---------------------------------------------------------------------------
+
+def _fields(dtype, seed=4):
+    rng = np.random.default_rng(seed)
+    return [rng.standard_normal((5, n)).astype(dtype) for n in (7, 12, 3)]
+
+
+@pytest.mark.parametrize("dtype,code,fmt", [(np.float32, 4, ">f4"), (np.float64, 8, ">f8")])
+def test_dnd_bytes_are_the_reference_stream_layout(tmp_path, dtype, code, fmt):
+    """QDataStream is big-endian: int32 precision, the time as `real`, then each
+    patch's field [var][cell] (patchgrid.cpp:563-583, patch.cpp:776-782)."""
+    fields = _fields(dtype)
+    path = str(tmp_path / "run_000010.dnd")
+    host.dnd_write(path, fields, 0.125)
+    want = np.array([code], dtype=">i4").tobytes() + np.array([0.125], dtype=fmt).tobytes()
+    want += b"".join(f.astype(fmt).tobytes() for f in fields)
+    assert open(path, "rb").read() == want
+
+
+@pytest.mark.parametrize("wdt,rdt", [(np.float32, np.float32), (np.float64, np.float64), (np.float32, np.float64),
+                                     (np.float64, np.float32)])
+def test_dnd_round_trip_and_conversion(tmp_path, wdt, rdt):
+    fields = _fields(wdt)
+    base = str(tmp_path / "state")
+    host.dnd_write(base + ".dnd", fields, 2.5)
+    got, t, prec = host.dnd_read(base, [f.shape for f in fields], rdt)   # ".dnd" appended when missing
+    assert t == 2.5 and prec == (4 if wdt == np.float32 else 8)
+    for a, b in zip(got, fields):
+        np.testing.assert_array_equal(a, b.astype(rdt))
+
+
+def test_dnd_rejects_a_file_of_another_grid(tmp_path):
+    fields = _fields(np.float32)
+    path = str(tmp_path / "x.dnd")
+    host.dnd_write(path, fields, 1.0)
+    with pytest.raises(RuntimeError):
+        host.dnd_read(path, [f.shape for f in fields[:2]], np.float32)         # file longer than the grid
+    with pytest.raises(RuntimeError):
+        host.dnd_read(path, [f.shape for f in fields] + [(5, 4)], np.float32)  # file shorter than the grid
+    with pytest.raises(RuntimeError):
+        host.dnd_read(str(tmp_path / "missing.dnd"), [(5, 1)], np.float32)

@@ -89,3 +89,33 @@ def test_dropin_spheres_levelset(gpu, tmp_path, dtype):
         assert err <= TOL[dtype]
     for a, b in zip(res["rk"], res["hip"]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_dropin_restart_file_continues_bitwise(gpu, tmp_path, dtype):
+    """3 steps written as a DrNUM restart file (PatchGrid::writeData), then 2 steps
+    started from it (PatchGrid::readData) give the 5-step state bit for bit."""
+    from drnum_amd import host
+
+    w = configs.jet(core=8, blocks=2)
+    f64 = ["f64"] if np.dtype(dtype) == np.float64 else []
+    grid = tmp_path / "grid"
+    case = tmp_path / "case.bin"
+    grid.write_text(grid_file_text(w))
+    case.write_bytes(case_file_bytes(w, dtype))
+
+    def app(steps, out, *extra):
+        r = subprocess.run([APP, str(grid), str(case), str(out), str(steps), repr(float(w.dt)), "hip"] + f64 +
+                           list(extra), capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        return np.frombuffer(out.read_bytes(), dtype=dtype)
+
+    first = app(3, tmp_path / "a.bin", "dnd=%s" % (tmp_path / "rs"))
+    shapes = [(5, P.cells) for P in w.patches]
+    fields, t, prec = host.dnd_read(str(tmp_path / "rs_000003.dnd"), shapes, dtype)
+    assert prec == np.dtype(dtype).itemsize
+    assert t == pytest.approx(3 * w.dt, rel=1e-6)
+    np.testing.assert_array_equal(np.concatenate([f.ravel() for f in fields]), first)
+    cont = app(2, tmp_path / "b.bin", "restart=%s" % (tmp_path / "rs_000003.dnd"))
+    full = app(5, tmp_path / "c.bin")
+    np.testing.assert_array_equal(cont, full)
