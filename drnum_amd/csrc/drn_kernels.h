@@ -1628,6 +1628,40 @@ __global__ __launch_bounds__(256) void k_ls_operate(uint32_t n, const uint32_t* 
 // reductions
 // ---------------------------------------------------------------------------
 
+// ---------------------------------------------------------------------------
+// post-processing variables
+// ---------------------------------------------------------------------------
+
+// CompressibleVariables<PerfectGas>::getScalar / getVector
+// (compressiblevariables.h:81-105) for the cells [0, n) of one patch starting
+// at global cell `first`: q = 0 Ma, 1 p, 2 T, 3 rho, 4 S (entropy against
+// ref_p, ref_T), 5..7 u, v, w.  The selected quantities (bit q of `mask`) are
+// written in increasing q to out[slot * n + cell].
+template <typename T>
+__global__ __launch_bounds__(256) void k_post(uint32_t n, uint64_t first, const T* __restrict__ U, uint64_t vs,
+                                              int mask, T ref_p, T ref_T, T* __restrict__ out) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  T var[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) var[v] = U[(uint64_t)v * vs + first + c];
+  T p, Tt, u, v, w;
+  cons_to_prim<T>(var, p, Tt, u, v, w);
+  T q[8];
+  q[0] = dsqrt<T>((u * u + v * v + w * w) / (Gas<T>::GAMMA * Gas<T>::R * Tt));
+  q[1] = p;
+  q[2] = Tt;
+  q[3] = var[0];
+  q[4] = Gas<T>::CP * log(Tt / ref_T) - Gas<T>::R * log(p / ref_p);
+  q[5] = u;
+  q[6] = v;
+  q[7] = w;
+  int slot = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if (mask & (1 << k)) out[(uint64_t)(slot++) * n + c] = q[k];
+}
+
 __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);

@@ -72,6 +72,8 @@ struct drn_ctx {
   TileCell* d_tyz_xcd_pp = nullptr;
   double* d_part = nullptr;  // reduction partials (5 per tile)
   double* d_out = nullptr;
+  void* d_post = nullptr;    // post-processing output of one patch
+  size_t post_bytes = 0;
   int tiling = 11;     // batched form
   int tiling_pp = 11;  // per-patch form
   int arith = -1;  // DRN_ARITH_*; -1 = precision default
@@ -656,6 +658,7 @@ void drn_ctx_destroy(drn_ctx* ctx) {
   dfree(ctx->d_wt_launch);
   dfree(ctx->d_part);
   dfree(ctx->d_out);
+  dfree(ctx->d_post);
   for (auto& s : ctx->timers) {
     for (auto& e : s.used) {
       (void)hipEventDestroy(e.first);
@@ -1238,6 +1241,41 @@ int drn_diagnostics(drn_ctx* ctx, int field, double dt, double* out) {
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const void* U = ctx->buf[ctx->map[field]];
   return ctx->prec == DRN_F32 ? diag_t(ctx, (const float*)U, dt, out) : diag_t(ctx, (const double*)U, dt, out);
+}
+
+template <typename T>
+static int post_t(drn_ctx* ctx, const T* U, int patch, int mask, double ref_p, double ref_T, void* host) {
+  const auto& d = ctx->descs[patch];
+  const uint32_t n = (uint32_t)((size_t)d.ni * d.nj * d.nk);
+  int nq = 0;
+  for (int q = 0; q < 8; ++q) nq += (mask >> q) & 1;
+  const size_t bytes = (size_t)nq * n * sizeof(T);
+  if (bytes > ctx->post_bytes) {
+    dfree(ctx->d_post);
+    ctx->d_post = nullptr;
+    ctx->post_bytes = 0;
+    HIPCHK(ctx, hipMalloc(&ctx->d_post, bytes));
+    ctx->post_bytes = bytes;
+  }
+  hipLaunchKernelGGL(k_post<T>, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, (uint64_t)ctx->offs[patch], U,
+                     ctx->vstride, mask, (T)ref_p, (T)ref_T, (T*)ctx->d_post);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipMemcpyAsync(host, ctx->d_post, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return DRN_OK;
+}
+
+int drn_post_variables(drn_ctx* ctx, int field, int patch, int mask, double ref_p, double ref_T, void* host_out) {
+  if (!ctx || !host_out) return DRN_E_ARG;
+  int rc;
+  if ((rc = check_field(ctx, field))) return rc;
+  if (patch < 0 || patch >= (int)ctx->descs.size()) return fail(ctx, DRN_E_ARG, "patch id out of range");
+  if (mask <= 0 || mask > 255) return fail(ctx, DRN_E_ARG, "variable mask must select 1..8 of the 8 quantities");
+  if (!(ref_p > 0 && ref_T > 0)) return fail(ctx, DRN_E_ARG, "reference pressure and temperature must be positive");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const void* U = ctx->buf[ctx->map[field]];
+  return ctx->prec == DRN_F32 ? post_t(ctx, (const float*)U, patch, mask, ref_p, ref_T, host_out)
+                              : post_t(ctx, (const double*)U, patch, mask, ref_p, ref_T, host_out);
 }
 
 int drn_set_tiling(drn_ctx* ctx, int launch, int choice) {

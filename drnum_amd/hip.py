@@ -61,6 +61,7 @@ _SIGS = {
     "drn_set_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "drn_get_tiling": (C.c_int, [C.c_void_p, C.c_int]),
     "drn_num_tilings": (C.c_int, []),
+    "drn_post_variables": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p]),
     "drn_set_arith": (C.c_int, [C.c_void_p, C.c_int]),
     "drn_timing": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "drn_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
@@ -232,6 +233,21 @@ class Context:
 
     def get_tiling(self, launch=DRN_LAUNCH_BATCHED):
         return self.L.drn_get_tiling(self.h, launch)
+
+    POST_NAMES = ("Ma", "p", "T", "rho", "S", "u", "v", "w")
+
+    def post_variables(self, patch, names=POST_NAMES, field=0, ref_p=1e5, ref_T=300.0):
+        """Post-processing variables of one patch computed on the device:
+        dict name -> (N,) array in the context precision."""
+        mask = 0
+        for nm in names:
+            mask |= 1 << self.POST_NAMES.index(nm)
+        sel = [nm for nm in self.POST_NAMES if mask & (1 << self.POST_NAMES.index(nm))]
+        n = self.patch_cells(patch)
+        out = np.zeros((len(sel), n), dtype=self.dtype)
+        self._chk(self.L.drn_post_variables(self.h, field, patch, mask, float(ref_p), float(ref_T),
+                                            out.ctypes.data_as(C.c_void_p)), "drn_post_variables")
+        return dict(zip(sel, out))
 
     def num_tilings(self):
         return self.L.drn_num_tilings()

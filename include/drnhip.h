@@ -169,6 +169,15 @@ int drn_core_sums(drn_ctx* ctx, int field, double* out);
  * out[2] = rho max, out[3] = number of non-finite values.  Synchronous. */
 int drn_diagnostics(drn_ctx* ctx, int field, double dt, double* out);
 
+/* ---- post-processing ----------------------------------------------------- */
+/* CompressibleVariables<PerfectGas>::getScalar / getVector (compressiblevariables.h:81-105)
+ * of every cell of one patch, evaluated on the device.  Bit q of `mask` selects
+ * q = 0 Ma, 1 p, 2 T, 3 rho, 4 S = cp ln(T/ref_T) - R ln(p/ref_p), 5 u, 6 v, 7 w
+ * (the reference defaults are ref_p = 1e5, ref_T = 300).  host_out receives the
+ * selected quantities in increasing q, N values each, in the context
+ * precision.  Only the selected quantities cross PCIe.  Synchronous. */
+int drn_post_variables(drn_ctx* ctx, int field, int patch, int mask, double ref_p, double ref_T, void* host_out);
+
 /* ---- tuning, timing, sync ------------------------------------------------ */
 /* Stage-kernel work-list choices, per launch form (drn_num_tilings() of them):
  * 0-10  the marching kernel (planes staged in LDS; in fast arithmetic the

@@ -212,3 +212,51 @@ def test_fp32_tracks_fp64_on_the_64_patch_jet(gpu):
           "fp64 perturbed 1e-10 at step 1000: %.3e" % (e50, e1000, env), flush=True)
     assert e50 <= 1e-5
     assert e1000 <= 3.0 * env
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_post_variables(gpu, dtype):
+    """CompressibleVariables<PerfectGas> (compressiblevariables.h:81-105) on the device:
+    closed forms on a uniform state at the reference p and T (S = 0), then the
+    jet after a few steps against the formulas evaluated in FP64 on the host."""
+    from drnum_amd import gas
+
+    rtol = 1e-12 if dtype == np.float64 else 2e-6
+    w = configs.jet(core=8, blocks=2)
+    gg = solver.GpuGrid(w, dtype)
+    p0, T0, vel = 1e5, 300.0, (30.0, -40.0, 0.0)
+    U = gas.prim_to_cons(*(np.full(w.patches[0].cells, x) for x in (p0, T0) + vel), dtype)
+    gg.ctx.upload(0, 0, U)
+    pv = gg.ctx.post_variables(0, ref_p=1e5, ref_T=300.0)
+    a = np.sqrt(1.4 * 287.2 * T0)
+    np.testing.assert_allclose(pv["Ma"], 50.0 / a, rtol=10 * rtol)
+    np.testing.assert_allclose(pv["p"], p0, rtol=10 * rtol)
+    np.testing.assert_allclose(pv["T"], T0, rtol=10 * rtol)
+    np.testing.assert_allclose(pv["rho"], p0 / (287.2 * T0), rtol=10 * rtol)
+    assert np.abs(pv["S"]).max() <= (1e-8 if dtype == np.float64 else 0.05)
+    for nm, x in zip(("u", "v", "w"), vel):
+        np.testing.assert_allclose(pv[nm], x, rtol=10 * rtol, atol=1e-12)
+    # the jet after three steps, every patch, and a subset selection in the stated order
+    og = O.OracleGrid(w, dtype)
+    og.compute_dependencies()
+    gg.upload_initial()
+    gg.set_tables(tables_of(og))
+    for _ in range(3):
+        gg.rk_step()
+    for pidx in range(len(w.patches)):
+        S = gg.ctx.download(pidx, 0).astype(np.float64)
+        rho = S[0]
+        u, v, ww = S[1] / rho, S[2] / rho, S[3] / rho
+        T = (S[4] / rho - 0.5 * (u * u + v * v + ww * ww)) / 717.5
+        p = rho * 287.2 * T
+        want = {"Ma": np.sqrt((u * u + v * v + ww * ww) / (1.4 * 287.2 * T)), "p": p, "T": T, "rho": rho,
+                "S": 1004.5 * np.log(T / 300.0) - 287.2 * np.log(p / 1e5), "u": u, "v": v, "w": ww}
+        got = gg.ctx.post_variables(pidx)
+        for nm in want:
+            scale = max(np.abs(want[nm]).max(), 1e-30)
+            atol = (1e-12 if dtype == np.float64 else 2e-6) * scale + (0.05 if (nm == "S" and dtype == np.float32) else 0)
+            np.testing.assert_allclose(got[nm], want[nm], rtol=0, atol=atol, err_msg=nm)
+        sub = gg.ctx.post_variables(pidx, names=("w", "p"))
+        assert list(sub) == ["p", "w"]
+        np.testing.assert_array_equal(sub["p"], got["p"])
+    gg.close()
