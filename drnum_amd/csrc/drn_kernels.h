@@ -327,6 +327,217 @@ __global__ __launch_bounds__(BS) void k_stage_cell(const PatchDev* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// cell kernel for two planes per thread
+// ---------------------------------------------------------------------------
+
+// face functors at the patch faces of cell (i,j,k) (cartesianiterator.h:190-308)
+template <typename T>
+__device__ __forceinline__ void add_walls(const PatchDev& P, int i, int j, int k, const T* u0, T dx, T dy, T dz,
+                                          T* res) {
+  const int ni = P.ni, nj = P.nj, nk = P.nk;
+  const bool wx = ni > 2 && (i == 0 || i == ni - 1);
+  const bool wy = nj > 2 && (j == 0 || j == nj - 1);
+  const bool wz = nk > 2 && (k == 0 || k == nk - 1);
+  if (!(wx || wy || wz)) return;
+  const T Ax = dy * dz, Ay = dx * dz, Az = dx * dy;
+  const T xc = T(0.5) * dx + T(i) * dx, yc = T(0.5) * dy + T(j) * dy, zc = T(0.5) * dz + T(k) * dz;
+  T f[5];
+  if (ni > 2 && i == 0) {
+    zero5(f);
+    wall_flux<T>(P.bc[0], P.par[0], u0, 0, false, yc, zc, j, k, Ax, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] += f[v];
+  }
+  if (ni > 2 && i == ni - 1) {
+    zero5(f);
+    wall_flux<T>(P.bc[1], P.par[1], u0, 0, true, yc, zc, j, k, Ax, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] -= f[v];
+  }
+  if (nj > 2 && j == 0) {
+    zero5(f);
+    wall_flux<T>(P.bc[2], P.par[2], u0, 1, false, xc, zc, i, k, Ay, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] += f[v];
+  }
+  if (nj > 2 && j == nj - 1) {
+    zero5(f);
+    wall_flux<T>(P.bc[3], P.par[3], u0, 1, true, xc, zc, i, k, Ay, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] -= f[v];
+  }
+  if (nk > 2 && k == 0) {
+    zero5(f);
+    wall_flux<T>(P.bc[4], P.par[4], u0, 2, false, xc, yc, i, j, Az, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] += f[v];
+  }
+  if (nk > 2 && k == nk - 1) {
+    zero5(f);
+    wall_flux<T>(P.bc[5], P.par[5], u0, 2, true, xc, yc, i, j, Az, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] -= f[v];
+  }
+}
+
+// One thread per column of two cells (i0, i0+1): the x face between them is
+// evaluated once and the column i0-2 .. i0+3 is loaded once; 63 cells per
+// wavefront with z faces shared by lane shuffle as in k_stage_cell<..., ZS>.
+// About 4.5 face evaluations per cell.  Each cell's residual keeps the order
+// Fx- + Fy- + Fz- - Fz+ - Fy+ - Fx+.  Work items start at even planes.
+template <typename T, int RECON, int LIM, bool FAST>
+__global__ __launch_bounds__(256) void k_stage_cell_x2(const PatchDev* __restrict__ pds,
+                                                       const TileCell* __restrict__ tiles, const T* __restrict__ src,
+                                                       const T* base, T* dst, uint64_t vs, T factor) {
+  const TileCell t = tiles[blockIdx.x];
+  const PatchDev& P = pds[t.patch];
+  const int ni = P.ni, nj = P.nj, nk = P.nk, njk = P.njk;
+  const int lane = (int)(threadIdx.x & 63);
+  const int c2 = t.c2 + 63 * (int)(threadIdx.x >> 6) + lane;
+  const bool in_plane = c2 < njk;
+  const bool own = in_plane && lane < 63;
+  const int i0 = t.i;
+  const bool has_b = i0 + 1 < ni;
+  const int cc2 = in_plane ? c2 : njk - 1;
+  const int j = cc2 / nk;
+  const int k = cc2 - j * nk;
+  constexpr uint32_t W = sizeof(T);
+  const uint32_t ca = (P.off + (uint32_t)i0 * (uint32_t)njk + (uint32_t)cc2) * W;
+  const uint32_t sx = (uint32_t)njk * W, sy = (uint32_t)nk * W;
+  const uint32_t cb = ca + (has_b ? sx : 0u);
+  const char* sv[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) sv[v] = reinterpret_cast<const char*>(src + (uint64_t)v * vs);
+  const T dx = T(P.dx), dy = T(P.dy), dz = T(P.dz);
+  const T Ax = dy * dz, Ay = dx * dz, Az = dx * dy;
+  const T pf = factor / (dx * dy * dz);
+
+  // column planes i0-2 .. i0+3 (entries outside the patch repeat a valid one)
+  T col[6][5];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const int ip = i0 - 2 + q;
+    const int ic = ip < 0 ? 0 : (ip >= ni ? ni - 1 : ip);
+    const uint32_t off = (P.off + (uint32_t)ic * (uint32_t)njk + (uint32_t)cc2) * W;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) col[q][v] = ldv<T>(sv[v], off);
+  }
+  T ua[5], ub[5], resa[5], resb[5], fx1[5], fx2[5];
+  zero5(resa);
+  zero5(resb);
+  zero5(fx1);
+  zero5(fx2);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    ua[v] = col[2][v];
+    ub[v] = col[3][v];
+  }
+  if (own && ni > 2) {
+    const bool f0 = i0 > 0, f1 = i0 < ni - 1, f2 = has_b && i0 + 1 < ni - 1;
+    if constexpr (FAST && RECON == RECON_UPWIND2 && (LIM == LIM_VANALBADA || LIM == LIM_MINMOD)) {
+      T sl[4][5];  // limited slopes of planes i0-1 .. i0+2
+#pragma unroll
+      for (int q = 1; q <= 4; ++q) {
+        const int ip = i0 - 2 + q;
+        const bool ok = ip >= 1 && ip + 1 < ni;
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+          sl[q - 1][v] = ok ? limited<T, LIM, true>(col[q][v] - col[q - 1][v], col[q + 1][v] - col[q][v]) : T(0);
+      }
+      if (f0) {
+        T vl[5], vr[5], f[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          vl[v] = col[1][v] + T(0.5) * sl[0][v];
+          vr[v] = col[2][v] - T(0.5) * sl[1][v];
+        }
+        zero5(f);
+        ausm_flux<T, FAST, 0>(vl, vr, Ax, f);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) resa[v] += f[v];
+      }
+      if (f1) {
+        T vl[5], vr[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          vl[v] = col[2][v] + T(0.5) * sl[1][v];
+          vr[v] = col[3][v] - T(0.5) * sl[2][v];
+        }
+        ausm_flux<T, FAST, 0>(vl, vr, Ax, fx1);
+      }
+      if (f2) {
+        T vl[5], vr[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          vl[v] = col[3][v] + T(0.5) * sl[2][v];
+          vr[v] = col[4][v] - T(0.5) * sl[3][v];
+        }
+        ausm_flux<T, FAST, 0>(vl, vr, Ax, fx2);
+      }
+    } else {
+      if (f0) {
+        T f[5];
+        zero5(f);
+        face_flux_r<T, RECON, LIM, FAST, 0>(col[0], col[1], col[2], col[3], i0 >= 2, i0 + 1 < ni, Ax, f);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) resa[v] += f[v];
+      }
+      if (f1) face_flux_r<T, RECON, LIM, FAST, 0>(col[1], col[2], col[3], col[4], i0 >= 1, i0 + 2 < ni, Ax, fx1);
+      if (f2) face_flux_r<T, RECON, LIM, FAST, 0>(col[2], col[3], col[4], col[5], true, i0 + 3 < ni, Ax, fx2);
+    }
+    // the face between the two cells is the minus x face of cell b
+#pragma unroll
+    for (int v = 0; v < 5; ++v) resb[v] += fx1[v];
+  }
+  // the two cells one after the other: y faces, z faces (shared), walls, update
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const bool live = h == 0 || has_b;
+    const int i = i0 + h;
+    const uint32_t c = h == 0 ? ca : cb;
+    T* u0 = h == 0 ? ua : ub;
+    T* res = h == 0 ? resa : resb;
+    T fyp[5], fzm[5], fzp[5];
+    zero5(fyp);
+    if (own && live) axis_faces_r<T, RECON, LIM, FAST, 1>(sv, c, sy, j, nj, u0, Ay, res, fyp);
+    zero5(fzm);
+    if (in_plane && live && nk > 2 && k >= 1) {
+      T um2[5], um1[5], up1[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        um1[v] = ldv<T>(sv[v], c - W);
+        um2[v] = k >= 2 ? ldv<T>(sv[v], c - 2 * W) : um1[v];
+        up1[v] = k + 1 < nk ? ldv<T>(sv[v], c + W) : u0[v];
+      }
+      face_flux_r<T, RECON, LIM, FAST, 2>(um2, um1, u0, up1, k >= 2, k + 1 < nk, Az, fzm);
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      res[v] += fzm[v];
+      const T up = __shfl_down(fzm[v], 1, 64);
+      fzp[v] = (nk > 2 && k + 1 < nk) ? up : T(0);
+    }
+    if (own && live) {
+      const T* fxp = h == 0 ? fx1 : fx2;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        res[v] -= fzp[v];
+        res[v] -= fyp[v];
+        res[v] -= fxp[v];
+      }
+      add_walls<T>(P, i, j, k, u0, dx, dy, dz, res);
+      // cartesianiterator.h:310-322
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const char* bv = reinterpret_cast<const char*>(base + (uint64_t)v * vs);
+        char* dv = reinterpret_cast<char*>(dst + (uint64_t)v * vs);
+        *reinterpret_cast<T*>(dv + c) = ldv<T>(bv, c) + pf * res[v];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // cell kernel with y and z faces shared inside the workgroup
 // ---------------------------------------------------------------------------
 

@@ -64,6 +64,12 @@ struct drn_ctx {
   TileCell* d_tz = nullptr;
   TileCell* d_tz_xcd = nullptr;
   TileCell* d_tz_xcd_pp = nullptr;
+  // 2-plane x 252-cell items of the two-plane cell kernel
+  std::vector<TileCell> tx2;
+  std::vector<int> tx2_first;
+  TileCell* d_tx2 = nullptr;
+  TileCell* d_tx2_xcd = nullptr;
+  TileCell* d_tx2_xcd_pp = nullptr;
   // 504-cell items of the y/z-shared cell kernel (8 wavefronts x 63 cells)
   std::vector<TileCell> tyz;
   std::vector<int> tyz_first;
@@ -182,7 +188,7 @@ static void timer_end(drn_ctx* ctx, hipEvent_t stop) {
 constexpr int MARCH_BS = 256;
 
 // tiling choices: kernel, i-range length (0 = whole patch) and work-list order
-enum { K_MARCH = 0, K_CELL = 1, K_COLUMN = 2, K_CELL_SLOPE = 3, K_WTILE = 4, K_CELLZ = 5, K_CELLYZ = 6 };
+enum { K_MARCH = 0, K_CELL = 1, K_COLUMN = 2, K_CELL_SLOPE = 3, K_WTILE = 4, K_CELLZ = 5, K_CELLYZ = 6, K_CELLX2 = 7 };
 struct TilingChoice {
   int irange;  // planes per work item, 0 = whole patch
   int xcd;     // 1 = segment the work list so that neighbouring chunks share an XCD
@@ -195,7 +201,7 @@ static const TilingChoice kTilings[] = {
     {16, 0, K_COLUMN}, {16, 1, K_COLUMN}, {8, 1, K_COLUMN}, {4, 1, K_COLUMN}, {0, 0, K_CELL_SLOPE},
     {0, 1, K_CELL},    {0, 1, K_CELL_SLOPE}, {0, 1, K_WTILE}, {16, 1, K_WTILE}, {8, 1, K_WTILE},
     {4, 1, K_WTILE},   {8, 0, K_WTILE},      {0, 1, K_CELLZ},  {0, 0, K_CELLZ},  {0, 1, K_CELLYZ},
-    {0, 0, K_CELLYZ}};
+    {0, 0, K_CELLYZ},  {0, 1, K_CELLX2},     {0, 0, K_CELLX2}};
 static const int kNumTilings = sizeof(kTilings) / sizeof(kTilings[0]);
 
 // the column kernel covers the fast MUSCL forms with a symmetric limited slope
@@ -258,7 +264,11 @@ static void launch_range(drn_ctx* ctx, int kind, const void* tiles, int ntiles, 
                                        kind == K_CELL_SLOPE);
   else if (kind == K_WTILE)
     launch_wtile_t<T, RECON, LIM, FAST>(ctx, (const TileWave*)tiles, ntiles, src, base, dst, factor);
-  else if (kind == K_CELLYZ) {
+  else if (kind == K_CELLX2) {
+    if (ntiles > 0)
+      hipLaunchKernelGGL((k_stage_cell_x2<T, RECON, LIM, FAST>), dim3(ntiles), dim3(256), 0, ctx->stream, ctx->d_pd,
+                         (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
+  } else if (kind == K_CELLYZ) {
     if (ntiles > 0)
       hipLaunchKernelGGL((k_stage_cell_yz<T, RECON, LIM, FAST>), dim3(ntiles), dim3(512), 0, ctx->stream, ctx->d_pd,
                          (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
@@ -316,7 +326,9 @@ static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int l
   };
   if (launch == DRN_LAUNCH_BATCHED) {
     const int kind = effective_kind(ctx, ctx->tiling);
-    if (kind == K_CELLYZ) {
+    if (kind == K_CELLX2) {
+      go(K_CELLX2, kTilings[ctx->tiling].xcd ? ctx->d_tx2_xcd : ctx->d_tx2, (int)ctx->tx2.size());
+    } else if (kind == K_CELLYZ) {
       go(K_CELLYZ, kTilings[ctx->tiling].xcd ? ctx->d_tyz_xcd : ctx->d_tyz, (int)ctx->tyz.size());
     } else if (kind == K_CELLZ) {
       go(K_CELLZ, kTilings[ctx->tiling].xcd ? ctx->d_tz_xcd : ctx->d_tz, (int)ctx->tz.size());
@@ -335,7 +347,11 @@ static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int l
     for (int p = 0; p < np; ++p) {
       const int m0 = ctx->mt_first[p], mn = ctx->mt_first[p + 1] - m0;
       const int w0 = ctx->wt_first[p], wn = ctx->wt_first[p + 1] - w0;
-      if (kind == K_CELLYZ) {
+      if (kind == K_CELLX2) {
+        const int z0 = ctx->tx2_first[p];
+        const TileCell* tl = (kTilings[ctx->tiling_pp].xcd ? ctx->d_tx2_xcd_pp : ctx->d_tx2) + z0;
+        go(K_CELLX2, tl, ctx->tx2_first[p + 1] - z0);
+      } else if (kind == K_CELLYZ) {
         const int z0 = ctx->tyz_first[p];
         const TileCell* tl = (kTilings[ctx->tiling_pp].xcd ? ctx->d_tyz_xcd_pp : ctx->d_tyz) + z0;
         go(K_CELLYZ, tl, ctx->tyz_first[p + 1] - z0);
@@ -651,6 +667,9 @@ void drn_ctx_destroy(drn_ctx* ctx) {
   dfree(ctx->d_tyz);
   dfree(ctx->d_tyz_xcd);
   dfree(ctx->d_tyz_xcd_pp);
+  dfree(ctx->d_tx2);
+  dfree(ctx->d_tx2_xcd);
+  dfree(ctx->d_tx2_xcd_pp);
   dfree(ctx->d_mt_pp);
   dfree(ctx->d_mt_launch);
   dfree(ctx->d_thin_tiles);
@@ -776,6 +795,16 @@ int drn_layout(drn_ctx* ctx) {
       for (int c2 = 0; c2 < njk; c2 += 504) ctx->tyz.push_back(TileCell{p, i, c2, 0});
   }
   ctx->tyz_first[np] = (int)ctx->tyz.size();
+  ctx->tx2.clear();
+  ctx->tx2_first.assign(np + 1, 0);
+  for (int p = 0; p < np; ++p) {
+    const auto& d = ctx->descs[p];
+    ctx->tx2_first[p] = (int)ctx->tx2.size();
+    const int njk = d.nj * d.nk;
+    for (int i = 0; i < d.ni; i += 2)
+      for (int c2 = 0; c2 < njk; c2 += 252) ctx->tx2.push_back(TileCell{p, i, c2, 0});
+  }
+  ctx->tx2_first[np] = (int)ctx->tx2.size();
   // the cell kernel addresses a variable array with 32-bit byte offsets
   if (off * ctx->esize() >= (1ull << 32))
     return fail(ctx, DRN_E_ARG, "grid exceeds 2^32 bytes per variable array (2^30 FP32 / 2^29 FP64 cells)");
@@ -833,6 +862,18 @@ int drn_layout(drn_ctx* ctx) {
     HIPCHK(ctx, hipMemcpy(ctx->d_tyz, ctx->tyz.data(), ny * sizeof(TileCell), hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(ctx->d_tyz_xcd, yall.data(), ny * sizeof(TileCell), hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(ctx->d_tyz_xcd_pp, ypp.data(), ny * sizeof(TileCell), hipMemcpyHostToDevice));
+    const int n2 = (int)ctx->tx2.size();
+    std::vector<TileCell> xall(n2), xpp(n2);
+    segment(ctx->tx2.data(), n2, xall.data());
+    for (int p = 0; p < np; ++p)
+      segment(ctx->tx2.data() + ctx->tx2_first[p], ctx->tx2_first[p + 1] - ctx->tx2_first[p],
+              xpp.data() + ctx->tx2_first[p]);
+    HIPCHK(ctx, dalloc(&ctx->d_tx2, n2));
+    HIPCHK(ctx, dalloc(&ctx->d_tx2_xcd, n2));
+    HIPCHK(ctx, dalloc(&ctx->d_tx2_xcd_pp, n2));
+    HIPCHK(ctx, hipMemcpy(ctx->d_tx2, ctx->tx2.data(), n2 * sizeof(TileCell), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->d_tx2_xcd, xall.data(), n2 * sizeof(TileCell), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->d_tx2_xcd_pp, xpp.data(), n2 * sizeof(TileCell), hipMemcpyHostToDevice));
   }
   HIPCHK(ctx, dalloc(&ctx->d_part, ctx->tiles.size() * 5));
   HIPCHK(ctx, dalloc(&ctx->d_out, 8));

@@ -202,7 +202,9 @@ int drn_post_variables(drn_ctx* ctx, int field, int patch, int mask, double ref_
  *       are evaluated once and handed to the neighbouring lane (XCD segmented /
  *       natural order);
  * 29, 30 as 27 / 28 with 8 wavefronts per workgroup sharing their y faces
- *       through LDS (one barrier).
+ *       through LDS (one barrier);
+ * 31, 32 as 27 / 28 with two cells per thread along i, the x face between
+ *       them evaluated once.
  * Default for both forms: 11; drn_autotune picks per device. */
 int drn_num_tilings(void);
 int drn_set_tiling(drn_ctx* ctx, int launch, int choice);
