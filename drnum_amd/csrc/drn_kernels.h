@@ -196,11 +196,10 @@ __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, 
 //             next cell.  Every lane's z face k-1/2 is the plus z face of the
 //             lane below it, handed over by a lane shuffle, so z faces are
 //             evaluated once: 5 face evaluations per cell instead of 6.
-template <typename T, int RECON, int LIM, bool FAST, int BS, bool ZS = false>
-__global__ __launch_bounds__(BS) void k_stage_cell(const PatchDev* __restrict__ pds, const TileCell* __restrict__ tiles,
-                                                   const T* __restrict__ src, const T* base, T* dst, uint64_t vs,
-                                                   T factor) {
-  const TileCell t = tiles[blockIdx.x];
+template <typename T, int RECON, int LIM, bool FAST, int BS, bool ZS>
+__device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds, const TileCell t,
+                                                const T* __restrict__ src, const T* base, T* dst, uint64_t vs,
+                                                T factor) {
   const PatchDev& P = pds[t.patch];
   const int ni = P.ni, nj = P.nj, nk = P.nk, njk = P.njk;
   const int lane = (int)(threadIdx.x & 63);
@@ -324,6 +323,26 @@ __global__ __launch_bounds__(BS) void k_stage_cell(const PatchDev* __restrict__ 
     char* dv = reinterpret_cast<char*>(dst + (uint64_t)v * vs);
     *reinterpret_cast<T*>(dv + c) = ldv<T>(bv, c) + pf * res[v];
   }
+}
+
+template <typename T, int RECON, int LIM, bool FAST, int BS, bool ZS = false>
+__global__ __launch_bounds__(BS) void k_stage_cell(const PatchDev* __restrict__ pds, const TileCell* __restrict__ tiles,
+                                                   const T* __restrict__ src, const T* base, T* dst, uint64_t vs,
+                                                   T factor) {
+  stage_cell_item<T, RECON, LIM, FAST, BS, ZS>(pds, tiles[blockIdx.x], src, base, dst, vs, factor);
+}
+
+// Persistent form of k_stage_cell<..., ZS = true>: a grid sized to the resident
+// capacity of the device walks the work list with a stride of gridDim.x
+// (a multiple of 8, so each workgroup stays on one XCD and keeps taking items
+// of that XCD's segment).  Every workgroup ends after its last item.
+template <typename T, int RECON, int LIM, bool FAST>
+__global__ __launch_bounds__(256) void k_stage_cell_persist(const PatchDev* __restrict__ pds,
+                                                            const TileCell* __restrict__ tiles, int n_items,
+                                                            const T* __restrict__ src, const T* base, T* dst,
+                                                            uint64_t vs, T factor) {
+  for (int it = (int)blockIdx.x; it < n_items; it += (int)gridDim.x)
+    stage_cell_item<T, RECON, LIM, FAST, 256, true>(pds, tiles[it], src, base, dst, vs, factor);
 }
 
 // ---------------------------------------------------------------------------

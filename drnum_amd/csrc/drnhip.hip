@@ -40,6 +40,7 @@ struct TimerSlot {
 
 struct drn_ctx {
   int device = 0;
+  int n_cu = 0;  // compute units of the device (persistent launch size)
   int prec = DRN_F64;
   hipStream_t stream = nullptr;
   std::string err;
@@ -188,7 +189,17 @@ static void timer_end(drn_ctx* ctx, hipEvent_t stop) {
 constexpr int MARCH_BS = 256;
 
 // tiling choices: kernel, i-range length (0 = whole patch) and work-list order
-enum { K_MARCH = 0, K_CELL = 1, K_COLUMN = 2, K_CELL_SLOPE = 3, K_WTILE = 4, K_CELLZ = 5, K_CELLYZ = 6, K_CELLX2 = 7 };
+enum {
+  K_MARCH = 0,
+  K_CELL = 1,
+  K_COLUMN = 2,
+  K_CELL_SLOPE = 3,
+  K_WTILE = 4,
+  K_CELLZ = 5,
+  K_CELLYZ = 6,
+  K_CELLX2 = 7,
+  K_CELLZP = 8  // K_CELLZ items, one persistent launch
+};
 struct TilingChoice {
   int irange;  // planes per work item, 0 = whole patch
   int xcd;     // 1 = segment the work list so that neighbouring chunks share an XCD
@@ -201,7 +212,7 @@ static const TilingChoice kTilings[] = {
     {16, 0, K_COLUMN}, {16, 1, K_COLUMN}, {8, 1, K_COLUMN}, {4, 1, K_COLUMN}, {0, 0, K_CELL_SLOPE},
     {0, 1, K_CELL},    {0, 1, K_CELL_SLOPE}, {0, 1, K_WTILE}, {16, 1, K_WTILE}, {8, 1, K_WTILE},
     {4, 1, K_WTILE},   {8, 0, K_WTILE},      {0, 1, K_CELLZ},  {0, 0, K_CELLZ},  {0, 1, K_CELLYZ},
-    {0, 0, K_CELLYZ},  {0, 1, K_CELLX2},     {0, 0, K_CELLX2}};
+    {0, 0, K_CELLYZ},  {0, 1, K_CELLX2},     {0, 0, K_CELLX2}, {0, 1, K_CELLZP}, {0, 0, K_CELLZP}};
 static const int kNumTilings = sizeof(kTilings) / sizeof(kTilings[0]);
 
 // the column kernel covers the fast MUSCL forms with a symmetric limited slope
@@ -276,6 +287,21 @@ static void launch_range(drn_ctx* ctx, int kind, const void* tiles, int ntiles, 
     if (ntiles > 0)
       hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, FAST, 256, true>), dim3(ntiles), dim3(256), 0, ctx->stream,
                          ctx->d_pd, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
+  } else if (kind == K_CELLZP) {
+    if (ntiles > 0) {
+      // resident workgroups per CU of this instantiation, queried once
+      static int per_cu = 0;
+      if (per_cu <= 0 &&
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stage_cell_persist<T, RECON, LIM, FAST>, 256, 0) !=
+              hipSuccess)
+        per_cu = 1;
+      if (per_cu <= 0) per_cu = 1;
+      // a multiple of 8 keeps every workgroup on the XCD of its first item
+      int g = std::max(8, (ctx->n_cu * per_cu) / 8 * 8);
+      if (g > ntiles) g = ntiles;
+      hipLaunchKernelGGL((k_stage_cell_persist<T, RECON, LIM, FAST>), dim3(g), dim3(256), 0, ctx->stream, ctx->d_pd,
+                         (const TileCell*)tiles, ntiles, src, base, dst, ctx->vstride, factor);
+    }
   }
   else
     launch_march_t<T, RECON, LIM, FAST>(ctx, (const TileMarch*)tiles, ntiles, src, base, dst, factor,
@@ -330,8 +356,8 @@ static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int l
       go(K_CELLX2, kTilings[ctx->tiling].xcd ? ctx->d_tx2_xcd : ctx->d_tx2, (int)ctx->tx2.size());
     } else if (kind == K_CELLYZ) {
       go(K_CELLYZ, kTilings[ctx->tiling].xcd ? ctx->d_tyz_xcd : ctx->d_tyz, (int)ctx->tyz.size());
-    } else if (kind == K_CELLZ) {
-      go(K_CELLZ, kTilings[ctx->tiling].xcd ? ctx->d_tz_xcd : ctx->d_tz, (int)ctx->tz.size());
+    } else if (kind == K_CELLZ || kind == K_CELLZP) {
+      go(kind, kTilings[ctx->tiling].xcd ? ctx->d_tz_xcd : ctx->d_tz, (int)ctx->tz.size());
     } else if (kind == K_CELL || kind == K_CELL_SLOPE) {
       go(kind, kTilings[ctx->tiling].xcd ? ctx->d_tiles_xcd : ctx->d_tiles, (int)ctx->tiles.size());
     } else if (kind == K_WTILE) {
@@ -355,10 +381,10 @@ static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int l
         const int z0 = ctx->tyz_first[p];
         const TileCell* tl = (kTilings[ctx->tiling_pp].xcd ? ctx->d_tyz_xcd_pp : ctx->d_tyz) + z0;
         go(K_CELLYZ, tl, ctx->tyz_first[p + 1] - z0);
-      } else if (kind == K_CELLZ) {
+      } else if (kind == K_CELLZ || kind == K_CELLZP) {
         const int z0 = ctx->tz_first[p];
         const TileCell* tl = (kTilings[ctx->tiling_pp].xcd ? ctx->d_tz_xcd_pp : ctx->d_tz) + z0;
-        go(K_CELLZ, tl, ctx->tz_first[p + 1] - z0);
+        go(kind, tl, ctx->tz_first[p + 1] - z0);
       } else if (kind == K_WTILE && wn > 0) {
         go(K_WTILE, ctx->d_wt_pp + w0, wn);
       } else if (kind == K_CELL || kind == K_CELL_SLOPE || kind == K_WTILE || mn == 0) {
@@ -611,7 +637,10 @@ int drn_ctx_create(int device, int precision, drn_ctx** out) {
   drn_ctx* ctx = new drn_ctx();
   ctx->device = device;
   ctx->prec = precision;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      ctx->n_cu <= 0) {
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return DRN_E_HIP;
   }

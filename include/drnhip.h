@@ -204,7 +204,10 @@ int drn_post_variables(drn_ctx* ctx, int field, int patch, int mask, double ref_
  * 29, 30 as 27 / 28 with 8 wavefronts per workgroup sharing their y faces
  *       through LDS (one barrier);
  * 31, 32 as 27 / 28 with two cells per thread along i, the x face between
- *       them evaluated once.
+ *       them evaluated once;
+ * 33, 34 the work list of 27 / 28 in one persistent launch: as many
+ *       workgroups as the device holds at once (a multiple of 8), each
+ *       walking the list with a stride of the grid size.
  * Default for both forms: 11; drn_autotune picks per device. */
 int drn_num_tilings(void);
 int drn_set_tiling(drn_ctx* ctx, int launch, int choice);
