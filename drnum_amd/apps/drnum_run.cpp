@@ -7,7 +7,7 @@
 // coefficients, run the time steps, copy the result back to the host.
 //
 // usage: drnum_run <grid file> <case file> <out file> <steps> <dt> <mode> [f64] [levelset]
-//                  [restart=<file.dnd>] [dnd=<base name>]
+//                  [restart=<file.dnd>] [dnd=<base name>] [codes=<code string>]
 //   case file (binary, little endian), per patch in grid order:
 //     int32 bc[6], float64 par[6][16], then 5*N initial values in `real`
 //     (variable-major, the layout of field 0 in Patch::m_Data);
@@ -21,6 +21,10 @@
 //              file (PatchGrid::readData), the time from its header
 //   dnd=B:     field 0 is also written to B_<steps, 6 digits>.dnd at the end
 //              (PatchGrid::writeData), with time = restart time + steps*dt
+//   codes=S:   the iterator gets the code string S (commas may separate the
+//              words) and an IteratorFeeder hands it the patch groups of equal
+//              code string; a group it does not match stops the run (exit 5).
+//              Without codes=, every patch is added to the iterator.
 // Prints one line per phase and exits non-zero on any error.
 #include <cstdio>
 #include <cstdlib>
@@ -37,7 +41,8 @@ typedef AusmPlus<5, Upwind2<5, VanAlbada>, PerfectGas> euler_t;
 
 template <typename real>
 static int run(const std::string& grid_file, const std::string& case_file, const std::string& out_file, int steps,
-               double dt, const std::string& mode, bool levelset, const std::string& restart, const std::string& dnd) {
+               double dt, const std::string& mode, bool levelset, const std::string& restart, const std::string& dnd,
+               const std::string& codes) {
   PatchGrid<double> grid;
   std::string err;
   if (grid.readGrid(grid_file, 1.0, &err)) {
@@ -96,7 +101,20 @@ static int run(const std::string& grid_file, const std::string& case_file, const
   }
 
   HIP_CartesianIterator<5, euler_t, real> iterator(euler_t(), 0);
-  for (auto& P : patches) iterator.addPatch(&P);
+  if (codes.empty()) {
+    for (auto& P : patches) iterator.addPatch(&P);
+  } else {
+    iterator.setCodeString(CodeString(codes));
+    IteratorFeeder<real> feeder;
+    feeder.addIterator(&iterator);
+    try {
+      feeder.feed(grid, pp);
+    } catch (const FeedError& e) {
+      fprintf(stderr, "feed: %s\n", e.what());
+      return 5;
+    }
+    printf("feed: %zu patches for \"%s\"\n", iterator.numPatches(), iterator.getCodeString().c_str());
+  }
   iterator.updateDevice();
   HIP_LevelSetBC<5, euler_t, real>* ls = nullptr;
   if (levelset) ls = new HIP_LevelSetBC<5, euler_t, real>(&iterator);
@@ -137,21 +155,27 @@ static int run(const std::string& grid_file, const std::string& case_file, const
 
 int main(int argc, char** argv) {
   if (argc < 7) {
-    fprintf(stderr, "usage: %s <grid> <case> <out> <steps> <dt> <rk|hip|graph> [f64] [levelset]\n", argv[0]);
+    fprintf(stderr,
+            "usage: %s <grid> <case> <out> <steps> <dt> <rk|hip|graph> [f64] [levelset] [restart=F] [dnd=B] "
+            "[codes=S]\n",
+            argv[0]);
     return 2;
   }
   bool f64 = false, levelset = false;
-  std::string restart, dnd;
+  std::string restart, dnd, codes;
   for (int a = 7; a < argc; ++a) {
     if (!strcmp(argv[a], "f64")) f64 = true;
     if (!strcmp(argv[a], "levelset")) levelset = true;
     if (!strncmp(argv[a], "restart=", 8)) restart = argv[a] + 8;
     if (!strncmp(argv[a], "dnd=", 4)) dnd = argv[a] + 4;
+    if (!strncmp(argv[a], "codes=", 6)) codes = argv[a] + 6;
   }
   try {
     if (f64)
-      return run<double>(argv[1], argv[2], argv[3], atoi(argv[4]), atof(argv[5]), argv[6], levelset, restart, dnd);
-    return run<float>(argv[1], argv[2], argv[3], atoi(argv[4]), atof(argv[5]), argv[6], levelset, restart, dnd);
+      return run<double>(argv[1], argv[2], argv[3], atoi(argv[4]), atof(argv[5]), argv[6], levelset, restart, dnd,
+                         codes);
+    return run<float>(argv[1], argv[2], argv[3], atoi(argv[4]), atof(argv[5]), argv[6], levelset, restart, dnd,
+                      codes);
   } catch (const DeviceError& e) {
     fprintf(stderr, "device error: %s\n", e.what());
     return 4;

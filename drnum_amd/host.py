@@ -41,6 +41,12 @@ _SIGS = {
     "drnh_dnd_write": (C.c_int, [C.c_char_p, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_void_p), _I64P]),
     "drnh_dnd_read": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_void_p), _I64P,
                                 C.POINTER(C.c_int)]),
+    "drnh_codestring": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int]),
+    "drnh_codestring_compare": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_int32), C.c_char_p, C.c_int]),
+    "drnh_grid_patch_codes": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int]),
+    "drnh_grid_num_patch_groups": (C.c_int, [C.c_void_p]),
+    "drnh_grid_patch_group": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_char_p,
+                                        C.c_int]),
     "drnh_grid_to_device": (C.c_int, [C.c_void_p, C.c_void_p]),
     "drnh_grid_upload_exchange": (C.c_int, [C.c_void_p, C.c_void_p]),
     "drnh_grid_upload_levelset": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -69,6 +75,33 @@ def exported_symbols():
 
 def _ptr(a, t):
     return a.ctypes.data_as(C.POINTER(t))
+
+
+def _string_out(call, what, cap=256):
+    """Run call(buf, cap) of a drnh_* string getter, growing the buffer on -2."""
+    while True:
+        buf = C.create_string_buffer(cap)
+        rc = call(buf, cap)
+        if rc == -2:
+            cap *= 4
+            continue
+        if rc < 0:
+            raise RuntimeError("%s failed (%d)" % (what, rc))
+        return buf.value.decode()
+
+
+def codestring(text: str) -> str:
+    """CodeString(text) (codestring.cpp:33-103): the normalised code string."""
+    return _string_out(lambda b, n: lib().drnh_codestring(text.encode(), b, n), "drnh_codestring")
+
+
+def codestring_compare(a: str, b: str):
+    """CodeString(a).compareCodeString(CodeString(b)) (codestring.cpp:106-174):
+    (allsame, overruled, underruled, merged string)."""
+    flags = (C.c_int32 * 3)()
+    m = _string_out(lambda buf, n: lib().drnh_codestring_compare(a.encode(), b.encode(), flags, buf, n),
+                    "drnh_codestring_compare")
+    return bool(flags[0]), bool(flags[1]), bool(flags[2]), m
 
 
 class HostGrid:
@@ -107,6 +140,25 @@ class HostGrid:
 
     def read(self, path, scale=1.0):
         self._chk(self.L.drnh_grid_read(self.h, path.encode(), scale), "drnh_grid_read")
+
+    def patch_codes(self, p) -> str:
+        """The code string of patch p as read from the grid file."""
+        return _string_out(lambda b, n: self.L.drnh_grid_patch_codes(self.h, p, b, n), "drnh_grid_patch_codes")
+
+    def patch_groups(self):
+        """PatchGroups of the grid (patchgroups.cpp:35-101), in creation order:
+        dicts with type, patches (grid indices), codes, exceptions."""
+        out = []
+        for g in range(self.L.drnh_grid_num_patch_groups(self.h)):
+            info = (C.c_int32 * 3)()
+            codes = _string_out(lambda b, n: self.L.drnh_grid_patch_group(self.h, g, info, None, b, n),
+                                "drnh_grid_patch_group")
+            members = np.zeros(max(info[1], 1), np.int32)
+            _string_out(lambda b, n: self.L.drnh_grid_patch_group(self.h, g, info, _ptr(members, C.c_int32), b, n),
+                        "drnh_grid_patch_group")
+            out.append(dict(type=int(info[0]), patches=members[:info[1]].tolist(), codes=codes,
+                            exceptions=bool(info[2])))
+        return out
 
     def num_patches(self):
         return self.L.drnh_grid_num_patches(self.h)

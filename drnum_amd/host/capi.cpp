@@ -91,6 +91,55 @@ int drnh_grid_read(drnh_grid* g, const char* path, double scale) {
   return with(g, [&](auto& G) { return G.readGrid(path, scale, &g->err); });
 }
 
+namespace {
+
+int put_string(const std::string& s, char* out, int cap) {
+  if (!out || cap < 1) return -1;
+  if (s.size() + 1 > (size_t)cap) return -2;
+  std::memcpy(out, s.c_str(), s.size() + 1);
+  return (int)s.size();
+}
+
+}  // namespace
+
+int drnh_codestring(const char* text, char* out, int cap) {
+  if (!text) return -1;
+  return put_string(CodeString(text), out, cap);
+}
+
+int drnh_codestring_compare(const char* a, const char* b, int32_t* flags3, char* out, int cap) {
+  if (!a || !b || !flags3) return -1;
+  bool same, over, under;
+  const CodeString m = CodeString(a).compareCodeString(CodeString(b), same, over, under);
+  flags3[0] = same;
+  flags3[1] = over;
+  flags3[2] = under;
+  return put_string(m, out, cap);
+}
+
+int drnh_grid_patch_codes(const drnh_grid* g, int p, char* out, int cap) {
+  if (bad_patch(g, p)) return -1;
+  return with(g, [&](const auto& G) { return put_string(G.patch((size_t)p).solver_codes, out, cap); });
+}
+
+int drnh_grid_num_patch_groups(const drnh_grid* g) {
+  if (!g) return -1;
+  return with(g, [&](const auto& G) { return (int)G.getPatchGroups().accessNumPatchGroups(); });
+}
+
+int drnh_grid_patch_group(const drnh_grid* g, int group, int32_t* info3, int32_t* patches, char* out, int cap) {
+  if (!g || !info3 || group < 0 || group >= drnh_grid_num_patch_groups(g)) return -1;
+  return with(g, [&](const auto& G) {
+    const SinglePatchGroup& s = G.getPatchGroups().accessSinglePatchGroup((size_t)group);
+    info3[0] = (int32_t)s.patch_type;
+    info3[1] = (int32_t)s.patches.size();
+    info3[2] = s.solver_exceptions ? 1 : 0;
+    if (patches)
+      for (size_t i = 0; i < s.patches.size(); ++i) patches[i] = (int32_t)s.patches[i];
+    return put_string(s.solver_codes, out, cap);
+  });
+}
+
 int drnh_grid_num_patches(const drnh_grid* g) {
   if (!g) return -1;
   return with(g, [&](const auto& G) { return (int)G.numPatches(); });

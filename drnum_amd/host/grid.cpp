@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <sstream>
 
 #ifdef _OPENMP
@@ -283,9 +284,10 @@ int PatchGrid<G>::readGrid(const std::string& path, double scale, std::string* e
     }
     CartesianPatch<G> P(n[0], n[1], n[2], s, od, Ld);
     P.comment = comment;
-    std::string tok;
-    while (is >> tok) P.solver_codes.push_back(tok);
-    addPatch(P);
+    // the rest of the entry is the patch's code string (CodeString::buildFrom)
+    P.solver_codes = CodeString(std::string(std::istreambuf_iterator<char>(is), std::istreambuf_iterator<char>()));
+    const int id = addPatch(P);
+    groups_.insertPatch((size_t)id, 1001, patches_[(size_t)id].solver_codes);
   }
   return 0;
 }

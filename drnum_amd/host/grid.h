@@ -16,6 +16,8 @@
 #include <string>
 #include <vector>
 
+#include "codestring.h"
+
 namespace drnum {
 
 // face functor codes (include/drnhip.h DRN_BC_*)
@@ -75,7 +77,7 @@ class CartesianPatch {
   int face[6] = {FACE_SLIP, FACE_SLIP, FACE_SLIP, FACE_SLIP, FACE_SLIP, FACE_SLIP};
   double face_par[6][16] = {};
   std::string comment;
-  std::vector<std::string> solver_codes;
+  CodeString solver_codes;  // Patch::m_solvercodes, read by Patch::readSolverCodes (patch.cpp:259-262)
 
  private:
   int lin(G s, int axis, size_t& ref, G& coeff) const;  // linearCCInterpolCoeffs
@@ -112,8 +114,12 @@ template <typename G>
 class PatchGrid {
  public:
   int addPatch(const CartesianPatch<G>& p);
-  // PatchGrid::readGrid (patchgrid.cpp:479-561): "1001 {...}" entries, "0" ends
+  // PatchGrid::readGrid (patchgrid.cpp:479-561): "1001 {...}" entries, "0" ends;
+  // every patch read also enters the patch groups (patchgrid.cpp:549)
   int readGrid(const std::string& path, double scale, std::string* err);
+  // PatchGrid::getPatchGroups (patchgrid.h:310); groups allow solver exceptions
+  // as the reference's grid constructs them (patchgrid.cpp:35)
+  const PatchGroups& getPatchGroups() const { return groups_; }
   size_t numPatches() const { return patches_.size(); }
   CartesianPatch<G>& patch(size_t p) { return patches_[p]; }
   const CartesianPatch<G>& patch(size_t p) const { return patches_[p]; }
@@ -138,6 +144,7 @@ class PatchGrid {
 
  private:
   std::vector<CartesianPatch<G>> patches_;
+  PatchGroups groups_{true};
   std::vector<DonorTables> tables_;
   std::vector<std::vector<G>> lsg_;
   std::vector<std::vector<std::vector<LayerCell>>> inner_, outer_;

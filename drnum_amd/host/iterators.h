@@ -10,6 +10,8 @@
 //                            updateHost() / updateDevice() / addPatch(patch)
 //   HIP_CartesianIterator<DIM, OP>(OP op, int device = 0, size_t thread_limit = 0)
 //                            the GPU_CartesianIterator<DIM, OP> replacement
+//   IteratorFeeder           addIterator / feed: patch groups to the iterators
+//                            of equal code string (iteratorfeeder.cpp:23-62)
 //   GenericOperation         post operation (genericoperation.h:24-31)
 //   HIP_LevelSetBC           level-set fill as a post operation (the
 //                            CPU_LevelSetObjectBC<LSOBCCompressibleEulerOp> role)
@@ -196,13 +198,49 @@ class PatchIterator {
   void activatePatch(size_t i) { active_[i] = true; }
   void deactivatePatch(size_t i) { active_[i] = false; }
   bool patchActive(size_t i) const { return active_[i]; }
-  void setCodeString(const std::string& s) { codes_ = s; }
-  std::string getCodeString() const { return codes_; }
+  // setCodeString / getCodeString (patchiterator.h:61-70, 96-104): the string
+  // is normalised as a CodeString; the default is "void" (patchiterator.h:77)
+  void setCodeString(const CodeString& s) { codes_ = s; }
+  CodeString getCodeString() const { return codes_; }
 
  protected:
   std::vector<HostPatch<real>*> patches_;
   std::vector<bool> active_;
-  std::string codes_ = "void";
+  CodeString codes_{"void"};
+};
+
+// IteratorFeeder (iteratorfeeder.cpp:23-62).  feed() walks the grid's patch
+// groups; every iterator whose code string equals a group's receives all of
+// the group's patches, in group order.  A group no iterator matches stops the
+// run (the reference prints the code string and calls BUG): here FeedError,
+// naming the code string.  The reference's patches live in its PatchGrid; here
+// the caller passes the HostPatch of every grid patch, indexed as the grid.
+class FeedError : public std::runtime_error {
+ public:
+  using std::runtime_error::runtime_error;
+};
+
+template <typename real>
+class IteratorFeeder {
+ public:
+  void addIterator(PatchIterator<real>* iterator) { iterators_.push_back(iterator); }
+  void feed(const PatchGrid<double>& grid, const std::vector<HostPatch<real>*>& patches) {
+    if (patches.size() != grid.numPatches()) throw FeedError("feed: one HostPatch per grid patch expected");
+    const PatchGroups& groups = grid.getPatchGroups();
+    for (size_t g = 0; g < groups.accessNumPatchGroups(); ++g) {
+      const SinglePatchGroup& spg = groups.accessSinglePatchGroup(g);
+      bool found = false;
+      for (auto* it : iterators_) {
+        if (it->getCodeString() != spg.solver_codes) continue;
+        found = true;
+        for (size_t p : spg.patches) it->addPatch(patches[p]);
+      }
+      if (!found) throw FeedError("no iterator for the code string \"" + spg.solver_codes + "\"");
+    }
+  }
+
+ private:
+  std::vector<PatchIterator<real>*> iterators_;
 };
 
 template <typename real>

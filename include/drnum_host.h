@@ -46,6 +46,26 @@ int drnh_grid_patch_info(const drnh_grid* g, int p, int32_t* out_i, double* out_
  * drnh_grid_to_device converts it to conserved variables in the state precision. */
 int drnh_grid_set_faces(drnh_grid* g, int p, const int32_t* bc6, const double* par6x16);
 
+/* ---- solver code strings and patch groups -------------------------------
+ * CodeString (codestring.cpp:33-212) and PatchGroups (patchgroups.cpp:27-112),
+ * as the reference's IteratorFeeder (iteratorfeeder.cpp:29-62) uses them to
+ * route patches to iterators.  Strings are written NUL-terminated into
+ * out[cap]; the functions return the string length, or -2 when cap is too
+ * small, -1 on a bad argument. */
+/* CodeString(text): the normalised code string */
+int drnh_codestring(const char* text, char* out, int cap);
+/* CodeString(a).compareCodeString(CodeString(b), ...): flags3 = allsame,
+ * overruled, underruled; out = the word-wise merge ("" when they differ) */
+int drnh_codestring_compare(const char* a, const char* b, int32_t* flags3, char* out, int cap);
+/* code string of patch p as read from the grid file */
+int drnh_grid_patch_codes(const drnh_grid* g, int p, char* out, int cap);
+/* the grid's patch groups (filled by drnh_grid_read, exceptions allowed) */
+int drnh_grid_num_patch_groups(const drnh_grid* g);
+/* info3 = patch type, number of patches, solver exceptions (0/1); patches gets
+ * the grid indices of the group's patches (info3[1] of them, may be NULL);
+ * out = the group's code string */
+int drnh_grid_patch_group(const drnh_grid* g, int group, int32_t* info3, int32_t* patches, char* out, int cap);
+
 /* PatchGrid::computeDependencies; period[3] > 0 adds periodic images per axis
  * (not in the reference). */
 int drnh_grid_compute_dependencies(drnh_grid* g, const double* period3, int threads);

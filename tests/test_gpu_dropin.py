@@ -28,10 +28,10 @@ APP = os.path.join(ROOT, "drnum_amd", "lib", "drnum_run")
 pytestmark = pytest.mark.gpu
 
 
-def run_app(tmp_path, w, dtype, steps, mode, levelset=None):
+def run_app(tmp_path, w, dtype, steps, mode, levelset=None, extra=(), tag=""):
     grid = tmp_path / "grid"
     case = tmp_path / "case.bin"
-    out = tmp_path / ("out_%s.bin" % mode)
+    out = tmp_path / ("out_%s%s.bin" % (mode, tag))
     grid.write_text(grid_file_text(w))
     case.write_bytes(case_file_bytes(w, dtype, levelset))
     args = [APP, str(grid), str(case), str(out), str(steps), repr(float(w.dt)), mode]
@@ -39,7 +39,8 @@ def run_app(tmp_path, w, dtype, steps, mode, levelset=None):
         args.append("f64")
     if levelset is not None:
         args.append("levelset")
-    print("drnum_run", mode, np.dtype(dtype).name, flush=True)
+    args += list(extra)
+    print("drnum_run", mode, np.dtype(dtype).name, *extra, flush=True)
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     raw = np.frombuffer(out.read_bytes(), dtype=dtype)
@@ -119,3 +120,21 @@ def test_dropin_restart_file_continues_bitwise(gpu, tmp_path, dtype):
     cont = app(2, tmp_path / "b.bin", "restart=%s" % (tmp_path / "rs_000003.dnd"))
     full = app(5, tmp_path / "c.bin")
     np.testing.assert_array_equal(cont, full)
+
+
+def test_dropin_iterator_feeder(gpu, tmp_path):
+    """IteratorFeeder (iteratorfeeder.cpp:29-62): the iterator built with the
+    code string of the jet grid's one patch group gets every patch and gives the
+    same bits as the iterator the patches were added to directly; an iterator
+    whose code string matches no group stops the run, naming the group's codes."""
+    w = configs.jet(core=8, blocks=2)
+    codes = "fx fy fz trans trans trans trans jet trans 0"
+    direct = run_app(tmp_path, w, np.float32, 3, "hip")
+    fed = run_app(tmp_path, w, np.float32, 3, "hip", extra=["codes=" + codes.replace(" ", ",")], tag="_fed")
+    for a, b in zip(direct, fed):
+        assert np.array_equal(a, b)
+    grid, case = tmp_path / "grid", tmp_path / "case.bin"
+    r = subprocess.run([APP, str(grid), str(case), str(tmp_path / "x.bin"), "1", repr(float(w.dt)), "hip",
+                        "codes=fx,fy,fz,far,far,far,far,far,far,0"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 5, r.stdout + r.stderr
+    assert codes in r.stderr
