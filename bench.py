@@ -46,7 +46,7 @@ def setup(w, dtype, launch_tables=True):
     hg = host.HostGrid.from_workload(w, fp64_geometry=True)
     hg.compute_dependencies(w.periodic)
     ctx = hip.Context(dtype)
-    ctx.set_operator(w.recon, w.limiter)
+    ctx.set_operator(w.recon, w.limiter, w.viscous)
     hg.to_device(ctx)
     if launch_tables:
         hg.upload_exchange(ctx)

@@ -6,13 +6,14 @@
 // operator AusmPlus<5, Upwind2<5, VanAlbada>, PerfectGas>, add the RK
 // coefficients, run the time steps, copy the result back to the host.
 //
-// usage: drnum_run <grid file> <case file> <out file> <steps> <dt> <mode> [f64] [levelset]
+// usage: drnum_run <grid file> <case file> <out file> <steps> <dt> <mode> [f64] [levelset] [visc]
 //                  [restart=<file.dnd>] [dnd=<base name>] [codes=<code string>]
 //   case file (binary, little endian), per patch in grid order:
 //     int32 bc[6], float64 par[6][16], then 5*N initial values in `real`
 //     (variable-major, the layout of field 0 in Patch::m_Data);
 //     with `levelset`, followed by int32 n_spheres, int32 n_inner,
 //     float64 min_inner_rel and n_spheres x (xc, yc, zc, r) in float64
+//   visc: the operator is EulerViscFlux<AusmPlus<...>, CompressibleViscFlux<5, PerfectGas>>
 //   mode: rk     RungeKutta over the virtual iterator interface (reference loop)
 //         hip    HIP_RungeKutta (one library call per step)
 //         graph  HIP_RungeKutta::graph (captured steps)
@@ -38,8 +39,11 @@
 using namespace drnum;
 
 typedef AusmPlus<5, Upwind2<5, VanAlbada>, PerfectGas> euler_t;
+// the same Euler flux plus the viscous flux on every interior face (EaFlux,
+// drnumCylinder/main.h:127-163)
+typedef EulerViscFlux<euler_t, CompressibleViscFlux<5, PerfectGas>> visc_t;
 
-template <typename real>
+template <typename real, class OP>
 static int run(const std::string& grid_file, const std::string& case_file, const std::string& out_file, int steps,
                double dt, const std::string& mode, bool levelset, const std::string& restart, const std::string& dnd,
                const std::string& codes) {
@@ -100,7 +104,7 @@ static int run(const std::string& grid_file, const std::string& case_file, const
     printf("restart: t = %g from %s\n", t0, restart.c_str());
   }
 
-  HIP_CartesianIterator<5, euler_t, real> iterator(euler_t(), 0);
+  HIP_CartesianIterator<5, OP, real> iterator(OP(), 0);
   if (codes.empty()) {
     for (auto& P : patches) iterator.addPatch(&P);
   } else {
@@ -116,11 +120,11 @@ static int run(const std::string& grid_file, const std::string& case_file, const
     printf("feed: %zu patches for \"%s\"\n", iterator.numPatches(), iterator.getCodeString().c_str());
   }
   iterator.updateDevice();
-  HIP_LevelSetBC<5, euler_t, real>* ls = nullptr;
-  if (levelset) ls = new HIP_LevelSetBC<5, euler_t, real>(&iterator);
+  HIP_LevelSetBC<5, OP, real>* ls = nullptr;
+  if (levelset) ls = new HIP_LevelSetBC<5, OP, real>(&iterator);
 
   RungeKutta<real> rk;
-  HIP_RungeKutta<5, euler_t, real> hrk(&iterator);
+  HIP_RungeKutta<5, OP, real> hrk(&iterator);
   for (real a : {real(0.25), real(0.5), real(1.0)}) {
     rk.addAlpha(a);
     hrk.addAlpha(a);
@@ -156,26 +160,29 @@ static int run(const std::string& grid_file, const std::string& case_file, const
 int main(int argc, char** argv) {
   if (argc < 7) {
     fprintf(stderr,
-            "usage: %s <grid> <case> <out> <steps> <dt> <rk|hip|graph> [f64] [levelset] [restart=F] [dnd=B] "
+            "usage: %s <grid> <case> <out> <steps> <dt> <rk|hip|graph> [f64] [levelset] [visc] [restart=F] [dnd=B] "
             "[codes=S]\n",
             argv[0]);
     return 2;
   }
-  bool f64 = false, levelset = false;
+  bool f64 = false, levelset = false, visc = false;
   std::string restart, dnd, codes;
   for (int a = 7; a < argc; ++a) {
     if (!strcmp(argv[a], "f64")) f64 = true;
     if (!strcmp(argv[a], "levelset")) levelset = true;
+    if (!strcmp(argv[a], "visc")) visc = true;
     if (!strncmp(argv[a], "restart=", 8)) restart = argv[a] + 8;
     if (!strncmp(argv[a], "dnd=", 4)) dnd = argv[a] + 4;
     if (!strncmp(argv[a], "codes=", 6)) codes = argv[a] + 6;
   }
   try {
-    if (f64)
-      return run<double>(argv[1], argv[2], argv[3], atoi(argv[4]), atof(argv[5]), argv[6], levelset, restart, dnd,
-                         codes);
-    return run<float>(argv[1], argv[2], argv[3], atoi(argv[4]), atof(argv[5]), argv[6], levelset, restart, dnd,
-                      codes);
+    const int steps = atoi(argv[4]);
+    const double dt = atof(argv[5]);
+    if (f64 && visc)
+      return run<double, visc_t>(argv[1], argv[2], argv[3], steps, dt, argv[6], levelset, restart, dnd, codes);
+    if (f64) return run<double, euler_t>(argv[1], argv[2], argv[3], steps, dt, argv[6], levelset, restart, dnd, codes);
+    if (visc) return run<float, visc_t>(argv[1], argv[2], argv[3], steps, dt, argv[6], levelset, restart, dnd, codes);
+    return run<float, euler_t>(argv[1], argv[2], argv[3], steps, dt, argv[6], levelset, restart, dnd, codes);
   } catch (const DeviceError& e) {
     fprintf(stderr, "device error: %s\n", e.what());
     return 4;

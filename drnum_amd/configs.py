@@ -62,6 +62,7 @@ class Workload:
     ls_inner_layers: int = 3
     ls_min_inner_rel: float = 0.1
     steps: int = 1
+    viscous: bool = False   # CompressibleViscFlux added to every interior face flux
 
     @property
     def total_cells(self) -> int:
@@ -230,6 +231,43 @@ def kelvin_helmholtz(core: int = 64, blocks: int = 8, steps: int = 500, seed: in
     dt = 0.5 * h / (0.5 + np.sqrt(1.4 * P_AMB / 1.0))
     return Workload("kh%dx%d" % (blocks, core), patches, RECON_UPWIND2, LIM_VANALBADA, [1.0] * 3,
                     [0.25, 0.5, 1.0], dt, init, steps=steps)
+
+
+# ----------------------------------------------------------------------------
+# Viscous shear layer (SI units, so that Sutherland's viscosity matters)
+# ----------------------------------------------------------------------------
+
+def viscous_shear(core: int = 32, blocks=(2, 2, 2), cell: float = 1.0e-6, u_max: float = 30.0,
+                  steps: int = 200, periodic: bool = True) -> Workload:
+    """Periodic double shear layer of cubic cells of side `cell` (1 um by default)
+    at 1 bar and 300 K: u = u_max tanh((y - L/4)/d) for y < L/2 and its mirror
+    above, d = L/16, v = 0.05 u_max sin(2 pi x / L).  AUSM+ Upwind2 van Albada
+    plus CompressibleViscFlux (the EaFlux composition of drnumCylinder/main.h:
+    127-163).  The cell Reynolds number u_max h / nu is about 2 at any
+    resolution, so the viscous terms are of the order of the convective ones,
+    and the explicit viscous limit h^2/(6 nu) stays 8x above the convective time
+    step.  periodic=False puts slip walls on the domain faces instead of
+    periodic images."""
+    nb = list(blocks)
+    nc = [core * b for b in nb]
+    h = cell
+    L = [h * nc[a] for a in range(3)]
+    patches = block_grid(nc, nb, L, 2, [periodic] * 3, [BC_SLIP] * 6)
+    d = L[1] / 16.0
+
+    def init(x, y, z):
+        yy = np.mod(y, L[1])
+        u = np.where(yy < 0.5 * L[1], u_max * np.tanh((yy - 0.25 * L[1]) / d),
+                     -u_max * np.tanh((yy - 0.75 * L[1]) / d))
+        v = 0.05 * u_max * np.sin(2.0 * np.pi * x / L[0])
+        p = np.full_like(x, 1.0e5)
+        T = np.full_like(x, 300.0)
+        return np.stack([p, T, u, v, np.zeros_like(x)])
+
+    a = np.sqrt(1.4 * gas.R * 300.0)
+    dt = 0.5 * h / (u_max + a)
+    return Workload("shear%dx%d" % (nb[0] * nb[1] * nb[2], core), patches, RECON_UPWIND2, LIM_VANALBADA,
+                    L if periodic else [0.0] * 3, [0.25, 0.5, 1.0], dt, init, steps=steps, viscous=True)
 
 
 def _hash_uniform(key: np.ndarray, seed: int) -> np.ndarray:

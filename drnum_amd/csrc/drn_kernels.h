@@ -116,9 +116,57 @@ __device__ __forceinline__ void face_flux_r(const T* m, const T* a, const T* b, 
 // the two faces of a cell along DIR (index idx of n along that axis, byte
 // stride st): loads the four neighbours once, adds the minus face to res and
 // returns the plus face in fp (zero when the face does not exist)
-template <typename T, int RECON, int LIM, bool FAST, int DIR>
+// geometry of one cell for the viscous flux: byte strides, indices, cells per
+// axis, (real)0.5/delta (the cell gradients) and 1/delta (the face gradient)
+template <typename T>
+struct ViscGeom {
+  uint32_t st[3];
+  int idx[3], n[3];
+  T dh[3], dn[3];
+};
+
+// getXGrad / getYGrad / getZGrad (cartesianpatch_common.h:188-300) at the cell
+// with byte offset cc: central difference along AX, one sided at the patch
+// ends with D doubled per clamped side; the y gradient is 0 when sizeJ() <= 2
+template <typename T, int AX>
+__device__ __forceinline__ void cell_grad(const char* const* sv, const ViscGeom<T>& G, uint32_t cc, T* g) {
+  if (AX == 1 && G.n[1] <= 2) {
+    zero5(g);
+    return;
+  }
+  T D = G.dh[AX];
+  uint32_t lo = cc - G.st[AX], hi = cc + G.st[AX];
+  if (G.idx[AX] < 1) {
+    lo = cc;
+    D *= T(2);
+  }
+  if (G.idx[AX] + 1 >= G.n[AX]) {
+    hi = cc;
+    D *= T(2);
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) g[v] = (ldv<T>(sv[v], hi) - ldv<T>(sv[v], lo)) * D;
+}
+
+// viscous part of the face between the cells at byte offsets cl (minus side)
+// and cr along DIR, added to the face flux after its Euler part (the EaFlux
+// composition, drnumCylinder/main.h:127-163); both cells share the indices of
+// G on the tangential axes
+template <typename T, bool FAST, int DIR>
+__device__ __forceinline__ void visc_add(const char* const* sv, const ViscGeom<T>& G, uint32_t cl, uint32_t cr,
+                                         const T* vl, const T* vr, T A, T* flux) {
+  constexpr int TA = DIR == 0 ? 1 : 0, TB = DIR == 2 ? 1 : 2;
+  T gla[5], glb[5], gra[5], grb[5];
+  cell_grad<T, TA>(sv, G, cl, gla);
+  cell_grad<T, TB>(sv, G, cl, glb);
+  cell_grad<T, TA>(sv, G, cr, gra);
+  cell_grad<T, TB>(sv, G, cr, grb);
+  visc_face<T, FAST, DIR>(vl, vr, gla, gra, glb, grb, G.dn[DIR], A, flux);
+}
+
+template <typename T, int RECON, int LIM, bool FAST, int DIR, bool VISC = false>
 __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, uint32_t st, int idx, int n,
-                                             const T* u0, T A, T* res, T* fp) {
+                                             const T* u0, T A, T* res, T* fp, const ViscGeom<T>* G = nullptr) {
   T um2[5], um1[5], up1[5], up2[5];
   const bool have = n > 2;
 #pragma unroll
@@ -166,6 +214,7 @@ __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, 
       }
       zero5(f);
       ausm_flux<T, FAST, DIR>(vl, vr, A, f);
+      if constexpr (VISC) visc_add<T, FAST, DIR>(sv, *G, c - st, c, um1, u0, A, f);
 #pragma unroll
       for (int v = 0; v < 5; ++v) res[v] += f[v];
     }
@@ -177,16 +226,21 @@ __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, 
         vr[v] = up1[v] - T(0.5) * sp[v];
       }
       ausm_flux<T, FAST, DIR>(vl, vr, A, fp);
+      if constexpr (VISC) visc_add<T, FAST, DIR>(sv, *G, c, c + st, u0, up1, A, fp);
     }
   } else {
     if (have && idx > 0) {
       T f[5];
       zero5(f);
       face_flux_r<T, RECON, LIM, FAST, DIR>(um2, um1, u0, up1, idx >= 2, idx + 1 < n, A, f);
+      if constexpr (VISC) visc_add<T, FAST, DIR>(sv, *G, c - st, c, um1, u0, A, f);
 #pragma unroll
       for (int v = 0; v < 5; ++v) res[v] += f[v];
     }
-    if (have && idx < n - 1) face_flux_r<T, RECON, LIM, FAST, DIR>(um1, u0, up1, up2, idx >= 1, idx + 2 < n, A, fp);
+    if (have && idx < n - 1) {
+      face_flux_r<T, RECON, LIM, FAST, DIR>(um1, u0, up1, up2, idx >= 1, idx + 2 < n, A, fp);
+      if constexpr (VISC) visc_add<T, FAST, DIR>(sv, *G, c, c + st, u0, up1, A, fp);
+    }
   }
 }
 
@@ -196,7 +250,7 @@ __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, 
 //             next cell.  Every lane's z face k-1/2 is the plus z face of the
 //             lane below it, handed over by a lane shuffle, so z faces are
 //             evaluated once: 5 face evaluations per cell instead of 6.
-template <typename T, int RECON, int LIM, bool FAST, int BS, bool ZS>
+template <typename T, int RECON, int LIM, bool FAST, int BS, bool ZS, bool VISC = false>
 __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds, const TileCell t,
                                                 const T* __restrict__ src, const T* base, T* dst, uint64_t vs,
                                                 T factor) {
@@ -230,9 +284,27 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
   T u0[5], fxp[5], fyp[5], fzp[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) u0[v] = ldv<T>(sv[v], c);
+  ViscGeom<T> G;
+  if constexpr (VISC) {
+    G.st[0] = sx;
+    G.st[1] = sy;
+    G.st[2] = W;
+    G.idx[0] = i;
+    G.idx[1] = j;
+    G.idx[2] = k;
+    G.n[0] = ni;
+    G.n[1] = nj;
+    G.n[2] = nk;
+    G.dh[0] = T(0.5) / dx;
+    G.dh[1] = T(0.5) / dy;
+    G.dh[2] = T(0.5) / dz;
+    G.dn[0] = T(1.0) / dx;
+    G.dn[1] = T(1.0) / dy;
+    G.dn[2] = T(1.0) / dz;
+  }
   if (own) {
-    axis_faces_r<T, RECON, LIM, FAST, 0>(sv, c, sx, i, ni, u0, Ax, res, fxp);
-    axis_faces_r<T, RECON, LIM, FAST, 1>(sv, c, sy, j, nj, u0, Ay, res, fyp);
+    axis_faces_r<T, RECON, LIM, FAST, 0, VISC>(sv, c, sx, i, ni, u0, Ax, res, fxp, &G);
+    axis_faces_r<T, RECON, LIM, FAST, 1, VISC>(sv, c, sy, j, nj, u0, Ay, res, fyp, &G);
   }
   if constexpr (ZS) {
     // z face k-1/2 of every lane's cell (lane 63 included), then the plus face
@@ -248,6 +320,7 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
         up1[v] = k + 1 < nk ? ldv<T>(sv[v], c + W) : u0[v];
       }
       face_flux_r<T, RECON, LIM, FAST, 2>(um2, um1, u0, up1, k >= 2, k + 1 < nk, Az, fzm);
+      if constexpr (VISC) visc_add<T, FAST, 2>(sv, G, c - W, c, um1, u0, Az, fzm);
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
@@ -257,7 +330,7 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
     }
     if (!own) return;
   } else {
-    axis_faces_r<T, RECON, LIM, FAST, 2>(sv, c, W, k, nk, u0, Az, res, fzp);
+    axis_faces_r<T, RECON, LIM, FAST, 2, VISC>(sv, c, W, k, nk, u0, Az, res, fzp, &G);
   }
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
@@ -325,11 +398,11 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
   }
 }
 
-template <typename T, int RECON, int LIM, bool FAST, int BS, bool ZS = false>
+template <typename T, int RECON, int LIM, bool FAST, int BS, bool ZS = false, bool VISC = false>
 __global__ __launch_bounds__(BS) void k_stage_cell(const PatchDev* __restrict__ pds, const TileCell* __restrict__ tiles,
                                                    const T* __restrict__ src, const T* base, T* dst, uint64_t vs,
                                                    T factor) {
-  stage_cell_item<T, RECON, LIM, FAST, BS, ZS>(pds, tiles[blockIdx.x], src, base, dst, vs, factor);
+  stage_cell_item<T, RECON, LIM, FAST, BS, ZS, VISC>(pds, tiles[blockIdx.x], src, base, dst, vs, factor);
 }
 
 // Persistent form of k_stage_cell<..., ZS = true>: a grid sized to the resident

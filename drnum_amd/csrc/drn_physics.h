@@ -180,6 +180,113 @@ __device__ __forceinline__ void ausm_flux(const T* vl, const T* vr, T A, T* f) {
   f[4] += h0 * (L.H + R.H) - ha * (R.H - L.H);
 }
 
+// ---------------------------------------------------------------------------
+// viscous flux: CompressibleViscFlux<5, PerfectGas> (compressibleviscflux.h:33-268)
+// ---------------------------------------------------------------------------
+
+// PerfectGas::mu (perfectgas.h:100-111), Sutherland's law of the conserved
+// state; the fast form writes (T/T0)^1.5 as r*sqrt(r) and divides by a reciprocal
+template <typename T, bool FAST>
+__device__ __forceinline__ T gas_mu(const T* var) {
+  const T T0 = T(300.55), C = T(111.00), mu0 = T(1.781e-5);
+  const T ir = drcp<T, FAST>(var[0]);
+  const T u = var[1] * ir, v = var[2] * ir, w = var[3] * ir;
+  const T e = var[4] * ir - T(0.5) * (u * u + v * v + w * w);
+  if constexpr (FAST) {
+    const T Tt = e * (T(1) / Gas<T>::CV);
+    const T r = Tt * (T(1) / T0);
+    return mu0 * (r * dsqrt(r)) * (T0 + C) * drcp<T, true>(Tt + C);
+  } else {
+    const T Tt = e / Gas<T>::CV;
+    T p15;
+    if constexpr (sizeof(T) == 4)
+      p15 = powf(Tt / T0, 1.5f);
+    else
+      p15 = pow(Tt / T0, 1.5);
+    return mu0 * p15 * (T0 + C) / (Tt + C);
+  }
+}
+
+// derivative of the velocity component c from the conserved gradient g
+template <typename T>
+__device__ __forceinline__ T dvel(const T* g, const T* var, T ir, int c) {
+  return (g[c] - g[0] * var[c] * ir) * ir;
+}
+
+// (gamma - 1)/R times the temperature derivative, as the reference's dT_d*
+template <typename T>
+__device__ __forceinline__ T dtemp(const T* g, const T* var, T ir, T gamma1) {
+  return gamma1 * ir * ir *
+         (g[4] * var[0] - g[0] * var[4] - var[1] * g[1] - var[2] * g[2] - var[3] * g[3] +
+          g[0] * ir * (var[1] * var[1] + var[2] * var[2] + var[3] * var[3]));
+}
+
+// viscous part of the face between the cells vl (minus side) and vr along DIR:
+// ga/gb are the gradients along the first and second tangential axis (x face:
+// y, z; y face: x, z; z face: x, y) of the left (l) and right (r) cell, D is
+// 1/delta along DIR.  Subtracts the stresses and the heat flux from flux[1..4].
+template <typename T, bool FAST, int DIR>
+__device__ __forceinline__ void visc_face(const T* vl, const T* vr, const T* gla, const T* gra, const T* glb,
+                                          const T* grb, T D, T A, T* flux) {
+  T var[5], gn[5], ga[5], gb[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    var[v] = T(0.5) * (vl[v] + vr[v]);
+    gn[v] = (vr[v] - vl[v]) * D;
+    ga[v] = T(0.5) * (gla[v] + gra[v]);
+    gb[v] = T(0.5) * (glb[v] + grb[v]);
+  }
+  const T gamma1 = Gas<T>::GAMMA - T(1.0);
+  const T ir = drcp<T, FAST>(var[0]);
+  const T mu = gas_mu<T, FAST>(var);
+  const T FR23 = T(0.6666666);  // drnum.h:68
+  const T k_heat = FAST ? mu * Gas<T>::CP * (T(1) / T(0.7)) : mu * Gas<T>::CP / T(0.7);
+  const T F2 = FR23 * mu, F3 = mu, F4 = FAST ? k_heat * (T(1) / Gas<T>::R) : k_heat / Gas<T>::R;
+  const T* gx = DIR == 0 ? gn : ga;
+  const T* gy = DIR == 0 ? ga : (DIR == 1 ? gn : gb);
+  const T* gz = DIR == 2 ? gn : gb;
+  if constexpr (DIR == 0) {
+    const T du_dx = dvel(gx, var, ir, 1), dv_dx = dvel(gx, var, ir, 2), dw_dx = dvel(gx, var, ir, 3);
+    const T dT_dx = dtemp(gx, var, ir, gamma1);
+    const T du_dy = dvel(gy, var, ir, 1), dv_dy = dvel(gy, var, ir, 2);
+    const T du_dz = dvel(gz, var, ir, 1), dw_dz = dvel(gz, var, ir, 3);
+    const T tau_xx = F2 * (T(2.0) * du_dx - dv_dy - dw_dz);
+    const T tau_xy = F3 * (du_dy + dv_dx);
+    const T tau_xz = F3 * (du_dz + dw_dx);
+    const T q = F4 * dT_dx;
+    flux[1] -= tau_xx * A;
+    flux[2] -= tau_xy * A;
+    flux[3] -= tau_xz * A;
+    flux[4] -= ((tau_xx * var[1] + tau_xy * var[2] + tau_xz * var[3]) * ir + q) * A;
+  } else if constexpr (DIR == 1) {
+    const T du_dx = dvel(gx, var, ir, 1), dv_dx = dvel(gx, var, ir, 2);
+    const T du_dy = dvel(gy, var, ir, 1), dv_dy = dvel(gy, var, ir, 2), dw_dy = dvel(gy, var, ir, 3);
+    const T dT_dy = dtemp(gy, var, ir, gamma1);
+    const T dv_dz = dvel(gz, var, ir, 2), dw_dz = dvel(gz, var, ir, 3);
+    const T tau_yy = F2 * (-du_dx + T(2.0) * dv_dy - dw_dz);
+    const T tau_xy = F3 * (du_dy + dv_dx);
+    const T tau_yz = F3 * (dv_dz + dw_dy);
+    const T q = F4 * dT_dy;
+    flux[1] -= tau_xy * A;
+    flux[2] -= tau_yy * A;
+    flux[3] -= tau_yz * A;
+    flux[4] -= ((tau_xy * var[1] + tau_yy * var[2] + tau_yz * var[3]) * ir + q) * A;
+  } else {
+    const T du_dx = dvel(gx, var, ir, 1), dw_dx = dvel(gx, var, ir, 3);
+    const T dv_dy = dvel(gy, var, ir, 2), dw_dy = dvel(gy, var, ir, 3);
+    const T du_dz = dvel(gz, var, ir, 1), dv_dz = dvel(gz, var, ir, 2), dw_dz = dvel(gz, var, ir, 3);
+    const T dT_dz = dtemp(gz, var, ir, gamma1);
+    const T tau_zz = F2 * (-du_dx - dv_dy + T(2.0) * dw_dz);
+    const T tau_xz = F3 * (du_dz + dw_dx);
+    const T tau_yz = F3 * (dv_dz + dw_dy);
+    const T q = F4 * dT_dz;
+    flux[1] -= tau_xz * A;
+    flux[2] -= tau_yz * A;
+    flux[3] -= tau_zz * A;
+    flux[4] -= ((tau_xz * var[1] + tau_yz * var[2] + tau_zz * var[3]) * ir + q) * A;
+  }
+}
+
 // perfectgas.h:83-92
 template <typename T>
 __device__ __forceinline__ void cons_to_prim(const T* var, T& p, T& Tt, T& u, T& v, T& w) {

@@ -45,6 +45,7 @@ struct drn_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   int recon = DRN_RECON_UPWIND2, lim = DRN_LIM_VANALBADA;
+  bool visc = false;  // CompressibleViscFlux added to the interior face fluxes
 
   std::vector<drn_patch_desc> descs;
   std::vector<uint64_t> offs;
@@ -284,7 +285,10 @@ static void launch_range(drn_ctx* ctx, int kind, const void* tiles, int ntiles, 
       hipLaunchKernelGGL((k_stage_cell_yz<T, RECON, LIM, FAST>), dim3(ntiles), dim3(512), 0, ctx->stream, ctx->d_pd,
                          (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
   } else if (kind == K_CELLZ) {
-    if (ntiles > 0)
+    if (ntiles > 0 && ctx->visc)
+      hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, FAST, 256, true, true>), dim3(ntiles), dim3(256), 0,
+                         ctx->stream, ctx->d_pd, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
+    else if (ntiles > 0)
       hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, FAST, 256, true>), dim3(ntiles), dim3(256), 0, ctx->stream,
                          ctx->d_pd, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
   } else if (kind == K_CELLZP) {
@@ -326,8 +330,11 @@ static bool column_capable(const drn_ctx* ctx) {
   return fast && ctx->recon == DRN_RECON_UPWIND2 && (ctx->lim == DRN_LIM_VANALBADA || ctx->lim == DRN_LIM_MINMOD);
 }
 
-// effective kernel kind of a tiling choice in the current arithmetic/operator
+// effective kernel kind of a tiling choice in the current arithmetic/operator;
+// with the viscous flux every choice runs the z-shared cell kernel on its list
+// order (the other kernels carry the Euler flux only)
 static int effective_kind(const drn_ctx* ctx, int choice) {
+  if (ctx->visc) return K_CELLZ;
   const int k = kTilings[choice].kind;
   if (k == K_COLUMN && !column_capable(ctx)) return K_MARCH;
   if (k == K_CELL_SLOPE && !column_capable(ctx)) return K_CELL;
@@ -733,6 +740,23 @@ int drn_set_operator(drn_ctx* ctx, int recon, int limiter) {
   ctx->lim = limiter;
   if (ctx->laid_out) {
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    int rc = build_march(ctx);
+    if (rc) return rc;
+  }
+  if (ctx->gexec) {
+    (void)hipGraphExecDestroy(ctx->gexec);
+    ctx->gexec = nullptr;
+  }
+  return DRN_OK;
+}
+
+int drn_set_viscous(drn_ctx* ctx, int on) {
+  if (!ctx) return DRN_E_ARG;
+  if (on != 0 && on != 1) return fail(ctx, DRN_E_ARG, "viscous switch must be 0 or 1");
+  if (ctx->laid_out) HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->visc = on != 0;
+  // the kernel kinds of the tiling choices change with the switch
+  if (ctx->laid_out) {
     int rc = build_march(ctx);
     if (rc) return rc;
   }
@@ -1394,7 +1418,17 @@ int drn_autotune(drn_ctx* ctx, int launch, int* choice) {
   int best = drn_get_tiling(ctx, launch);
   float best_ms = 1e30f;
   int rc = DRN_OK;
+  std::vector<std::pair<int, int>> tried;  // (effective kind, list order) already timed as the same
   for (int c = 0; c < kNumTilings && rc == DRN_OK; ++c) {
+    const int ek = effective_kind(ctx, c);
+    // choices that fall back to the same cell kernel on the same list are one candidate
+    if (ek == K_CELLZ || ek == K_CELL) {
+      const std::pair<int, int> key(ek, kTilings[c].xcd);
+      bool seen = false;
+      for (const auto& t : tried) seen = seen || t == key;
+      if (seen) continue;
+      tried.push_back(key);
+    }
     if (drn_set_tiling(ctx, launch, c) != DRN_OK) continue;
     rc = do_stage(ctx, ctx->map[0], ctx->map[1], ctx->map[2], 0.0, launch);  // warm-up
     if (rc) break;

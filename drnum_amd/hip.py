@@ -36,6 +36,7 @@ _SIGS = {
     "drn_last_error": (C.c_char_p, [C.c_void_p]),
     "drn_precision": (C.c_int, [C.c_void_p]),
     "drn_set_operator": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "drn_set_viscous": (C.c_int, [C.c_void_p, C.c_int]),
     "drn_patch_add": (C.c_int, [C.c_void_p, C.POINTER(PatchDesc), C.POINTER(C.c_int)]),
     "drn_layout": (C.c_int, [C.c_void_p]),
     "drn_num_patches": (C.c_int, [C.c_void_p]),
@@ -128,8 +129,9 @@ class Context:
             raise DrnError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
 
     # ---- set-up -----------------------------------------------------------
-    def set_operator(self, recon, limiter):
+    def set_operator(self, recon, limiter, viscous=False):
         self._chk(self.L.drn_set_operator(self.h, recon, limiter), "drn_set_operator")
+        self._chk(self.L.drn_set_viscous(self.h, 1 if viscous else 0), "drn_set_viscous")
 
     def add_patch(self, ni, nj, nk, seek, dxyz, bc, bc_par):
         d = PatchDesc()

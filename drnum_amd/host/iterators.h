@@ -60,6 +60,13 @@ struct Upwind1 {};       // reconstruction/upwind1.h:29-39
 struct PerfectGas {};    // perfectgas.h:31-34
 template <unsigned DIM, class TReconstruction, class TGas>
 struct AusmPlus {};      // fluxes/ausmplus.h:39-106
+template <unsigned DIM, class TGas>
+struct CompressibleViscFlux {};  // fluxes/compressibleviscflux.h:27-268
+// An Euler flux followed by a viscous flux on every interior face, the
+// composition the reference's applications write as their own flux class
+// (EaFlux::xField/yField/zField, drnumCylinder/main.h:127-163).
+template <class TEuler, class TViscous>
+struct EulerViscFlux {};
 
 template <class L>
 struct limiter_code;
@@ -84,11 +91,19 @@ template <class L>
 struct op_traits<AusmPlus<5, Upwind2<5, L>, PerfectGas>> {
   static constexpr int recon = DRN_RECON_UPWIND2;
   static constexpr int limiter = limiter_code<L>::value;
+  static constexpr int viscous = 0;
 };
 template <>
 struct op_traits<AusmPlus<5, Upwind1<5>, PerfectGas>> {
   static constexpr int recon = DRN_RECON_UPWIND1;
   static constexpr int limiter = DRN_LIM_VANALBADA;  // unused by first order
+  static constexpr int viscous = 0;
+};
+template <class TEuler>
+struct op_traits<EulerViscFlux<TEuler, CompressibleViscFlux<5, PerfectGas>>> {
+  static constexpr int recon = op_traits<TEuler>::recon;
+  static constexpr int limiter = op_traits<TEuler>::limiter;
+  static constexpr int viscous = 1;
 };
 
 class DeviceError : public std::runtime_error {
@@ -269,6 +284,7 @@ class HIP_CartesianIterator : public PatchIterator<real> {
     (void)thread_limit;
     check(drn_ctx_create(device, real_precision<real>::value, &ctx_), "drn_ctx_create");
     check(drn_set_operator(ctx_, op_traits<OP>::recon, op_traits<OP>::limiter), "drn_set_operator");
+    check(drn_set_viscous(ctx_, op_traits<OP>::viscous), "drn_set_viscous");
   }
   ~HIP_CartesianIterator() override { drn_ctx_destroy(ctx_); }
   HIP_CartesianIterator(const HIP_CartesianIterator&) = delete;

@@ -37,7 +37,7 @@ class GpuGrid:
         self.w = workload
         self.dtype = np.dtype(dtype)
         self.ctx = hip.Context(self.dtype, device)
-        self.ctx.set_operator(workload.recon, workload.limiter)
+        self.ctx.set_operator(workload.recon, workload.limiter, workload.viscous)
         for P in workload.patches:
             self.ctx.add_patch(P.ni, P.nj, P.nk, P.seek, patch_spacing(P, self.dtype), P.bc,
                                device_bc_par(P, self.dtype))

@@ -89,6 +89,13 @@ int drn_precision(const drn_ctx* ctx);
 
 /* The operator: AusmPlus<5, Upwind1|Upwind2<5, LIM>, PerfectGas> (ausmplus.h:27-109). */
 int drn_set_operator(drn_ctx* ctx, int recon, int limiter);
+/* on = 1 adds CompressibleViscFlux<5, PerfectGas> (compressibleviscflux.h:27-268)
+ * to every interior face flux after the Euler flux, as the applications' flux
+ * classes compose them (EaFlux::xField, drnumCylinder/main.h:127-163; JetFlux,
+ * drnumJetDemo/jetflux.h:100-139); the face functors are unchanged.  Every
+ * tiling choice then runs the z-shared cell kernel (27 or 28 by list order).
+ * Default 0. */
+int drn_set_viscous(drn_ctx* ctx, int on);
 
 /* ---- patches ------------------------------------------------------------ */
 /* Replaces PatchIterator::addPatch (patchiterator.h:124-128) + the lazy device

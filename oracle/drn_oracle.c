@@ -21,7 +21,9 @@
  *   intercoeffws.h, intercoeffpad.h, math/coordtransform*.{h,cpp},
  *   levelsetobject.cpp, spherelevelset.cpp, combilevelsetor.cpp,
  *   cpu_levelsetobjectbc.h, lsobccompressibleeulerop.h, levelsetobjectbc.cpp,
- *   applications/drnumJetDemo/jetflux.h.
+ *   applications/drnumJetDemo/jetflux.h, fluxes/compressibleviscflux.h,
+ *   cartesianpatch_common.h (getXGrad/getYGrad/getZGrad),
+ *   applications/drnumCylinder/main.h (EaFlux: Euler flux, then viscous flux).
  *
  * Scope restrictions (documented in DESIGN.md): patches are axis aligned
  * (base vectors (1,0,0),(0,1,0)); there are no split faces (the immersed
@@ -58,8 +60,9 @@ typedef struct O_GRID GRID;
 #define GAS_GAMMA ((REAL)1.4)
 #define GAS_CP ((REAL)1004.5)
 #define GAS_CV ((REAL)717.5)
-/* drnum.h:65,70 */
+/* drnum.h:65,68,70 */
 #define FR12 ((REAL)0.5)
+#define FR23 ((REAL)0.6666666)
 #define GLOBAL_EPS ((REAL)1e-10)
 
 /* ------------------------------------------------------------------------ */
@@ -133,6 +136,7 @@ struct O_GRID {
   int np;
   opatch* p;
   int recon, lim;
+  int visc; /* add CompressibleViscFlux to every interior face flux */
   int nthreads;
   int have_ls;
   int error;
@@ -413,6 +417,150 @@ static void get_var(const opatch* P, const REAL* f, size_t i, size_t j, size_t k
   for (int v = 0; v < 5; ++v) var[v] = f[v * P->N + c];
 }
 
+/* ------------------------------------------------------------------------ */
+/* viscous flux                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* getXGrad / getYGrad / getZGrad (cartesianpatch_common.h:188-300): central
+ * difference of the conserved variables at cell (i,j,k) along axis ax, one
+ * sided at the patch ends (D doubled per clamped side); the y gradient is 0
+ * when sizeJ() <= 2, the x and z gradients have no such guard. */
+static void cell_grad(const opatch* P, const REAL* f, int ax, size_t i, size_t j, size_t k, REAL* grad) {
+  const size_t n = ax == 0 ? (size_t)P->ni : (ax == 1 ? (size_t)P->nj : (size_t)P->nk);
+  if (ax == 1 && !(n > 2)) {
+    for (int v = 0; v < 5; ++v) grad[v] = 0;
+    return;
+  }
+  const REAL d = ax == 0 ? P->dx : (ax == 1 ? P->dy : P->dz);
+  REAL D = (REAL)0.5 / d;
+  const size_t c = ax == 0 ? i : (ax == 1 ? j : k);
+  size_t c1 = c - 1, c2 = c + 1;
+  if (c1 >= n) {
+    c1 = c;
+    D *= 2;
+  }
+  if (c2 >= n) {
+    c2 = c;
+    D *= 2;
+  }
+  REAL var[5];
+  get_var(P, f, ax == 0 ? c2 : i, ax == 1 ? c2 : j, ax == 2 ? c2 : k, grad);
+  get_var(P, f, ax == 0 ? c1 : i, ax == 1 ? c1 : j, ax == 2 ? c1 : k, var);
+  for (int v = 0; v < 5; ++v) {
+    grad[v] -= var[v];
+    grad[v] *= D;
+  }
+}
+
+/* PerfectGas::mu (perfectgas.h:100-111), Sutherland's law; the float build
+ * calls powf (DRNUM_SINGLE_PRECISION, drnum.h:43), the double build pow */
+REAL F(gas_mu)(const REAL* var) {
+  const REAL T0_suth = 300.55;
+  const REAL C_suth = 111.00;
+  const REAL mu0_suth = 1.781e-5;
+  REAL p, T, u, v, w;
+  cons_to_prim(var, &p, &T, &u, &v, &w);
+#if O_IS_FLOAT
+  return mu0_suth * powf(T / T0_suth, 1.5) * (T0_suth + C_suth) / (T + C_suth);
+#else
+  return mu0_suth * pow(T / T0_suth, 1.5) * (T0_suth + C_suth) / (T + C_suth);
+#endif
+}
+
+/* derivative of the velocity component c from conserved gradients g */
+static REAL dvel(const REAL* g, const REAL* var, REAL ir, int c) { return (g[c] - g[0] * var[c] * ir) * ir; }
+
+/* gamma1/R times the derivative of T (the reference's dT_d*) */
+static REAL dtemp(const REAL* g, const REAL* var, REAL ir, REAL gamma1) {
+  return gamma1 * ir * ir *
+         (g[4] * var[0] - g[0] * var[4] - var[1] * g[1] - var[2] * g[2] - var[3] * g[3] +
+          g[0] * ir * (var[1] * var[1] + var[2] * var[2] + var[3] * var[3]));
+}
+
+/* CompressibleViscFlux::xField/yField/zField (compressibleviscflux.h:33-268)
+ * for the face between the cell (i,j,k) - e_dir and (i,j,k); subtracts the
+ * viscous stresses and heat flux from flux[1..4] */
+void F(visc_face)(const REAL* var_l, const REAL* var_r, const REAL* gl_a, const REAL* gr_a, const REAL* gl_b,
+                  const REAL* gr_b, REAL D, int dir, REAL A, REAL* flux) {
+  REAL var[5], gn[5], ga[5], gb[5];
+  for (int v = 0; v < 5; ++v) var[v] = (REAL)0.5 * (var_l[v] + var_r[v]);
+  const REAL gamma = GAS_GAMMA;
+  const REAL gamma1 = gamma - (REAL)1.0;
+  const REAL ir = (REAL)1.0 / var[0];
+  const REAL mu = F(gas_mu)(var);
+  const REAL Pr = (REAL)0.7;
+  const REAL cp = GAS_CP;
+  const REAL R = GAS_R;
+  const REAL k_heat = mu * cp / Pr;
+  const REAL F2 = FR23 * mu, F3 = mu, F4 = k_heat / R;
+  for (int v = 0; v < 5; ++v) {
+    gn[v] = (var_r[v] - var_l[v]) * D;
+    ga[v] = (REAL)0.5 * (gl_a[v] + gr_a[v]);
+    gb[v] = (REAL)0.5 * (gl_b[v] + gr_b[v]);
+  }
+  /* the gradients along x, y, z: the normal one and the two tangential ones in
+   * the reference's order (x face: y, z; y face: x, z; z face: x, y) */
+  const REAL* gx = dir == 0 ? gn : ga;
+  const REAL* gy = dir == 0 ? ga : (dir == 1 ? gn : gb);
+  const REAL* gz = dir == 2 ? gn : gb;
+  if (dir == 0) {
+    const REAL du_dx = dvel(gx, var, ir, 1), dv_dx = dvel(gx, var, ir, 2), dw_dx = dvel(gx, var, ir, 3);
+    const REAL dT_dx = dtemp(gx, var, ir, gamma1);
+    const REAL du_dy = dvel(gy, var, ir, 1), dv_dy = dvel(gy, var, ir, 2);
+    const REAL du_dz = dvel(gz, var, ir, 1), dw_dz = dvel(gz, var, ir, 3);
+    const REAL tau_xx = F2 * ((REAL)2.0 * du_dx - dv_dy - dw_dz);
+    const REAL tau_xy = F3 * (du_dy + dv_dx);
+    const REAL tau_xz = F3 * (du_dz + dw_dx);
+    const REAL q = F4 * dT_dx;
+    flux[1] -= tau_xx * A;
+    flux[2] -= tau_xy * A;
+    flux[3] -= tau_xz * A;
+    flux[4] -= ((tau_xx * var[1] + tau_xy * var[2] + tau_xz * var[3]) * ir + q) * A;
+  } else if (dir == 1) {
+    const REAL du_dx = dvel(gx, var, ir, 1), dv_dx = dvel(gx, var, ir, 2);
+    const REAL du_dy = dvel(gy, var, ir, 1), dv_dy = dvel(gy, var, ir, 2), dw_dy = dvel(gy, var, ir, 3);
+    const REAL dT_dy = dtemp(gy, var, ir, gamma1);
+    const REAL dv_dz = dvel(gz, var, ir, 2), dw_dz = dvel(gz, var, ir, 3);
+    const REAL tau_yy = F2 * (-du_dx + (REAL)2.0 * dv_dy - dw_dz);
+    const REAL tau_xy = F3 * (du_dy + dv_dx);
+    const REAL tau_yz = F3 * (dv_dz + dw_dy);
+    const REAL q = F4 * dT_dy;
+    flux[1] -= tau_xy * A;
+    flux[2] -= tau_yy * A;
+    flux[3] -= tau_yz * A;
+    flux[4] -= ((tau_xy * var[1] + tau_yy * var[2] + tau_yz * var[3]) * ir + q) * A;
+  } else {
+    const REAL du_dx = dvel(gx, var, ir, 1), dw_dx = dvel(gx, var, ir, 3);
+    const REAL dv_dy = dvel(gy, var, ir, 2), dw_dy = dvel(gy, var, ir, 3);
+    const REAL du_dz = dvel(gz, var, ir, 1), dv_dz = dvel(gz, var, ir, 2), dw_dz = dvel(gz, var, ir, 3);
+    const REAL dT_dz = dtemp(gz, var, ir, gamma1);
+    const REAL tau_zz = F2 * (-du_dx - dv_dy + (REAL)2.0 * dw_dz);
+    const REAL tau_xz = F3 * (du_dz + dw_dx);
+    const REAL tau_yz = F3 * (dv_dz + dw_dy);
+    const REAL q = F4 * dT_dz;
+    flux[1] -= tau_xz * A;
+    flux[2] -= tau_yz * A;
+    flux[3] -= tau_zz * A;
+    flux[4] -= ((tau_xz * var[1] + tau_yz * var[2] + tau_zz * var[3]) * ir + q) * A;
+  }
+}
+
+/* the viscous part of the face between (i,j,k) - e_dir and (i,j,k) on field f */
+static void visc_call(const opatch* P, const REAL* f, int dir, size_t i, size_t j, size_t k, REAL A, REAL* flux) {
+  const size_t il = dir == 0 ? i - 1 : i, jl = dir == 1 ? j - 1 : j, kl = dir == 2 ? k - 1 : k;
+  REAL vl[5], vr[5], gla[5], gra[5], glb[5], grb[5];
+  get_var(P, f, il, jl, kl, vl);
+  get_var(P, f, i, j, k, vr);
+  const int ta = dir == 0 ? 1 : 0, tb = dir == 2 ? 1 : 2;
+  /* compressibleviscflux.h:69-72 order: left a, left b, right a, right b */
+  cell_grad(P, f, ta, il, jl, kl, gla);
+  cell_grad(P, f, tb, il, jl, kl, glb);
+  cell_grad(P, f, ta, i, j, k, gra);
+  cell_grad(P, f, tb, i, j, k, grb);
+  const REAL d = dir == 0 ? P->dx : (dir == 1 ? P->dy : P->dz);
+  F(visc_face)(vl, vr, gla, gra, glb, grb, (REAL)1.0 / d, dir, A, flux);
+}
+
 /* one wall face call; (ta, tb) are the accumulated tangential coordinates of
  * the wall loops in cartesianiterator.h:190-308, (la, lb) the tangential indices */
 static void wall_call(const opatch* P, const REAL* f0, int face, size_t i, size_t j, size_t k, REAL ta, REAL tb,
@@ -458,6 +606,7 @@ static void patch_residual(const GRID* g, const opatch* P, REAL* res) {
           project(P, f0, g->recon, g->lim, i - 1, j, k, i, j, k, vl);
           project(P, f0, g->recon, g->lim, i, j, k, i - 1, j, k, vr);
           F(ausm_flux)(vl, vr, 0, Ax, flux);
+          if (g->visc) visc_call(P, f0, 0, i, j, k, Ax, flux);
           size_t cm = IDX(P, i - 1, j, k);
           for (int v = 0; v < 5; ++v) {
             res[v * N + cm] -= flux[v];
@@ -469,6 +618,7 @@ static void patch_residual(const GRID* g, const opatch* P, REAL* res) {
           project(P, f0, g->recon, g->lim, i, j - 1, k, i, j, k, vl);
           project(P, f0, g->recon, g->lim, i, j, k, i, j - 1, k, vr);
           F(ausm_flux)(vl, vr, 1, Ay, flux);
+          if (g->visc) visc_call(P, f0, 1, i, j, k, Ay, flux);
           size_t cm = IDX(P, i, j - 1, k);
           for (int v = 0; v < 5; ++v) {
             res[v * N + cm] -= flux[v];
@@ -480,6 +630,7 @@ static void patch_residual(const GRID* g, const opatch* P, REAL* res) {
           project(P, f0, g->recon, g->lim, i, j, k - 1, i, j, k, vl);
           project(P, f0, g->recon, g->lim, i, j, k, i, j, k - 1, vr);
           F(ausm_flux)(vl, vr, 2, Az, flux);
+          if (g->visc) visc_call(P, f0, 2, i, j, k, Az, flux);
           size_t cm = IDX(P, i, j, k - 1);
           for (int v = 0; v < 5; ++v) {
             res[v * N + cm] -= flux[v];
@@ -679,6 +830,7 @@ REAL F(grid_dx)(const GRID* g, int p, int axis) {
   return axis == 0 ? g->p[p].dx : (axis == 1 ? g->p[p].dy : g->p[p].dz);
 }
 void F(grid_set_threads)(GRID* g, int n) { g->nthreads = n < 1 ? 1 : n; }
+void F(grid_set_viscous)(GRID* g, int on) { g->visc = on != 0; }
 
 /* patchgrid.cpp:645-651 */
 void F(grid_set_const)(GRID* g, int field, const REAL* var) {

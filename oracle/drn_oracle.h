@@ -62,6 +62,13 @@ typedef struct o64_grid o64_grid;
   REAL PFX##limiter(int lim, REAL d1, REAL d2);                                                  \
   /* fluxes/ausmplus.h:39-106 on two reconstructed states, dir 0/1/2 = x/y/z */                 \
   void PFX##ausm_flux(const REAL* var_l, const REAL* var_r, int dir, REAL A, REAL* flux);        \
+  /* perfectgas.h:100-111 Sutherland viscosity of a conserved state */                         \
+  REAL PFX##gas_mu(const REAL* var);                                                            \
+  /* fluxes/compressibleviscflux.h:33-268: face between var_l and var_r along dir; gl_a/gr_a   \
+     and gl_b/gr_b are the cell gradients along the first and second tangential axis of the    \
+     left and right cell, D = 1/delta along dir; subtracts from flux[1..4] */                  \
+  void PFX##visc_face(const REAL* var_l, const REAL* var_r, const REAL* gl_a, const REAL* gr_a,  \
+                      const REAL* gl_b, const REAL* gr_b, REAL D, int dir, REAL A, REAL* flux);  \
   /* grid object */                                                                             \
   GRID* PFX##grid_create(int n_patches, const o_patch_desc* pd, int recon, int lim);            \
   void  PFX##grid_destroy(GRID* g);                                                             \
@@ -103,6 +110,9 @@ typedef struct o64_grid o64_grid;
   /* rungekutta.cpp:27-42 (+ level-set fill as post operation when set up) */                   \
   void  PFX##grid_rk_step(GRID* g, REAL dt, int n_alpha, const REAL* alpha);                    \
   void  PFX##grid_set_threads(GRID* g, int n);                                                  \
+  /* EaFlux composition (drnumCylinder/main.h:127-163): the viscous flux is added to every     \
+     interior face flux after the Euler flux; off by default */                                 \
+  void  PFX##grid_set_viscous(GRID* g, int on);                                                 \
   /* FP64 sums over core cells: mass, x/y/z momentum, energy */                                 \
   void  PFX##grid_core_sums(const GRID* g, int field, double* out);
 

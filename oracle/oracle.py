@@ -48,6 +48,9 @@ def lib():
                 "prim_to_cons": (None, [rt, rt, rt, rt, rt, rp]),
                 "limiter": (rt, [C.c_int, rt, rt]),
                 "ausm_flux": (None, [rp, rp, C.c_int, rt, rp]),
+                "gas_mu": (rt, [rp]),
+                "visc_face": (None, [rp, rp, rp, rp, rp, rp, rt, C.c_int, rt, rp]),
+                "grid_set_viscous": (None, [C.c_void_p, C.c_int]),
                 "grid_create": (C.c_void_p, [C.c_int, C.POINTER(PatchDesc), C.c_int, C.c_int]),
                 "grid_destroy": (None, [C.c_void_p]),
                 "grid_patch_cells": (C.c_size_t, [C.c_void_p, C.c_int]),
@@ -137,6 +140,23 @@ def ausm_flux(var_l, var_r, direction, area, dtype):
     return out
 
 
+def gas_mu(var, dtype):
+    """PerfectGas::mu (perfectgas.h:100-111) of a conserved state."""
+    v = np.ascontiguousarray(var, dtype=dtype)
+    return fn(dtype, "gas_mu")(v.ctypes.data_as(C.POINTER(_ctype(dtype))))
+
+
+def visc_face(var_l, var_r, gl_a, gr_a, gl_b, gr_b, D, direction, area, dtype, flux=None):
+    """CompressibleViscFlux::{x,y,z}Field (compressibleviscflux.h:33-268) on given
+    states and cell gradients; returns flux - viscous terms (flux defaults to 0)."""
+    rt = _ctype(dtype)
+    p = C.POINTER(rt)
+    arrs = [np.ascontiguousarray(a, dtype=dtype) for a in (var_l, var_r, gl_a, gr_a, gl_b, gr_b)]
+    out = np.zeros(5, dtype=dtype) if flux is None else np.array(flux, dtype=dtype)
+    fn(dtype, "visc_face")(*[a.ctypes.data_as(p) for a in arrs], rt(D), direction, rt(area), out.ctypes.data_as(p))
+    return out
+
+
 class OracleGrid:
     """A grid of patches held by the oracle in one precision (np.float32 or np.float64)."""
 
@@ -148,6 +168,8 @@ class OracleGrid:
         self.h = fn(dtype, "grid_create")(len(workload.patches), self._d, workload.recon, workload.limiter)
         self.np = len(workload.patches)
         fn(dtype, "grid_set_threads")(self.h, threads)
+        if getattr(workload, "viscous", False):
+            fn(dtype, "grid_set_viscous")(self.h, 1)
 
     def __del__(self):
         try:

@@ -138,3 +138,15 @@ def test_dropin_iterator_feeder(gpu, tmp_path):
                         "codes=fx,fy,fz,far,far,far,far,far,far,0"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 5, r.stdout + r.stderr
     assert codes in r.stderr
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_dropin_viscous_operator(gpu, tmp_path, dtype):
+    """HIP_CartesianIterator<5, EulerViscFlux<AusmPlus<...>, CompressibleViscFlux<5,
+    PerfectGas>>> (the EaFlux composition) against the oracle with the viscous flux."""
+    w = configs.viscous_shear(core=8, blocks=(2, 1, 1), periodic=False)
+    og = oracle_after(w, dtype, 4)
+    st = run_app(tmp_path, w, dtype, 4, "hip", extra=["visc"])
+    err = rel_linf(st, og)
+    print("drop-in viscous", np.dtype(dtype).name, err, flush=True)
+    assert err <= TOL[dtype]
