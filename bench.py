@@ -145,6 +145,9 @@ def _workloads():
                                               "cells, Mach 0.3 farfield, slip level-set fill every stage"),
         "kh": (configs.kelvin_helmholtz, "periodic Kelvin-Helmholtz box, 512^3 core as 8x8x8 patches of 64^3 + 2 "
                                          "seek layers on every face"),
+        "shear": (lambda: configs.viscous_shear(core=64, blocks=(4, 4, 4)),
+                  "viscous periodic shear layer, 256^3 core of 1 um cells at 1 bar / 300 K as 4x4x4 patches of 64^3 "
+                  "+ 2 seek layers on every face; AUSM+ MUSCL van Albada + CompressibleViscFlux, 3-stage RK"),
     }
 
 
