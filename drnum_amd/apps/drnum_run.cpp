@@ -7,7 +7,7 @@
 // coefficients, run the time steps, copy the result back to the host.
 //
 // usage: drnum_run <grid file> <case file> <out file> <steps> <dt> <mode> [f64] [levelset] [visc]
-//                  [restart=<file.dnd>] [dnd=<base name>] [codes=<code string>]
+//                  [restart=<file.dnd>] [dnd=<base name>] [codes=<code string>] [vtk=<base name>]
 //   case file (binary, little endian), per patch in grid order:
 //     int32 bc[6], float64 par[6][16], then 5*N initial values in `real`
 //     (variable-major, the layout of field 0 in Patch::m_Data);
@@ -26,6 +26,8 @@
 //              words) and an IteratorFeeder hands it the patch groups of equal
 //              code string; a group it does not match stops the run (exit 5).
 //              Without codes=, every patch is added to the iterator.
+//   vtk=B:     at the end, B_<steps, 6 digits>.vtm and B_<steps>/ with one .vts
+//              per patch (PatchGrid::writeToVtk with CompressibleVariables)
 // Prints one line per phase and exits non-zero on any error.
 #include <cstdio>
 #include <cstdlib>
@@ -43,10 +45,20 @@ typedef AusmPlus<5, Upwind2<5, VanAlbada>, PerfectGas> euler_t;
 // drnumCylinder/main.h:127-163)
 typedef EulerViscFlux<euler_t, CompressibleViscFlux<5, PerfectGas>> visc_t;
 
+struct Options {
+  std::string grid_file, case_file, out_file, mode, restart, dnd, codes, vtk;
+  int steps = 0;
+  double dt = 0;
+  bool levelset = false;
+};
+
 template <typename real, class OP>
-static int run(const std::string& grid_file, const std::string& case_file, const std::string& out_file, int steps,
-               double dt, const std::string& mode, bool levelset, const std::string& restart, const std::string& dnd,
-               const std::string& codes) {
+static int run(const Options& o) {
+  const std::string &grid_file = o.grid_file, &case_file = o.case_file, &out_file = o.out_file, &mode = o.mode;
+  const std::string &restart = o.restart, &dnd = o.dnd, &codes = o.codes, &vtk = o.vtk;
+  const int steps = o.steps;
+  const double dt = o.dt;
+  const bool levelset = o.levelset;
   PatchGrid<double> grid;
   std::string err;
   if (grid.readGrid(grid_file, 1.0, &err)) {
@@ -154,6 +166,11 @@ static int run(const std::string& grid_file, const std::string& case_file, const
     writeData<real>(pp, 0, dnd, real(t0 + steps * dt), steps);
     printf("wrote restart file %s_%06d.dnd\n", dnd.c_str(), steps);
   }
+  if (!vtk.empty()) {
+    CompressibleVariables<PerfectGas> vars;
+    iterator.writeToVtk(0, vtk, vars, steps);
+    printf("wrote %s_%06d.vtm\n", vtk.c_str(), steps);
+  }
   return out ? 0 : 3;
 }
 
@@ -161,28 +178,32 @@ int main(int argc, char** argv) {
   if (argc < 7) {
     fprintf(stderr,
             "usage: %s <grid> <case> <out> <steps> <dt> <rk|hip|graph> [f64] [levelset] [visc] [restart=F] [dnd=B] "
-            "[codes=S]\n",
+            "[codes=S] [vtk=B]\n",
             argv[0]);
     return 2;
   }
-  bool f64 = false, levelset = false, visc = false;
-  std::string restart, dnd, codes;
+  Options o;
+  o.grid_file = argv[1];
+  o.case_file = argv[2];
+  o.out_file = argv[3];
+  o.steps = atoi(argv[4]);
+  o.dt = atof(argv[5]);
+  o.mode = argv[6];
+  bool f64 = false, visc = false;
   for (int a = 7; a < argc; ++a) {
     if (!strcmp(argv[a], "f64")) f64 = true;
-    if (!strcmp(argv[a], "levelset")) levelset = true;
+    if (!strcmp(argv[a], "levelset")) o.levelset = true;
     if (!strcmp(argv[a], "visc")) visc = true;
-    if (!strncmp(argv[a], "restart=", 8)) restart = argv[a] + 8;
-    if (!strncmp(argv[a], "dnd=", 4)) dnd = argv[a] + 4;
-    if (!strncmp(argv[a], "codes=", 6)) codes = argv[a] + 6;
+    if (!strncmp(argv[a], "restart=", 8)) o.restart = argv[a] + 8;
+    if (!strncmp(argv[a], "dnd=", 4)) o.dnd = argv[a] + 4;
+    if (!strncmp(argv[a], "codes=", 6)) o.codes = argv[a] + 6;
+    if (!strncmp(argv[a], "vtk=", 4)) o.vtk = argv[a] + 4;
   }
   try {
-    const int steps = atoi(argv[4]);
-    const double dt = atof(argv[5]);
-    if (f64 && visc)
-      return run<double, visc_t>(argv[1], argv[2], argv[3], steps, dt, argv[6], levelset, restart, dnd, codes);
-    if (f64) return run<double, euler_t>(argv[1], argv[2], argv[3], steps, dt, argv[6], levelset, restart, dnd, codes);
-    if (visc) return run<float, visc_t>(argv[1], argv[2], argv[3], steps, dt, argv[6], levelset, restart, dnd, codes);
-    return run<float, euler_t>(argv[1], argv[2], argv[3], steps, dt, argv[6], levelset, restart, dnd, codes);
+    if (f64 && visc) return run<double, visc_t>(o);
+    if (f64) return run<double, euler_t>(o);
+    if (visc) return run<float, visc_t>(o);
+    return run<float, euler_t>(o);
   } catch (const DeviceError& e) {
     fprintf(stderr, "device error: %s\n", e.what());
     return 4;

@@ -47,6 +47,10 @@ _SIGS = {
     "drnh_grid_num_patch_groups": (C.c_int, [C.c_void_p]),
     "drnh_grid_patch_group": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_char_p,
                                         C.c_int]),
+    "drnh_vtk_write_multiblock": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_char_p, C.c_int]),
+    "drnh_vtk_write_patch": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _DP, _DP, C.c_int,
+                                       C.POINTER(C.c_char_p), C.POINTER(C.POINTER(C.c_float)), C.c_int,
+                                       C.POINTER(C.c_char_p), C.POINTER(C.POINTER(C.c_float))]),
     "drnh_grid_to_device": (C.c_int, [C.c_void_p, C.c_void_p]),
     "drnh_grid_upload_exchange": (C.c_int, [C.c_void_p, C.c_void_p]),
     "drnh_grid_upload_levelset": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -102,6 +106,34 @@ def codestring_compare(a: str, b: str):
     m = _string_out(lambda buf, n: lib().drnh_codestring_compare(a.encode(), b.encode(), flags, buf, n),
                     "drnh_codestring_compare")
     return bool(flags[0]), bool(flags[1]), bool(flags[2]), m
+
+
+def vtk_write_multiblock(base: str, count: int, n_patches: int) -> str:
+    """The .vtm index of PatchGrid::writeToVtk (patchgrid.cpp:673-695); returns
+    the block directory (base with the count suffix)."""
+    return _string_out(lambda b, n: lib().drnh_vtk_write_multiblock(base.encode(), count, n_patches, b, n),
+                       "drnh_vtk_write_multiblock", cap=len(base) + 64)
+
+
+def vtk_write_patch(path: str, n3, seek6, origin3, delta3, scalars: dict, vectors: dict):
+    """One patch's core cells as a .vts block (CartesianPatch::createVtkDataSet,
+    cartesianpatch.cpp:1661-1730): scalars name -> (N,), vectors name -> (3, N)
+    in the patch layout."""
+    n = np.ascontiguousarray(n3, dtype=np.int32)
+    sk = np.ascontiguousarray(seek6, dtype=np.int32)
+    o = np.ascontiguousarray(origin3, dtype=np.float64)
+    d = np.ascontiguousarray(delta3, dtype=np.float64)
+    fp = C.POINTER(C.c_float)
+    sc = [np.ascontiguousarray(v, dtype=np.float32).reshape(-1) for v in scalars.values()]
+    vc = [np.ascontiguousarray(v, dtype=np.float32).reshape(-1) for v in vectors.values()]
+    sn = (C.c_char_p * max(len(sc), 1))(*[k.encode() for k in scalars])
+    vn = (C.c_char_p * max(len(vc), 1))(*[k.encode() for k in vectors])
+    sp = (fp * max(len(sc), 1))(*[a.ctypes.data_as(fp) for a in sc])
+    vp = (fp * max(len(vc), 1))(*[a.ctypes.data_as(fp) for a in vc])
+    rc = lib().drnh_vtk_write_patch(path.encode(), _ptr(n, C.c_int32), _ptr(sk, C.c_int32), _ptr(o, C.c_double),
+                                    _ptr(d, C.c_double), len(sc), sn, sp, len(vc), vn, vp)
+    if rc:
+        raise RuntimeError("drnh_vtk_write_patch failed (%d)" % rc)
 
 
 class HostGrid:

@@ -184,6 +184,33 @@ real readData(std::vector<HostPatch<real>*>& patches, size_t i_field, const std:
   return (real)t;
 }
 
+// CompressibleVariables<TGas> (compressiblevariables.h:29-105): the
+// post-processing quantities Ma, p, T, rho, S and the vector U, with the
+// reference pressure and temperature of the entropy S (1e5 and 300 by default).
+// The values themselves are computed on the device by drn_post_variables.
+template <class TGas>
+class CompressibleVariables {
+ public:
+  int numScalars() const { return 5; }
+  int numVectors() const { return 1; }
+  void setReferenceTemperature(double T) { ref_T_ = T; }
+  void setReferencePressure(double p) { ref_p_ = p; }
+  double referenceTemperature() const { return ref_T_; }
+  double referencePressure() const { return ref_p_; }
+  std::string getScalarName(int i) const {
+    static const char* const names[5] = {"Ma", "p", "T", "rho", "S"};
+    if (i < 0 || i >= 5) throw std::out_of_range("CompressibleVariables: scalar index");
+    return names[i];
+  }
+  std::string getVectorName(int i) const {
+    if (i != 0) throw std::out_of_range("CompressibleVariables: vector index");
+    return "U";
+  }
+
+ private:
+  double ref_p_ = 1e5, ref_T_ = 300;
+};
+
 // ---------------------------------------------------------------------------
 // iterators
 // ---------------------------------------------------------------------------
@@ -342,6 +369,57 @@ class HIP_CartesianIterator : public PatchIterator<real> {
   }
   void sync() { check(drn_sync(ctx_), "drn_sync"); }
   int launchForm() const { return launch_; }
+
+  // PatchGrid::writeToVtk (patchgrid.cpp:673-695): the donor data of field 0 are
+  // exchanged first (accessAllDonorData(0)), then the core cells of every patch
+  // of this iterator go to <file_name>[_<count, 6 digits>].vtm as block p (the
+  // iterator's patch order), with the quantities of `vars` evaluated on the
+  // device from field i_field (drn_post_variables)
+  template <class TGas>
+  void writeToVtk(size_t i_field, const std::string& file_name, const CompressibleVariables<TGas>& vars,
+                  int count = -1) {
+    setup();
+    check(drn_exchange(ctx_, 0), "drn_exchange");
+    std::vector<char> dir(file_name.size() + 64);
+    if (drnh_vtk_write_multiblock(file_name.c_str(), count, (int)this->patches_.size(), dir.data(),
+                                  (int)dir.size()) < 0)
+      throw DeviceError("writeToVtk: cannot write " + file_name + ".vtm");
+    const std::string base(dir.data());
+    const size_t cut = base.find_last_of('/');
+    const std::string leaf = cut == std::string::npos ? base : base.substr(cut + 1);
+    std::vector<std::string> names;
+    for (int i = 0; i < vars.numScalars(); ++i) names.push_back(vars.getScalarName(i));
+    const std::string vname = vars.getVectorName(0);
+    for (size_t p = 0; p < this->patches_.size(); ++p) {
+      const HostPatch<real>& hp = *this->patches_[p];
+      const CartesianPatch<double>& P = hp.geometry();
+      const size_t N = hp.variableSize();
+      std::vector<real> q(8 * N);
+      check(drn_post_variables(ctx_, (int)i_field, (int)p, 0xFF, vars.referencePressure(),
+                               vars.referenceTemperature(), q.data()),
+            "drn_post_variables");
+      const std::vector<float> qf(q.begin(), q.end());
+      const float* sc[5];
+      const char* sn[5];
+      for (int i = 0; i < 5; ++i) {
+        sc[i] = qf.data() + (size_t)i * N;
+        sn[i] = names[(size_t)i].c_str();
+      }
+      const float* vec[1] = {qf.data() + 5 * N};
+      const char* vn[1] = {vname.c_str()};
+      const int32_t n3[3] = {P.ni(), P.nj(), P.nk()};
+      int32_t seek6[6];
+      double o[3], d[3];
+      for (int f = 0; f < 6; ++f) seek6[f] = P.seek(f);
+      for (int a = 0; a < 3; ++a) {
+        o[a] = P.origin(a);
+        d[a] = P.delta(a);
+      }
+      const std::string path = base + "/" + leaf + "_" + std::to_string(p) + ".vts";
+      if (drnh_vtk_write_patch(path.c_str(), n3, seek6, o, d, 5, sn, sc, 1, vn, vec))
+        throw DeviceError("writeToVtk: cannot write " + path);
+    }
+  }
   const OP& op() const { return op_; }
 
   void check(int rc, const char* what) const {

@@ -106,6 +106,25 @@ int drnh_dnd_read(const char* path, int precision, double* time, int n_patches, 
 /* PatchGrid::computeMinChLength (patchgrid.cpp:659-671) */
 double drnh_grid_min_ch_length(const drnh_grid* g);
 
+/* ---- VTK output -----------------------------------------------------------
+ * PatchGrid::writeToVtk (patchgrid.cpp:673-695) with CartesianPatch::
+ * createVtkDataSet (cartesianpatch.cpp:1661-1730), as VTK XML files written
+ * without the VTK library: <base>[_<count, 6 digits>].vtm (a vtkMultiBlockDataSet
+ * of n_patches blocks) and the block files <base>/<leaf>_<p>.vts. */
+/* Writes the .vtm index (count < 0: no count suffix), creates the block
+ * directory and returns its path in block_dir (the string length, -2 on I/O
+ * error or a too small cap). */
+int drnh_vtk_write_multiblock(const char* base_path, int count, int n_patches, char* block_dir, int cap);
+/* One patch as a vtkStructuredGrid of its core cells (seek layers left out):
+ * node points origin + (i, j, k) * delta (float32, i fastest), then the cell
+ * arrays (float32, i fastest): n_scalars arrays of ni*nj*nk values and
+ * n_vectors arrays of 3 consecutive ni*nj*nk blocks (x, y, z components), all
+ * in the patch layout cell = i*nj*nk + j*nk + k. */
+int drnh_vtk_write_patch(const char* path, const int32_t* n3, const int32_t* seek6, const double* origin3,
+                         const double* delta3, int n_scalars, const char* const* scalar_names,
+                         const float* const* scalars, int n_vectors, const char* const* vector_names,
+                         const float* const* vectors);
+
 /* Add every patch of the grid to a libdrnhip context (in grid order, with the
  * face functors set by drnh_grid_set_faces) and lay the context out. */
 int drnh_grid_to_device(drnh_grid* g, drn_ctx* ctx);

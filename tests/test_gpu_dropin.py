@@ -150,3 +150,44 @@ def test_dropin_viscous_operator(gpu, tmp_path, dtype):
     err = rel_linf(st, og)
     print("drop-in viscous", np.dtype(dtype).name, err, flush=True)
     assert err <= TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_dropin_vtk_output(gpu, tmp_path, dtype):
+    """HIP_CartesianIterator::writeToVtk (PatchGrid::writeToVtk, patchgrid.cpp:673-695):
+    one block per patch with the core cells; rho is the state's density as
+    float32, the other quantities follow CompressibleVariables
+    (compressiblevariables.h:81-105) of the same state, and the node points sit
+    at origin + index * delta of the patch's core."""
+    from vtkread import read_vtm, read_vts
+
+    w = configs.jet(core=8, blocks=2)
+    st = run_app(tmp_path, w, dtype, 3, "hip", extra=["vtk=%s" % (tmp_path / "flow")])
+    blocks = read_vtm(str(tmp_path / "flow_000003.vtm"))
+    assert [b[0] for b in blocks] == list(range(len(w.patches)))
+    R, CV, CP, G = 287.2, 717.5, 1004.5, 1.4
+    for p, P in enumerate(w.patches):
+        assert blocks[p][1] == "flow_000003/flow_000003_%d.vts" % p
+        r = read_vts(str(tmp_path / blocks[p][1]))
+        s = P.seek
+        core = np.asarray(st[p], dtype=np.float64).reshape(5, P.ni, P.nj, P.nk)[
+            :, s[0]:P.ni - s[1], s[2]:P.nj - s[3], s[4]:P.nk - s[5]]
+        core = np.transpose(core, (0, 3, 2, 1))            # VTK order: k, j, i
+        rho = core[0]
+        u, v, ww = core[1] / rho, core[2] / rho, core[3] / rho
+        T = (core[4] / rho - 0.5 * (u * u + v * v + ww * ww)) / CV
+        pr = rho * R * T
+        np.testing.assert_array_equal(r["cells"]["rho"], rho.astype(np.float32))
+        np.testing.assert_allclose(r["cells"]["p"], pr, rtol=2e-5)
+        np.testing.assert_allclose(r["cells"]["T"], T, rtol=2e-5)
+        np.testing.assert_allclose(r["cells"]["Ma"], np.sqrt((u * u + v * v + ww * ww) / (G * R * T)), rtol=2e-4,
+                                   atol=1e-5)
+        S = CP * np.log(T / 300.0) - R * np.log(pr / 1e5)
+        np.testing.assert_allclose(r["cells"]["S"], S, rtol=2e-5)
+        np.testing.assert_allclose(r["cells"]["U"], np.stack([u, v, ww], axis=-1), rtol=2e-4, atol=1e-5)
+        h = [P.length[a] / n for a, n in enumerate((P.ni, P.nj, P.nk))]
+        first = [P.origin[a] + s[2 * a] * h[a] for a in range(3)]
+        np.testing.assert_allclose(r["points"][0, 0, 0], first, rtol=1e-6, atol=1e-7)
+        np.testing.assert_allclose(r["points"][-1, -1, -1],
+                                   [P.origin[a] + (n - s[2 * a + 1]) * h[a]
+                                    for a, n in enumerate((P.ni, P.nj, P.nk))], rtol=1e-6, atol=1e-7)
