@@ -157,9 +157,66 @@ __device__ __forceinline__ T p5f(T M, T s) {
   return m2f(M, s) * ((T(2) * s - M) - T(3) * s * M * m2f(M, -s));
 }
 
+// Fast form of the face flux below, equal to it in exact arithmetic:
+//  * the primitive state takes p and a from the specific internal energy e
+//    directly, p = rho e R/cv and a = sqrt(gamma e R/cv) (T = e/cv is not
+//    formed), and H = E/rho + e R/cv;
+//  * M+ and M- enter as rho_l (M + |M|) + rho_r (M - |M|) with the 1/2 folded
+//    into the face area;
+//  * the advected part 1/2 F0 (phi_l + phi_r) - 1/2 |F0| (phi_r - phi_l) is
+//    written in its upwind form max(F0, 0) phi_l + min(F0, 0) phi_r.
+template <typename T>
+struct PrimFast {
+  T r, u, v, w, p, a, H;
+};
+template <typename T>
+__device__ __forceinline__ PrimFast<T> lr_vars_fast(const T* var) {
+  constexpr T k_p = Gas<T>::R / Gas<T>::CV;                     // p = rho * k_p * e
+  constexpr T k_a = Gas<T>::GAMMA * Gas<T>::R / Gas<T>::CV;     // a^2 = k_a * e
+  PrimFast<T> s;
+  s.r = var[0];
+  const T ir = drcp<T, true>(s.r);
+  s.u = var[1] * ir;
+  s.v = var[2] * ir;
+  s.w = var[3] * ir;
+  const T E = var[4] * ir;
+  const T e = E - T(0.5) * (s.u * s.u + s.v * s.v + s.w * s.w);
+  s.p = (s.r * e) * k_p;
+  if constexpr (sizeof(T) == 4)
+    s.a = __builtin_amdgcn_sqrtf(k_a * e);  // v_sqrt_f32, 1 ulp
+  else
+    s.a = dsqrt(k_a * e);
+  s.H = E + e * k_p;
+  return s;
+}
+
+template <typename T, int DIR>
+__device__ __forceinline__ void ausm_flux_fast(const T* vl, const T* vr, T A, T* f) {
+  const PrimFast<T> L = lr_vars_fast(vl), R = lr_vars_fast(vr);
+  const T unl = DIR == 0 ? L.u : (DIR == 1 ? L.v : L.w);
+  const T unr = DIR == 0 ? R.u : (DIR == 1 ? R.v : R.w);
+  const T a = T(0.5) * (L.a + R.a);
+  const T ia = drcp<T, true>(a);
+  const T Ml = unl * ia, Mr = unr * ia;
+  const T M = m4f(Ml, T(1)) + m4f(Mr, T(-1));
+  const T aM = dabs(M);
+  const T p = p5f<T, true>(Ml, T(1)) * L.p + p5f<T, true>(Mr, T(-1)) * R.p;
+  const T f0 = (a * (T(0.5) * A)) * (L.r * (M + aM) + R.r * (M - aM));
+  const T fp = f0 > T(0) ? f0 : T(0), fm = f0 < T(0) ? f0 : T(0);
+  f[0] += f0;
+  f[1] += fp * L.u + fm * R.u + (DIR == 0 ? A * p : T(0));
+  f[2] += fp * L.v + fm * R.v + (DIR == 1 ? A * p : T(0));
+  f[3] += fp * L.w + fm * R.w + (DIR == 2 ? A * p : T(0));
+  f[4] += fp * L.H + fm * R.H;
+}
+
 // ausmplus.h:39-106 for the normal direction DIR, flux accumulated into f[5]
 template <typename T, bool FAST, int DIR>
 __device__ __forceinline__ void ausm_flux(const T* vl, const T* vr, T A, T* f) {
+  if constexpr (FAST) {
+    ausm_flux_fast<T, DIR>(vl, vr, A, f);
+    return;
+  }
   Prim<T> L = lr_vars<T, FAST>(vl), R = lr_vars<T, FAST>(vr);
   const T unl = DIR == 0 ? L.u : (DIR == 1 ? L.v : L.w);
   const T unr = DIR == 0 ? R.u : (DIR == 1 ? R.v : R.w);
