@@ -14,6 +14,10 @@ is one RK step over all patches.
   roofline    stage kernel: algorithmic bytes per launch (10 or 15 values of
               4 B per stored cell: stage 1 reads U^n only) / average launch time
               from HIP events on the context stream over the timed region.
+  valu_roofline the same kernel against its compute ceiling: VALU wave
+              instructions per launch (committed rocprofv3 summary) / CUs /
+              launch time, against the FP32 VALU issue rate measured by
+              drnum_amd/lib/valu_probe after the timed region.
   cpu_baseline the oracle (C restatement of the reference iterator, OpenMP over
               patches) on this host, one RK step of the same workload.
 
@@ -24,6 +28,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -174,6 +179,42 @@ def measured_traffic(tiling):
     return None
 
 
+VALU_TABLE = os.path.join(ROOT, "profiles", "stage_valu_jet4x64_fp32.md")
+VALU_PROBE = os.path.join(ROOT, "drnum_amd", "lib", "valu_probe")
+
+
+def measured_valu_instructions(tiling):
+    """SQ_INSTS_VALU (wave instructions) per launch of the FP32 stage kernel for a
+    work-list choice, from the committed rocprofv3 summary of this workload."""
+    try:
+        lines = open(VALU_TABLE).read().splitlines()
+    except OSError:
+        return None
+    for ln in lines:
+        cols = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if len(cols) == 7 and cols[0].isdigit() and int(cols[0]) == tiling and cols[1].startswith("k_stage"):
+            try:
+                return float(cols[2].replace(",", ""))
+            except ValueError:
+                return None
+    return None
+
+
+def valu_ceiling():
+    """The device's FP32 VALU issue rate (wave instructions per CU per ns) from
+    drnum_amd/lib/valu_probe, run as a child process after the measurement."""
+    if not os.path.exists(VALU_PROBE):
+        return None
+    try:
+        r = subprocess.run([VALU_PROBE], capture_output=True, text=True, timeout=120)
+    except (OSError, subprocess.TimeoutExpired):
+        return None
+    if r.returncode != 0:
+        log("valu_probe failed:", r.stderr.strip())
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def ctx_vstride(ctx, w):
     # cells per variable block: patches padded to 64 cells
     return sum((p.cells + 63) // 64 * 64 for p in w.patches)
@@ -232,6 +273,18 @@ def main():
     ach = f32["stage_gbs"]
     default_workload = args.workload == "jet" and (args.core, args.blocks) == (64, 4)
     traffic = measured_traffic(f32["tiling"]) if default_workload else None
+    # compute ceiling of the same kernel: VALU instructions per launch (committed
+    # profile of this workload) over the live launch time, against the probe
+    valu = None
+    insts = measured_valu_instructions(f32["tiling"]) if default_workload else None
+    probe = valu_ceiling() if insts else None
+    if insts and probe:
+        rate = insts / probe["cus"] / (f32["stage_ms"] * 1e6)
+        valu = {"bound": "valu", "achieved": round(rate, 3), "peak": probe["peak"], "unit": "wave-instr/CU/ns",
+                "frac": round(rate / probe["peak"], 4), "instructions_per_launch": insts,
+                "source": "SQ_INSTS_VALU from profiles/stage_valu_jet4x64_fp32.md (tiling %d); peak measured now by "
+                          "drnum_amd/lib/valu_probe" % f32["tiling"]}
+        log("valu", valu)
     out = {
         "metric": METRIC,
         "value": round(f32["mcups"], 2),
@@ -265,6 +318,7 @@ def main():
                                % f32["tiling"]) if traffic else None,
             "algorithmic_bytes_per_launch": round(f32["stage_bytes_per_launch"]),
         },
+        "valu_roofline": valu,
         "fp64_plain": None if f64 is None else {
             "value": round(f64["mcups"], 2),
             "ms_per_step": round(f64["ms_per_step"], 4),

@@ -1,6 +1,7 @@
 """Build the in-tree native libraries.
 
     python -m drnum_amd.build          # libdrnhip.so (hipcc, gfx950) + libdrnum.so (g++)
+                                       # + the applications and the VALU rate probe
 
 Every HIP source is compiled for gfx950 only.  Packed FP32 instructions are
 disabled (-packed-fp32-ops): hipcc would otherwise emit v_pk_* forms that are
@@ -32,6 +33,17 @@ def build_hip():
     out = os.path.join(LIB, "libdrnhip.so")
     tmp = out + ".tmp"
     _run([HIPCC] + HIP_FLAGS + ["-o", tmp, src])
+    os.replace(tmp, out)
+    return out
+
+
+def build_probe():
+    """The FP32 VALU issue-rate probe (drnum_amd/csrc/valu_probe.hip -> lib/valu_probe)."""
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "valu_probe")
+    tmp = out + ".tmp"
+    flags = [f for f in HIP_FLAGS if f not in ("-fPIC", "-shared")]
+    _run([HIPCC] + flags + ["-o", tmp, os.path.join(HERE, "csrc", "valu_probe.hip")])
     os.replace(tmp, out)
     return out
 
@@ -70,6 +82,7 @@ def build_all():
     build_hip()
     build_host()
     build_apps()
+    build_probe()
 
 
 if __name__ == "__main__":
