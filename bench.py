@@ -21,7 +21,12 @@ is one RK step over all patches.
   cpu_baseline the oracle (C restatement of the reference iterator, OpenMP over
               patches) on this host, one RK step of the same workload.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+  accuracy    (--accuracy) both forms after the configuration's step count:
+              FP32 density Linf against FP64, FP64-sum drift of mass and
+              energy, and the momentum flux through two planes of the
+              level-set case.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload W] [--accuracy]
 """
 from __future__ import annotations
 
@@ -215,6 +220,75 @@ def valu_ceiling():
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
+def plane_momentum_flux(w, rho_states, gi):
+    """x-momentum flux (rho u^2 + p) * dy * dz summed over the core cells of the
+    global i-plane gi (a control surface normal to the freestream), FP64."""
+    total = 0.0
+    for p, P in enumerate(w.patches):
+        li = gi - P.gidx[0]
+        if not (P.seek[0] <= li < P.ni - P.seek[1]):
+            continue
+        U = rho_states[p].reshape(5, P.ni, P.nj, P.nk)[:, li, P.seek[2]:P.nj - P.seek[3], P.seek[4]:P.nk - P.seek[5]]
+        U = U.astype(np.float64)
+        rho = U[0]
+        u, v, ww = U[1] / rho, U[2] / rho, U[3] / rho
+        T = (U[4] / rho - 0.5 * (u * u + v * v + ww * ww)) / 717.5
+        pr = rho * 287.2 * T
+        total += float(np.sum(rho * u * u + pr)) * P.dx(1) * P.dx(2)
+    return total
+
+
+def accuracy_report(w, steps):
+    """The FP64 per-patch and the FP32 batched form of a workload after `steps` RK
+    steps (outside any timed region): density Linf of FP32 against FP64 relative
+    to max rho, the drift of the FP64 core sums of mass and energy in both forms,
+    and for a level-set workload the x-momentum flux through an upstream and a
+    downstream plane with the FP32 error."""
+    from drnum_amd import hip
+
+    out = {"steps": steps}
+    rho64 = None
+    planes = None
+    if w.spheres is not None:
+        nx = max(P.gidx[0] + P.ni - P.seek[1] for P in w.patches)
+        planes = (nx // 12, nx - nx // 12)
+    for dtype, launch, name in ((np.float64, hip.DRN_LAUNCH_PER_PATCH, "fp64_plain"),
+                                (np.float32, hip.DRN_LAUNCH_BATCHED, "fp32_batched")):
+        hg, ctx = setup(w, dtype)
+        ctx.autotune(launch)
+        s0 = ctx.core_sums(0)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.rk_step(w.dt, w.alpha, launch)
+        ctx.sync()
+        log("accuracy: %s %d steps in %.1f s" % (name, steps, time.perf_counter() - t0))
+        s1 = ctx.core_sums(0)
+        diag = ctx.diagnostics(0, w.dt)
+        states = [ctx.download(p, 0) for p in range(len(w.patches))]
+        ctx.close()
+        r = {"mass_drift": float((s1[0] - s0[0]) / s0[0]), "energy_drift": float((s1[4] - s0[4]) / s0[4]),
+             "nonfinite": diag["nonfinite"]}
+        if planes:
+            r["momentum_flux_planes"] = [plane_momentum_flux(w, states, gi) for gi in planes]
+            r["momentum_flux_difference"] = r["momentum_flux_planes"][0] - r["momentum_flux_planes"][1]
+        out[name] = r
+        if dtype == np.float64:
+            rho64 = [s[0].astype(np.float64) for s in states]
+        else:
+            scale = max(float(np.abs(a).max()) for a in rho64)
+            out["rho_linf_fp32_vs_fp64"] = max(float(np.abs(s[0].astype(np.float64) - a).max())
+                                               for s, a in zip(states, rho64)) / scale
+        del states
+    if planes:
+        out["planes_global_i"] = list(planes)
+        a, b = out["fp64_plain"], out["fp32_batched"]
+        out["momentum_flux_rel_error"] = [abs(x - y) / abs(x) for x, y in zip(b["momentum_flux_planes"],
+                                                                              a["momentum_flux_planes"])]
+        out["momentum_flux_difference_rel_error"] = (abs(b["momentum_flux_difference"] - a["momentum_flux_difference"])
+                                                     / abs(a["momentum_flux_difference"]))
+    return out
+
+
 def ctx_vstride(ctx, w):
     # cells per variable block: patches padded to 64 cells
     return sum((p.cells + 63) // 64 * 64 for p in w.patches)
@@ -248,6 +322,11 @@ def main():
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="jet",
                     help="jet (the headline case) or one of the other configurations")
     ap.add_argument("--fp32-only", action="store_true", help="skip the FP64 per-patch form (profiling runs)")
+    ap.add_argument("--accuracy", action="store_true",
+                    help="after the measurement, run both forms for the configuration's step count and report FP32 "
+                         "against FP64 (density Linf, mass / energy drift, level-set momentum flux)")
+    ap.add_argument("--accuracy-steps", type=int, default=0, help="steps of the accuracy report (default: the "
+                                                                   "configuration's)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (patch sharding across GPUs is "
@@ -335,6 +414,9 @@ def main():
         "stream_copy_gbs": round(f32["copy_gbs"], 1),
         "cpu_baseline": cpu,
     }
+    if args.accuracy:
+        out["accuracy"] = accuracy_report(w, args.accuracy_steps or w.steps)
+        log("accuracy", out["accuracy"])
     print(json.dumps(out), flush=True)
 
 

@@ -95,6 +95,23 @@ def test_periodic_box_rk_and_sums(gpu, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_periodic_box_conserves_mass_and_energy(gpu, dtype):
+    """Periodic on every face: each face between two patches is evaluated by both
+    from the same values (the seek layers hold exact copies), so the FP64 sums of
+    mass and energy over the core cells stay constant to round-off in the FP64
+    form and to the rounding of the stored state in the FP32 form."""
+    w = configs.kelvin_helmholtz(core=8, blocks=2)
+    og, gg = make(w, dtype)
+    s0 = gg.ctx.core_sums(0)
+    for _ in range(50):
+        gg.rk_step()
+    s1 = gg.ctx.core_sums(0)
+    drift = np.abs(s1 - s0)[[0, 4]] / np.abs(s0)[[0, 4]]
+    print("kh2x8 50 steps", np.dtype(dtype).name, "mass / energy drift", drift, flush=True)
+    assert np.all(drift <= (1e-13 if dtype == np.float64 else 1e-5))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_uneven_nested_batch(gpu, dtype):
     w = configs.uneven_batch(n_patches=24, size_range=(10, 20))
     og, gg = make(w, dtype)
