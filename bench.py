@@ -11,6 +11,7 @@ is one RK step over all patches.
   value       FP32 batched form (one launch per stage over the tile work-list of
               all 64 patches), state resident in HBM, timed region = K steps.
   fp64_plain  FP64 plain per-patch form (one launch per patch per stage).
+  fp32_per_patch the FP32 arithmetic launched per patch (the batching gain).
   roofline    stage kernel: algorithmic bytes per launch (10 or 15 values of
               4 B per stored cell: stage 1 reads U^n only) / average launch time
               from HIP events on the context stream over the timed region.
@@ -339,10 +340,13 @@ def main():
     log("workload %s: %d patches, %d stored cells, %d core cells" % (w.name, len(w.patches), w.total_cells,
                                                                       w.core_cells))
     f32 = run_form(w, np.float32, args.steps, args.warmup, hip.DRN_LAUNCH_BATCHED, "fp32-batched", args.tiling)
-    f64 = None
+    f64 = f32pp = None
     if not args.fp32_only:
         f64 = run_form(w, np.float64, args.fp64_steps or args.steps, args.warmup, hip.DRN_LAUNCH_PER_PATCH,
                        "fp64-plain", args.tiling)
+        # the batching gain in one precision: the same FP32 arithmetic, one launch per patch per stage
+        f32pp = run_form(w, np.float32, args.steps, args.warmup, hip.DRN_LAUNCH_PER_PATCH, "fp32-per-patch",
+                         args.tiling)
     cpu = None
     if not args.no_cpu and args.workload == "jet":
         threads = min(16, os.cpu_count() or 1)
@@ -408,6 +412,12 @@ def main():
         "graph": {
             "ms_per_step": round(f32["graph_ms_per_step"], 4),
             "value": round(f32["graph_mcups"], 2),
+        },
+        "fp32_per_patch": None if f32pp is None else {
+            "value": round(f32pp["mcups"], 2),
+            "ms_per_step": round(f32pp["ms_per_step"], 4),
+            "tiling": f32pp["tiling"],
+            "batched_over_per_patch": round(f32["mcups"] / f32pp["mcups"], 3),
         },
         "stage_kernel_ms": round(f32["stage_ms"], 4),
         "exchange_ms": round(f32["exch_ms"], 4),
