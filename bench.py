@@ -330,8 +330,8 @@ def main():
                                                                    "configuration's)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (patch sharding across GPUs is "
-              "described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (patches would be sharded across GPUs "
+              "with one exchange per stage over xGMI); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from drnum_amd import configs, hip

@@ -25,7 +25,7 @@
  *   cartesianpatch_common.h (getXGrad/getYGrad/getZGrad),
  *   applications/drnumCylinder/main.h (EaFlux: Euler flux, then viscous flux).
  *
- * Scope restrictions (documented in DESIGN.md): patches are axis aligned
+ * Scope restrictions: patches are axis aligned
  * (base vectors (1,0,0),(0,1,0)); there are no split faces (the immersed
  * boundary path used here is the level set, which never sets
  * m_IsInsideCell, patch.cpp:711); all cells are active (patch.cpp:713).
