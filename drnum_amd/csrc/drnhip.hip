@@ -795,6 +795,15 @@ int drn_layout(drn_ctx* ctx) {
   if (!ctx) return DRN_E_ARG;
   if (ctx->laid_out) return fail(ctx, DRN_E_STATE, "layout already done");
   if (ctx->descs.empty()) return fail(ctx, DRN_E_STATE, "no patches");
+  {
+    // the kernels address a variable array with 32-bit byte offsets: check the
+    // padded size before building any list or allocating anything
+    uint64_t cells = 0;
+    for (const auto& d : ctx->descs)
+      cells += ((uint64_t)d.ni * d.nj * d.nk + ALIGN_CELLS - 1) / ALIGN_CELLS * ALIGN_CELLS;
+    if (cells * ctx->esize() >= (1ull << 32))
+      return fail(ctx, DRN_E_ARG, "grid exceeds 2^32 bytes per variable array (2^30 FP32 / 2^29 FP64 cells)");
+  }
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const int np = (int)ctx->descs.size();
   std::vector<PatchDev> pd(np);
@@ -858,9 +867,6 @@ int drn_layout(drn_ctx* ctx) {
       for (int c2 = 0; c2 < njk; c2 += 252) ctx->tx2.push_back(TileCell{p, i, c2, 0});
   }
   ctx->tx2_first[np] = (int)ctx->tx2.size();
-  // the cell kernel addresses a variable array with 32-bit byte offsets
-  if (off * ctx->esize() >= (1ull << 32))
-    return fail(ctx, DRN_E_ARG, "grid exceeds 2^32 bytes per variable array (2^30 FP32 / 2^29 FP64 cells)");
   ctx->vstride = off;
   for (int b = 0; b < NBUF; ++b) {
     if (hipMalloc(&ctx->buf[b], 5 * off * ctx->esize()) != hipSuccess) return fail(ctx, DRN_E_NOMEM, "state buffer allocation failed");
