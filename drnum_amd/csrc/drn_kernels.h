@@ -110,7 +110,7 @@ __device__ __forceinline__ void face_flux_r(const T* m, const T* a, const T* b, 
       for (int v = 0; v < 5; ++v) vr[v] = b[v] + T(0.5) * limited<T, LIM, FAST>(b[v] - q[v], a[v] - b[v]);
     }
   }
-  ausm_flux<T, FAST, DIR>(vl, vr, A, f);
+  ausm_flux_set<T, FAST, DIR>(vl, vr, A, f);  // every caller passes a zeroed f
 }
 
 // the two faces of a cell along DIR (index idx of n along that axis, byte
@@ -213,7 +213,7 @@ __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, 
         vr[v] = u0[v] - T(0.5) * s0[v];
       }
       zero5(f);
-      ausm_flux<T, FAST, DIR>(vl, vr, A, f);
+      ausm_flux_set<T, FAST, DIR>(vl, vr, A, f);
       if constexpr (VISC) visc_add<T, FAST, DIR>(sv, *G, c - st, c, um1, u0, A, f);
 #pragma unroll
       for (int v = 0; v < 5; ++v) res[v] += f[v];
@@ -225,7 +225,7 @@ __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, 
         vl[v] = u0[v] + T(0.5) * s0[v];
         vr[v] = up1[v] - T(0.5) * sp[v];
       }
-      ausm_flux<T, FAST, DIR>(vl, vr, A, fp);
+      ausm_flux_set<T, FAST, DIR>(vl, vr, A, fp);  // fp was zeroed above
       if constexpr (VISC) visc_add<T, FAST, DIR>(sv, *G, c, c + st, u0, up1, A, fp);
     }
   } else {
@@ -348,13 +348,13 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
     if (ni > 2) {
       if (i == 0) {
         zero5(f);
-        wall_flux<T>(P.bc[0], P.par[0], var, 0, false, yc, zc, j, k, Ax, f);
+        wall_flux<T, FAST>(P.bc[0], P.par[0], var, 0, false, yc, zc, j, k, Ax, f);
 #pragma unroll
         for (int v = 0; v < 5; ++v) res[v] += f[v];
       }
       if (i == ni - 1) {
         zero5(f);
-        wall_flux<T>(P.bc[1], P.par[1], var, 0, true, yc, zc, j, k, Ax, f);
+        wall_flux<T, FAST>(P.bc[1], P.par[1], var, 0, true, yc, zc, j, k, Ax, f);
 #pragma unroll
         for (int v = 0; v < 5; ++v) res[v] -= f[v];
       }
@@ -362,13 +362,13 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
     if (nj > 2) {
       if (j == 0) {
         zero5(f);
-        wall_flux<T>(P.bc[2], P.par[2], var, 1, false, xc, zc, i, k, Ay, f);
+        wall_flux<T, FAST>(P.bc[2], P.par[2], var, 1, false, xc, zc, i, k, Ay, f);
 #pragma unroll
         for (int v = 0; v < 5; ++v) res[v] += f[v];
       }
       if (j == nj - 1) {
         zero5(f);
-        wall_flux<T>(P.bc[3], P.par[3], var, 1, true, xc, zc, i, k, Ay, f);
+        wall_flux<T, FAST>(P.bc[3], P.par[3], var, 1, true, xc, zc, i, k, Ay, f);
 #pragma unroll
         for (int v = 0; v < 5; ++v) res[v] -= f[v];
       }
@@ -376,13 +376,13 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
     if (nk > 2) {
       if (k == 0) {
         zero5(f);
-        wall_flux<T>(P.bc[4], P.par[4], var, 2, false, xc, yc, i, j, Az, f);
+        wall_flux<T, FAST>(P.bc[4], P.par[4], var, 2, false, xc, yc, i, j, Az, f);
 #pragma unroll
         for (int v = 0; v < 5; ++v) res[v] += f[v];
       }
       if (k == nk - 1) {
         zero5(f);
-        wall_flux<T>(P.bc[5], P.par[5], var, 2, true, xc, yc, i, j, Az, f);
+        wall_flux<T, FAST>(P.bc[5], P.par[5], var, 2, true, xc, yc, i, j, Az, f);
 #pragma unroll
         for (int v = 0; v < 5; ++v) res[v] -= f[v];
       }
@@ -423,7 +423,7 @@ __global__ __launch_bounds__(256) void k_stage_cell_persist(const PatchDev* __re
 // ---------------------------------------------------------------------------
 
 // face functors at the patch faces of cell (i,j,k) (cartesianiterator.h:190-308)
-template <typename T>
+template <typename T, bool FAST>
 __device__ __forceinline__ void add_walls(const PatchDev& P, int i, int j, int k, const T* u0, T dx, T dy, T dz,
                                           T* res) {
   const int ni = P.ni, nj = P.nj, nk = P.nk;
@@ -436,37 +436,37 @@ __device__ __forceinline__ void add_walls(const PatchDev& P, int i, int j, int k
   T f[5];
   if (ni > 2 && i == 0) {
     zero5(f);
-    wall_flux<T>(P.bc[0], P.par[0], u0, 0, false, yc, zc, j, k, Ax, f);
+    wall_flux<T, FAST>(P.bc[0], P.par[0], u0, 0, false, yc, zc, j, k, Ax, f);
 #pragma unroll
     for (int v = 0; v < 5; ++v) res[v] += f[v];
   }
   if (ni > 2 && i == ni - 1) {
     zero5(f);
-    wall_flux<T>(P.bc[1], P.par[1], u0, 0, true, yc, zc, j, k, Ax, f);
+    wall_flux<T, FAST>(P.bc[1], P.par[1], u0, 0, true, yc, zc, j, k, Ax, f);
 #pragma unroll
     for (int v = 0; v < 5; ++v) res[v] -= f[v];
   }
   if (nj > 2 && j == 0) {
     zero5(f);
-    wall_flux<T>(P.bc[2], P.par[2], u0, 1, false, xc, zc, i, k, Ay, f);
+    wall_flux<T, FAST>(P.bc[2], P.par[2], u0, 1, false, xc, zc, i, k, Ay, f);
 #pragma unroll
     for (int v = 0; v < 5; ++v) res[v] += f[v];
   }
   if (nj > 2 && j == nj - 1) {
     zero5(f);
-    wall_flux<T>(P.bc[3], P.par[3], u0, 1, true, xc, zc, i, k, Ay, f);
+    wall_flux<T, FAST>(P.bc[3], P.par[3], u0, 1, true, xc, zc, i, k, Ay, f);
 #pragma unroll
     for (int v = 0; v < 5; ++v) res[v] -= f[v];
   }
   if (nk > 2 && k == 0) {
     zero5(f);
-    wall_flux<T>(P.bc[4], P.par[4], u0, 2, false, xc, yc, i, j, Az, f);
+    wall_flux<T, FAST>(P.bc[4], P.par[4], u0, 2, false, xc, yc, i, j, Az, f);
 #pragma unroll
     for (int v = 0; v < 5; ++v) res[v] += f[v];
   }
   if (nk > 2 && k == nk - 1) {
     zero5(f);
-    wall_flux<T>(P.bc[5], P.par[5], u0, 2, true, xc, yc, i, j, Az, f);
+    wall_flux<T, FAST>(P.bc[5], P.par[5], u0, 2, true, xc, yc, i, j, Az, f);
 #pragma unroll
     for (int v = 0; v < 5; ++v) res[v] -= f[v];
   }
@@ -617,7 +617,7 @@ __global__ __launch_bounds__(256) void k_stage_cell_x2(const PatchDev* __restric
         res[v] -= fyp[v];
         res[v] -= fxp[v];
       }
-      add_walls<T>(P, i, j, k, u0, dx, dy, dz, res);
+      add_walls<T, FAST>(P, i, j, k, u0, dx, dy, dz, res);
       // cartesianiterator.h:310-322
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
@@ -741,37 +741,37 @@ __global__ __launch_bounds__(512) void k_stage_cell_yz(const PatchDev* __restric
   const T xc = T(0.5) * dx + T(i) * dx, yc = T(0.5) * dy + T(j) * dy, zc = T(0.5) * dz + T(k) * dz;
   if (ni > 2 && i == 0) {
     zero5(f);
-    wall_flux<T>(P.bc[0], P.par[0], u0, 0, false, yc, zc, j, k, Ax, f);
+    wall_flux<T, FAST>(P.bc[0], P.par[0], u0, 0, false, yc, zc, j, k, Ax, f);
 #pragma unroll
     for (int v = 0; v < 5; ++v) res[v] += f[v];
   }
   if (ni > 2 && i == ni - 1) {
     zero5(f);
-    wall_flux<T>(P.bc[1], P.par[1], u0, 0, true, yc, zc, j, k, Ax, f);
+    wall_flux<T, FAST>(P.bc[1], P.par[1], u0, 0, true, yc, zc, j, k, Ax, f);
 #pragma unroll
     for (int v = 0; v < 5; ++v) res[v] -= f[v];
   }
   if (nj > 2 && j == 0) {
     zero5(f);
-    wall_flux<T>(P.bc[2], P.par[2], u0, 1, false, xc, zc, i, k, Ay, f);
+    wall_flux<T, FAST>(P.bc[2], P.par[2], u0, 1, false, xc, zc, i, k, Ay, f);
 #pragma unroll
     for (int v = 0; v < 5; ++v) res[v] += f[v];
   }
   if (nj > 2 && j == nj - 1) {
     zero5(f);
-    wall_flux<T>(P.bc[3], P.par[3], u0, 1, true, xc, zc, i, k, Ay, f);
+    wall_flux<T, FAST>(P.bc[3], P.par[3], u0, 1, true, xc, zc, i, k, Ay, f);
 #pragma unroll
     for (int v = 0; v < 5; ++v) res[v] -= f[v];
   }
   if (nk > 2 && k == 0) {
     zero5(f);
-    wall_flux<T>(P.bc[4], P.par[4], u0, 2, false, xc, yc, i, j, Az, f);
+    wall_flux<T, FAST>(P.bc[4], P.par[4], u0, 2, false, xc, yc, i, j, Az, f);
 #pragma unroll
     for (int v = 0; v < 5; ++v) res[v] += f[v];
   }
   if (nk > 2 && k == nk - 1) {
     zero5(f);
-    wall_flux<T>(P.bc[5], P.par[5], u0, 2, true, xc, yc, i, j, Az, f);
+    wall_flux<T, FAST>(P.bc[5], P.par[5], u0, 2, true, xc, yc, i, j, Az, f);
 #pragma unroll
     for (int v = 0; v < 5; ++v) res[v] -= f[v];
   }
@@ -862,9 +862,9 @@ __global__ __launch_bounds__(BS) void k_stage_march(const PatchDev* __restrict__
   auto xface = [&](int ia, T* f) {
     zero5(f);
     if (ia == 0) {
-      wall_flux<T>(P.bc[0], P.par[0], W[2], 0, false, T(0.5) * dy + T(j) * dy, T(0.5) * dz + T(k) * dz, j, k, Ax, f);
+      wall_flux<T, FAST>(P.bc[0], P.par[0], W[2], 0, false, T(0.5) * dy + T(j) * dy, T(0.5) * dz + T(k) * dz, j, k, Ax, f);
     } else if (ia == ni) {
-      wall_flux<T>(P.bc[1], P.par[1], W[1], 0, true, T(0.5) * dy + T(j) * dy, T(0.5) * dz + T(k) * dz, j, k, Ax, f);
+      wall_flux<T, FAST>(P.bc[1], P.par[1], W[1], 0, true, T(0.5) * dy + T(j) * dy, T(0.5) * dz + T(k) * dz, j, k, Ax, f);
     } else {
       const bool hm = ia >= 2, hp = ia + 1 < ni;
       T vl[5], vr[5];
@@ -958,9 +958,9 @@ __global__ __launch_bounds__(BS) void k_stage_march(const PatchDev* __restrict__
 #pragma unroll
         for (int v = 0; v < 5; ++v) var[v] = sU[v * lds_cells + cell];
         if (dir == 1)
-          wall_flux<T>(P.bc[face_wall], P.par[face_wall], var, 1, plus, xi, T(0.5) * dz + T(kb) * dz, i, kb, Ay, f);
+          wall_flux<T, FAST>(P.bc[face_wall], P.par[face_wall], var, 1, plus, xi, T(0.5) * dz + T(kb) * dz, i, kb, Ay, f);
         else
-          wall_flux<T>(P.bc[face_wall], P.par[face_wall], var, 2, plus, xi, T(0.5) * dy + T(ja) * dy, i, ja, Az, f);
+          wall_flux<T, FAST>(P.bc[face_wall], P.par[face_wall], var, 2, plus, xi, T(0.5) * dy + T(ja) * dy, i, ja, Az, f);
       } else {
         T vl[5], vr[5];
         const int qm1 = cm1 - h0, qp1 = cp1 - h0, qm2 = hm ? cm2 - h0 : qm1, qp2 = hp ? cp2 - h0 : qp1;
@@ -1087,9 +1087,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
     zero5(f);
     const T yc = half * dy + T(j) * dy, zc = half * dz + T(k) * dz;
     if (ia == 0) {
-      wall_flux<T>(P.bc[0], P.par[0], W[b + 1], 0, false, yc, zc, j, k, Ax, f);
+      wall_flux<T, true>(P.bc[0], P.par[0], W[b + 1], 0, false, yc, zc, j, k, Ax, f);
     } else if (ia == ni) {
-      wall_flux<T>(P.bc[1], P.par[1], W[b], 0, true, yc, zc, j, k, Ax, f);
+      wall_flux<T, true>(P.bc[1], P.par[1], W[b], 0, true, yc, zc, j, k, Ax, f);
     } else {
       T vl[5], vr[5];
 #pragma unroll
@@ -1231,9 +1231,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #pragma unroll
         for (int v = 0; v < 5; ++v) var[v] = sU[v * lds_cells + cell];
         if (dir == 1)
-          wall_flux<T>(P.bc[wall], P.par[wall], var, 1, plus, xi, half * dz + T(kb) * dz, i, kb, Ay, f);
+          wall_flux<T, true>(P.bc[wall], P.par[wall], var, 1, plus, xi, half * dz + T(kb) * dz, i, kb, Ay, f);
         else
-          wall_flux<T>(P.bc[wall], P.par[wall], var, 2, plus, xi, half * dy + T(ja) * dy, i, ja, Az, f);
+          wall_flux<T, true>(P.bc[wall], P.par[wall], var, 2, plus, xi, half * dy + T(ja) * dy, i, ja, Az, f);
       } else {
         T vl[5], vr[5];
         const int qm = cm1 - h0, qp = cp1 - h0;
@@ -1347,9 +1347,9 @@ __global__ __launch_bounds__(256) void k_stage_column(const PatchDev* __restrict
   auto xface = [&](int ia, int b, const T* sl, const T* sr, T* f) {
     zero5(f);
     if (ia == 0) {
-      wall_flux<T>(P.bc[0], P.par[0], W[b + 1], 0, false, yc, zc, j, k, Ax, f);
+      wall_flux<T, true>(P.bc[0], P.par[0], W[b + 1], 0, false, yc, zc, j, k, Ax, f);
     } else if (ia == ni) {
-      wall_flux<T>(P.bc[1], P.par[1], W[b], 0, true, yc, zc, j, k, Ax, f);
+      wall_flux<T, true>(P.bc[1], P.par[1], W[b], 0, true, yc, zc, j, k, Ax, f);
     } else {
       T vl[5], vr[5];
 #pragma unroll
@@ -1390,7 +1390,7 @@ __global__ __launch_bounds__(256) void k_stage_column(const PatchDev* __restrict
     zero5(fz);
     if (in_patch) {
       if (k == 0) {
-        wall_flux<T>(P.bc[4], P.par[4], U0, 2, false, xi, yc, i, j, Az, fz);
+        wall_flux<T, true>(P.bc[4], P.par[4], U0, 2, false, xi, yc, i, j, Az, fz);
       } else {
         T Um[5], Umm[5], Up[5], vl[5], vr[5];
 #pragma unroll
@@ -1415,7 +1415,7 @@ __global__ __launch_bounds__(256) void k_stage_column(const PatchDev* __restrict
     for (int v = 0; v < 5; ++v) fzp[v] = __shfl_down(fz[v], 1, 64);
     if (own && k == nk - 1) {
       zero5(fzp);
-      wall_flux<T>(P.bc[5], P.par[5], U0, 2, true, xi, yc, i, j, Az, fzp);
+      wall_flux<T, true>(P.bc[5], P.par[5], U0, 2, true, xi, yc, i, j, Az, fzp);
     }
     if (own) {
       // ---- y: both faces of the owned cell
@@ -1439,7 +1439,7 @@ __global__ __launch_bounds__(256) void k_stage_column(const PatchDev* __restrict
           sjp[v] = okj_p ? sym_slope<T, LIM>(Ujp[v] - U0[v], Ujpp[v] - Ujp[v]) : T(0);
         }
         if (j == 0) {
-          wall_flux<T>(P.bc[2], P.par[2], U0, 1, false, xi, zc, i, k, Ay, fym);
+          wall_flux<T, true>(P.bc[2], P.par[2], U0, 1, false, xi, zc, i, k, Ay, fym);
         } else {
           T vl[5], vr[5];
 #pragma unroll
@@ -1450,7 +1450,7 @@ __global__ __launch_bounds__(256) void k_stage_column(const PatchDev* __restrict
           ausm_flux<T, true, 1>(vl, vr, Ay, fym);
         }
         if (j == nj - 1) {
-          wall_flux<T>(P.bc[3], P.par[3], U0, 1, true, xi, zc, i, k, Ay, fyp);
+          wall_flux<T, true>(P.bc[3], P.par[3], U0, 1, true, xi, zc, i, k, Ay, fyp);
         } else {
           T vl[5], vr[5];
 #pragma unroll
@@ -1509,7 +1509,7 @@ __device__ __forceinline__ void axis_faces(const PatchDev& P, const char* const*
   // minus face
   zero5(f);
   if (!hm) {
-    wall_flux<T>(P.bc[2 * DIR], P.par[2 * DIR], U0, DIR, false, ta, tb, la, lb, A, f);
+    wall_flux<T, true>(P.bc[2 * DIR], P.par[2 * DIR], U0, DIR, false, ta, tb, la, lb, A, f);
   } else {
     T vl[5], vr[5];
 #pragma unroll
@@ -1525,7 +1525,7 @@ __device__ __forceinline__ void axis_faces(const PatchDev& P, const char* const*
   // plus face
   zero5(f);
   if (!hp) {
-    wall_flux<T>(P.bc[2 * DIR + 1], P.par[2 * DIR + 1], U0, DIR, true, ta, tb, la, lb, A, f);
+    wall_flux<T, true>(P.bc[2 * DIR + 1], P.par[2 * DIR + 1], U0, DIR, true, ta, tb, la, lb, A, f);
   } else {
     T vl[5], vr[5];
 #pragma unroll
@@ -1679,9 +1679,9 @@ __global__ __launch_bounds__(256) void k_stage_wtile(const PatchDev* __restrict_
     if (!in_col) return;
     const T yc = T(0.5) * dy + T(j) * dy, zc = T(0.5) * dz + T(k) * dz;
     if (ia == 0) {
-      wall_flux<T>(P.bc[0], P.par[0], Wn[2], 0, false, yc, zc, j, k, Ax, f);
+      wall_flux<T, FAST>(P.bc[0], P.par[0], Wn[2], 0, false, yc, zc, j, k, Ax, f);
     } else if (ia == ni) {
-      wall_flux<T>(P.bc[1], P.par[1], Wn[1], 0, true, yc, zc, j, k, Ax, f);
+      wall_flux<T, FAST>(P.bc[1], P.par[1], Wn[1], 0, true, yc, zc, j, k, Ax, f);
     } else {
       const bool hm = ia >= 2, hp = ia + 1 < ni;
       T vl[5], vr[5];
@@ -1724,12 +1724,12 @@ __global__ __launch_bounds__(256) void k_stage_wtile(const PatchDev* __restrict_
         T var[5];
 #pragma unroll
         for (int v = 0; v < 5; ++v) var[v] = S[v][(jl + 2) * RS + cc];
-        wall_flux<T>(P.bc[2], P.par[2], var, 1, false, xc, T(0.5) * dz + T(k) * dz, i, k, Ay, fym);
+        wall_flux<T, FAST>(P.bc[2], P.par[2], var, 1, false, xc, T(0.5) * dz + T(k) * dz, i, k, Ay, fym);
       } else if (j == nj) {
         T var[5];
 #pragma unroll
         for (int v = 0; v < 5; ++v) var[v] = S[v][(jl + 1) * RS + cc];
-        wall_flux<T>(P.bc[3], P.par[3], var, 1, true, xc, T(0.5) * dz + T(k) * dz, i, k, Ay, fym);
+        wall_flux<T, FAST>(P.bc[3], P.par[3], var, 1, true, xc, T(0.5) * dz + T(k) * dz, i, k, Ay, fym);
       } else {
         const bool hm = j >= 2, hp = j + 1 < nj;
         T vl[5], vr[5];
@@ -1752,12 +1752,12 @@ __global__ __launch_bounds__(256) void k_stage_wtile(const PatchDev* __restrict_
         T var[5];
 #pragma unroll
         for (int v = 0; v < 5; ++v) var[v] = S[v][rr + kl + 2];
-        wall_flux<T>(P.bc[4], P.par[4], var, 2, false, xc, T(0.5) * dy + T(j) * dy, i, j, Az, fzm);
+        wall_flux<T, FAST>(P.bc[4], P.par[4], var, 2, false, xc, T(0.5) * dy + T(j) * dy, i, j, Az, fzm);
       } else if (k == nk) {
         T var[5];
 #pragma unroll
         for (int v = 0; v < 5; ++v) var[v] = S[v][rr + kl + 1];
-        wall_flux<T>(P.bc[5], P.par[5], var, 2, true, xc, T(0.5) * dy + T(j) * dy, i, j, Az, fzm);
+        wall_flux<T, FAST>(P.bc[5], P.par[5], var, 2, true, xc, T(0.5) * dy + T(j) * dy, i, j, Az, fzm);
       } else {
         const bool hm = k >= 2, hp = k + 1 < nk;
         T vl[5], vr[5];

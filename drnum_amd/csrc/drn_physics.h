@@ -46,6 +46,17 @@ __device__ __forceinline__ float dsqrt<float>(float x) {
   return __builtin_sqrtf(x);
 }
 
+// fused multiply-add written out: a*b + c rounded once, whatever the
+// compiler's contraction would choose for the context it is inlined into
+template <typename T>
+__device__ __forceinline__ T dfma(T a, T b, T c) {
+  return __builtin_fma(a, b, c);
+}
+template <>
+__device__ __forceinline__ float dfma<float>(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+
 // reciprocal: exact division in the plain form, hardware reciprocal in the fast form
 template <typename T, bool FAST>
 __device__ __forceinline__ T drcp(T x) {
@@ -158,20 +169,23 @@ __device__ __forceinline__ T p5f(T M, T s) {
 }
 
 // Fast form of the face flux below, equal to it in exact arithmetic:
-//  * the primitive state takes p and a from the specific internal energy e
-//    directly, p = rho e R/cv and a = sqrt(gamma e R/cv) (T = e/cv is not
-//    formed), and H = E/rho + e R/cv;
-//  * M+ and M- enter as rho_l (M + |M|) + rho_r (M - |M|) with the 1/2 folded
-//    into the face area;
+//  * the primitive state takes the speed of sound from the specific internal
+//    energy e directly, a = sqrt(gamma e R/cv) (T = e/cv is not formed),
+//    H = E/rho + e R/cv, and keeps rho e for the pressure p = rho e R/cv, whose
+//    factor R/cv is applied once per face together with the face area;
+//  * M4 and P5 of each side are formed branch-free, sharing (M + s)^2 and
+//    (M - s)^2 between them; the supersonic M1(M, s) = (M + s|M|)/2 is
+//    max(M, 0) or min(M, 0), and P5 there is 1 or 0;
+//  * M+ and M- enter as rho_l max(M, 0) + rho_r min(M, 0);
 //  * the advected part 1/2 F0 (phi_l + phi_r) - 1/2 |F0| (phi_r - phi_l) is
 //    written in its upwind form max(F0, 0) phi_l + min(F0, 0) phi_r.
 template <typename T>
 struct PrimFast {
-  T r, u, v, w, p, a, H;
+  T r, u, v, w, re, a, H;  // re = rho * e
 };
 template <typename T>
 __device__ __forceinline__ PrimFast<T> lr_vars_fast(const T* var) {
-  constexpr T k_p = Gas<T>::R / Gas<T>::CV;                     // p = rho * k_p * e
+  constexpr T k_p = Gas<T>::R / Gas<T>::CV;                     // p = k_p * rho * e
   constexpr T k_a = Gas<T>::GAMMA * Gas<T>::R / Gas<T>::CV;     // a^2 = k_a * e
   PrimFast<T> s;
   s.r = var[0];
@@ -179,18 +193,41 @@ __device__ __forceinline__ PrimFast<T> lr_vars_fast(const T* var) {
   s.u = var[1] * ir;
   s.v = var[2] * ir;
   s.w = var[3] * ir;
+  // the fused chain is spelled out so that every inlined copy (the faces and
+  // the slip wall) rounds rho*e identically: contraction left to the compiler
+  // fuses differently where fewer of these values are live
   const T E = var[4] * ir;
-  const T e = E - T(0.5) * (s.u * s.u + s.v * s.v + s.w * s.w);
-  s.p = (s.r * e) * k_p;
+  const T q = dfma(s.u, s.u, dfma(s.v, s.v, s.w * s.w));
+  const T e = dfma(T(-0.5), q, E);
+  s.re = s.r * e;
   if constexpr (sizeof(T) == 4)
     s.a = __builtin_amdgcn_sqrtf(k_a * e);  // v_sqrt_f32, 1 ulp
   else
     s.a = dsqrt(k_a * e);
-  s.H = E + e * k_p;
+  s.H = dfma(e, k_p, E);
   return s;
 }
 
-template <typename T, int DIR>
+// M4(M, S) and P5(M, S) of compressibleflux.h:46-75 for S = +1 or -1:
+// M2(M, S) = S/4 (M + S)^2, so M4 = M2(M, S) (1 + (M - S)^2 / 2) and
+// P5 = M2(M, S) ((2S - M) + 3/4 M (M - S)^2) below |M| = 1
+template <typename T, int S>
+__device__ __forceinline__ void mach_split_fast(T M, T& m4, T& p5) {
+  const T t1 = M + T(S), t2 = M - T(S);
+  const T q2 = t2 * t2;
+  const T m2 = T(0.25 * S) * (t1 * t1);
+  const T m4p = m2 * (T(1) + T(0.5) * q2);
+  const T p5p = m2 * ((T(2 * S) - M) + (T(0.75) * M) * q2);
+  const bool sup = dabs(M) >= T(1);
+  const T m1 = S > 0 ? (M > T(0) ? M : T(0)) : (M < T(0) ? M : T(0));
+  const T p1 = T(S) * M > T(0) ? T(1) : T(0);
+  m4 = sup ? m1 : m4p;
+  p5 = sup ? p1 : p5p;
+}
+
+// SET = true assigns the face flux to f (the caller passes a zeroed f), SET =
+// false adds it
+template <typename T, int DIR, bool SET>
 __device__ __forceinline__ void ausm_flux_fast(const T* vl, const T* vr, T A, T* f) {
   const PrimFast<T> L = lr_vars_fast(vl), R = lr_vars_fast(vr);
   const T unl = DIR == 0 ? L.u : (DIR == 1 ? L.v : L.w);
@@ -198,23 +235,41 @@ __device__ __forceinline__ void ausm_flux_fast(const T* vl, const T* vr, T A, T*
   const T a = T(0.5) * (L.a + R.a);
   const T ia = drcp<T, true>(a);
   const T Ml = unl * ia, Mr = unr * ia;
-  const T M = m4f(Ml, T(1)) + m4f(Mr, T(-1));
-  const T aM = dabs(M);
-  const T p = p5f<T, true>(Ml, T(1)) * L.p + p5f<T, true>(Mr, T(-1)) * R.p;
-  const T f0 = (a * (T(0.5) * A)) * (L.r * (M + aM) + R.r * (M - aM));
+  T m4l, p5l, m4r, p5r;
+  mach_split_fast<T, 1>(Ml, m4l, p5l);
+  mach_split_fast<T, -1>(Mr, m4r, p5r);
+  const T M = m4l + m4r;
+  const T Mp = M > T(0) ? M : T(0), Mm = M < T(0) ? M : T(0);
+  const T f0 = (a * A) * (L.r * Mp + R.r * Mm);
+  // A p = A R/cv (P5_l rho_l e_l + P5_r rho_r e_r)
+  const T Ap = (A * (Gas<T>::R / Gas<T>::CV)) * (p5l * L.re + p5r * R.re);
   const T fp = f0 > T(0) ? f0 : T(0), fm = f0 < T(0) ? f0 : T(0);
-  f[0] += f0;
-  f[1] += fp * L.u + fm * R.u + (DIR == 0 ? A * p : T(0));
-  f[2] += fp * L.v + fm * R.v + (DIR == 1 ? A * p : T(0));
-  f[3] += fp * L.w + fm * R.w + (DIR == 2 ? A * p : T(0));
-  f[4] += fp * L.H + fm * R.H;
+  T g1 = fp * L.u + fm * R.u, g2 = fp * L.v + fm * R.v, g3 = fp * L.w + fm * R.w;
+  const T g4 = fp * L.H + fm * R.H;
+  // the pressure term on the normal momentum only (no additions of zero)
+  if constexpr (DIR == 0) g1 += Ap;
+  if constexpr (DIR == 1) g2 += Ap;
+  if constexpr (DIR == 2) g3 += Ap;
+  if constexpr (SET) {
+    f[0] = f0;
+    f[1] = g1;
+    f[2] = g2;
+    f[3] = g3;
+    f[4] = g4;
+  } else {
+    f[0] += f0;
+    f[1] += g1;
+    f[2] += g2;
+    f[3] += g3;
+    f[4] += g4;
+  }
 }
 
 // ausmplus.h:39-106 for the normal direction DIR, flux accumulated into f[5]
 template <typename T, bool FAST, int DIR>
 __device__ __forceinline__ void ausm_flux(const T* vl, const T* vr, T A, T* f) {
   if constexpr (FAST) {
-    ausm_flux_fast<T, DIR>(vl, vr, A, f);
+    ausm_flux_fast<T, DIR, false>(vl, vr, A, f);
     return;
   }
   Prim<T> L = lr_vars<T, FAST>(vl), R = lr_vars<T, FAST>(vr);
@@ -235,6 +290,16 @@ __device__ __forceinline__ void ausm_flux(const T* vl, const T* vr, T A, T* f) {
   f[2] += h0 * (L.v + R.v) - ha * (R.v - L.v) + (DIR == 1 ? A * p : T(0));
   f[3] += h0 * (L.w + R.w) - ha * (R.w - L.w) + (DIR == 2 ? A * p : T(0));
   f[4] += h0 * (L.H + R.H) - ha * (R.H - L.H);
+}
+
+// the face flux into a flux array the caller has zeroed: the fast form assigns
+// it (no additions of zero), the IEEE form adds it as the reference does
+template <typename T, bool FAST, int DIR>
+__device__ __forceinline__ void ausm_flux_set(const T* vl, const T* vr, T A, T* f) {
+  if constexpr (FAST)
+    ausm_flux_fast<T, DIR, true>(vl, vr, A, f);
+  else
+    ausm_flux<T, false, DIR>(vl, vr, A, f);
 }
 
 // ---------------------------------------------------------------------------
@@ -355,9 +420,18 @@ __device__ __forceinline__ void cons_to_prim(const T* var, T& p, T& Tt, T& u, T&
   p = var[0] * Gas<T>::R * Tt;
 }
 
-// compressibleslipflux.h:34-98
-template <typename T>
+// compressibleslipflux.h:34-98.  The fast form takes A p from the same
+// expression as the interior faces' pressure term (A R/cv rho e, rho e as
+// lr_vars_fast forms it): at a wall whose row is uniform in the normal
+// direction the wall and the first interior face then cancel exactly, as they
+// do in IEEE arithmetic, and no transverse momentum is created.
+template <typename T, bool FAST = false>
 __device__ __forceinline__ void wall_slip(const T* var, int dir, T A, T* f) {
+  if constexpr (FAST) {
+    const PrimFast<T> s = lr_vars_fast(var);
+    f[1 + dir] += (A * (Gas<T>::R / Gas<T>::CV)) * s.re;
+    return;
+  }
   T r = var[0];
   T ir = T(1) / r;
   T u = var[1] * ir, v = var[2] * ir, w = var[3] * ir;
@@ -471,11 +545,11 @@ __device__ __forceinline__ void wall_jet(const double* par, const T* var, int di
 }
 
 // one face functor call; `var` is the cell inside the face
-template <typename T>
+template <typename T, bool FAST = false>
 __device__ __forceinline__ void wall_flux(int code, const double* par, const T* var, int dir, bool plus, T ta, T tb,
                                           int64_t la, int64_t lb, T A, T* f) {
   switch (code) {
-    case BC_SLIP: wall_slip(var, dir, A, f); break;
+    case BC_SLIP: wall_slip<T, FAST>(var, dir, A, f); break;
     case BC_TRANSMISSIVE: wall_transmissive(var, dir, A, f); break;
     case BC_FARFIELD: wall_farfield(par, var, dir, plus, A, f); break;
     case BC_JET: wall_jet(par, var, dir, plus, ta, tb, la, lb, A, f); break;

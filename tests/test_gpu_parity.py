@@ -25,19 +25,30 @@ def tables_of(og):
 
 def rel_linf(gpu_states, og, field=0):
     """max over patches and variables of |gpu - oracle| / scale_v, with scale_v the
-    variable's max magnitude, floored at 1% of the density / energy magnitude so that
-    variables that vanish in exact arithmetic (e.g. transverse momentum) are measured
-    against the flow's own scale."""
-    floor = 0.0
+    variable's max magnitude.  For the three momenta scale_v is floored at 1% of
+    sqrt(rho_max (rho E)_max), about the momentum of a flow at the speed of sound,
+    so that components that vanish in exact arithmetic (transverse momentum) are
+    measured against the flow's own momentum scale in any system of units.  A
+    non-finite GPU value gives inf; a non-finite oracle value means the test
+    problem itself is invalid and raises."""
+    rho_max = e_max = 0.0
     for p in range(len(gpu_states)):
         R = og.field(p, field)
-        floor = max(floor, 0.01 * float(np.abs(R[0]).max()), 0.01 * float(np.abs(R[4]).max()))
+        if not np.isfinite(R).all():
+            raise AssertionError("oracle state of patch %d is not finite: invalid test problem" % p)
+        rho_max = max(rho_max, float(np.abs(R[0]).max()))
+        e_max = max(e_max, float(np.abs(R[4]).max()))
+    mom_floor = 0.01 * float(np.sqrt(rho_max * e_max))
     worst = 0.0
     for p, G in enumerate(gpu_states):
-        R = og.field(p, field).astype(np.float64)
         Gd = G.astype(np.float64)
+        if not np.isfinite(Gd).all():
+            return float("inf")
+        R = og.field(p, field).astype(np.float64)
         for v in range(5):
-            scale = max(float(np.abs(R[v]).max()), floor)
+            scale = float(np.abs(R[v]).max())
+            if 1 <= v <= 3:
+                scale = max(scale, mom_floor)
             worst = max(worst, float(np.abs(Gd[v] - R[v]).max()) / scale)
     return worst
 
@@ -69,8 +80,11 @@ def test_single_stage_sod(gpu, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("limiter", [configs.LIM_VANALBADA, configs.LIM_MINMOD, configs.LIM_SECONDORDER])
+@pytest.mark.parametrize("limiter", [configs.LIM_VANALBADA, configs.LIM_MINMOD])
 def test_sod_rk_steps(gpu, dtype, limiter):
+    """10 RK steps of the 16^3 Sod tube.  The unlimited second-order slopes are
+    not run here: at the contact they drive the pressure negative within a few
+    steps, in the oracle as on the GPU (test_smooth_shear_rk_steps covers them)."""
     w = configs.sod(n=16)
     w.limiter = limiter
     og, gg = make(w, dtype)
@@ -79,6 +93,44 @@ def test_sod_rk_steps(gpu, dtype, limiter):
         gg.rk_step()
     err = rel_linf(gg.state(0), og)
     print("sod16 10 steps lim", limiter, np.dtype(dtype).name, err, flush=True)
+    assert err <= TOL[dtype]
+
+
+def smooth_shear(core=16, blocks=2):
+    """Periodic double shear layer in the jet's units (rho = 1, p = 1/1.4, so
+    a = 1): u = 0.5 tanh((y - 1/4)/d) below y = 1/2 and its mirror above, d two
+    cells, v = 0.05 sin(2 pi x).  Mach 0.5 and no discontinuity anywhere."""
+    nc = core * blocks
+    h = 1.0 / nc
+    d = 2.0 * h
+    patches = configs.block_grid([nc] * 3, [blocks] * 3, [1.0] * 3, 2, [True] * 3, [configs.BC_SLIP] * 6)
+
+    def init(x, y, z):
+        yy = np.mod(y, 1.0)
+        u = np.where(yy < 0.5, 0.5 * np.tanh((yy - 0.25) / d), -0.5 * np.tanh((yy - 0.75) / d))
+        v = 0.05 * np.sin(2.0 * np.pi * x)
+        p = np.full_like(x, configs.P_AMB)
+        T = p / configs.gas.R
+        return np.stack([p, T, u, v, np.zeros_like(x)])
+
+    dt = 0.5 * h / 1.5
+    return configs.Workload("smooth_shear%dx%d" % (blocks ** 3, core), patches, configs.RECON_UPWIND2,
+                            configs.LIM_VANALBADA, [1.0] * 3, [0.25, 0.5, 1.0], dt, init)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("limiter", [configs.LIM_VANALBADA, configs.LIM_MINMOD, configs.LIM_SECONDORDER])
+def test_smooth_shear_rk_steps(gpu, dtype, limiter):
+    """Every limiter, the unlimited second-order slopes included, on a smooth
+    flow (smooth_shear: 8 patches of 16^3 cells, exchange every stage)."""
+    w = smooth_shear()
+    w.limiter = limiter
+    og, gg = make(w, dtype)
+    for s in range(10):
+        og.rk_step()
+        gg.rk_step()
+    err = rel_linf(gg.state(0), og)
+    print("shear8x16 10 steps lim", limiter, np.dtype(dtype).name, err, flush=True)
     assert err <= TOL[dtype]
 
 

