@@ -405,6 +405,17 @@ __global__ __launch_bounds__(BS) void k_stage_cell(const PatchDev* __restrict__ 
   stage_cell_item<T, RECON, LIM, FAST, BS, ZS, VISC>(pds, tiles[blockIdx.x], src, base, dst, vs, factor);
 }
 
+// The FP32 viscous form of k_stage_cell<..., ZS = true>.  Left to itself the
+// compiler gives it 114 VGPRs (4 waves per SIMD); capped at 5 waves per SIMD
+// (102 VGPRs) it runs 2.4 % faster on the viscous shear layer, and at 6 waves
+// it spills and runs 17 % slower.  The FP64 form keeps k_stage_cell.
+template <int RECON, int LIM, bool FAST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5))) void k_stage_cell_visc_f32(
+    const PatchDev* __restrict__ pds, const TileCell* __restrict__ tiles, const float* __restrict__ src,
+    const float* base, float* dst, uint64_t vs, float factor) {
+  stage_cell_item<float, RECON, LIM, FAST, 256, true, true>(pds, tiles[blockIdx.x], src, base, dst, vs, factor);
+}
+
 // Persistent form of k_stage_cell<..., ZS = true>: a grid sized to the resident
 // capacity of the device walks the work list with a stride of gridDim.x
 // (a multiple of 8, so each workgroup stays on one XCD and keeps taking items

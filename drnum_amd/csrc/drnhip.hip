@@ -285,10 +285,14 @@ static void launch_range(drn_ctx* ctx, int kind, const void* tiles, int ntiles, 
       hipLaunchKernelGGL((k_stage_cell_yz<T, RECON, LIM, FAST>), dim3(ntiles), dim3(512), 0, ctx->stream, ctx->d_pd,
                          (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
   } else if (kind == K_CELLZ) {
-    if (ntiles > 0 && ctx->visc)
-      hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, FAST, 256, true, true>), dim3(ntiles), dim3(256), 0,
-                         ctx->stream, ctx->d_pd, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
-    else if (ntiles > 0)
+    if (ntiles > 0 && ctx->visc) {
+      if constexpr (sizeof(T) == 4)
+        hipLaunchKernelGGL((k_stage_cell_visc_f32<RECON, LIM, FAST>), dim3(ntiles), dim3(256), 0, ctx->stream,
+                           ctx->d_pd, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
+      else
+        hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, FAST, 256, true, true>), dim3(ntiles), dim3(256), 0,
+                           ctx->stream, ctx->d_pd, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
+    } else if (ntiles > 0)
       hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, FAST, 256, true>), dim3(ntiles), dim3(256), 0, ctx->stream,
                          ctx->d_pd, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
   } else if (kind == K_CELLZP) {

@@ -252,6 +252,30 @@ def test_arith_modes_fp32(gpu, arith):
     assert err <= TOL[np.float32]
 
 
+@pytest.mark.parametrize("arith", [hip.DRN_ARITH_IEEE, hip.DRN_ARITH_FAST])
+@pytest.mark.parametrize("tiling", [11, 27, 0, 12, 22, 29, 31, 33])
+def test_sod_keeps_mirror_symmetry(gpu, tiling, arith):
+    """Sod along x between slip walls in y and z.  In the oracle the y and z
+    momenta stay exactly 0 and every x plane stays uniform; the GPU must keep
+    both exactly, in both FP32 arithmetic modes and for each kernel family
+    (cell, z-shared, march, column, wave tile, y/z-shared, two-plane,
+    persistent).  A slip wall whose pressure rounds differently from the
+    interior faces' pressure term breaks this at the wall cells."""
+    n = 16
+    w = configs.sod(n=n)
+    _, gg = make(w, np.float32)
+    gg.ctx.set_tiling(tiling)
+    gg.ctx.set_arith(arith)
+    for s in range(10):
+        gg.rk_step()
+    G = gg.state(0)[0].reshape(5, n, n, n)
+    assert np.isfinite(G).all()
+    assert np.count_nonzero(G[2]) == 0 and np.count_nonzero(G[3]) == 0
+    for v in (0, 1, 4):
+        assert np.array_equal(G[v], np.broadcast_to(G[v][:, :1, :1], G[v].shape)), v
+    gg.close()
+
+
 def test_autotune_keeps_state(gpu):
     w = configs.jet(core=8, blocks=2)
     og, gg = make(w, np.float32)
