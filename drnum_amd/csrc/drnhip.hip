@@ -1024,6 +1024,9 @@ int drn_exchange_set(drn_ctx* ctx, int recv_id, int n_donors, const int32_t* don
   const int np = (int)ctx->descs.size();
   if (recv_id < 0 || recv_id >= np) return fail(ctx, DRN_E_ARG, "receiver patch id out of range");
   if (n_donors < 0 || n_unique < 0 || n_rec_concat < 0 || n_wi < 0) return fail(ctx, DRN_E_ARG, "negative count");
+  if ((n_donors > 0 && (!donor_patch || !donor_info)) || (n_unique > 0 && !rec_unique) ||
+      (n_rec_concat > 0 && !rec_concat) || (n_wi > 0 && (!donor_index || !donor_weight)))
+    return fail(ctx, DRN_E_ARG, "null table pointer with a positive count");
   const int64_t nrecv_cells = (int64_t)drn_patch_cells(ctx, recv_id);
   ExchangeTables t;
   t.donor_patch.assign(donor_patch, donor_patch + n_donors);
@@ -1114,6 +1117,26 @@ int drn_exchange_commit(drn_ctx* ctx) {
       else
         multi.push_back({g, std::move(it->second)});
     }
+  }
+  // A donor cell that is also a receiver would make the result depend on the
+  // order in which receivers are filled (the reference fills them patch after
+  // patch, the exchange kernels all at once).  The reference's donor search
+  // takes donors from the core of a patch only, so such tables are rejected.
+  {
+    std::vector<bool> is_recv(ctx->vstride, false);
+    for (const auto& kv : single_by_stride)
+      for (const auto& e : kv.second) is_recv[e.first] = true;
+    for (const auto& e : multi) is_recv[e.first] = true;
+    auto donor_is_recv = [&](const std::vector<std::pair<uint32_t, std::vector<Group>>>& list) {
+      for (const auto& e : list)
+        for (const auto& g : e.second)
+          for (uint32_t d : g.idx)
+            if (is_recv[d]) return true;
+      return false;
+    };
+    bool clash = donor_is_recv(multi);
+    for (const auto& kv : single_by_stride) clash = clash || donor_is_recv(kv.second);
+    if (clash) return fail(ctx, DRN_E_RANGE, "a donor cell is also a receiver cell of the exchange");
   }
   for (auto& kv : single_by_stride) {
     drn_ctx::SingleClass c{};

@@ -141,7 +141,11 @@ int drn_exchange_set(drn_ctx* ctx, int recv_id, int n_donors, const int32_t* don
                      const int64_t* donor_info, int64_t n_unique, const int64_t* rec_unique,
                      int64_t n_rec_concat, const int64_t* rec_concat, int64_t n_wi,
                      const int64_t* donor_index, const double* donor_weight);
-/* Upload all tables set so far into the device gather lists. */
+/* Upload all tables set so far into the device gather lists.  Fails with
+ * DRN_E_RANGE when a donor cell is also a receiver cell: the reference fills
+ * receivers patch after patch, the device all at once, so such tables would
+ * give order-dependent results (the reference's donor search never produces
+ * them: donors lie in the core of their patch). */
 int drn_exchange_commit(drn_ctx* ctx);
 /* PatchIterator::copyDonorData(field) (patchiterator.h:103-108, gpu_patchiterator.h:191-233). */
 int drn_exchange(drn_ctx* ctx, int field);
