@@ -20,7 +20,8 @@ is one RK step over all patches.
               launch time, against the FP32 VALU issue rate measured by
               drnum_amd/lib/valu_probe after the timed region.
   cpu_baseline the oracle (C restatement of the reference iterator, OpenMP over
-              patches) on this host, one RK step of the same workload.
+              patches) on this host, one RK step of the same workload; its
+              single_thread entry is one stage of the same grid on one thread.
 
   accuracy    (--accuracy) both forms after the configuration's step count:
               FP32 density Linf against FP64, FP64-sum drift of mass and
@@ -305,9 +306,18 @@ def cpu_baseline(w, dtype, threads):
     og.rk_step()
     t1 = time.perf_counter()
     cells = w.total_cells
+    # the same grid on one host thread: one stage (CartesianIterator::compute
+    # over every patch, no exchange), which keeps the sample near 5 s
+    og.set_threads(1)
+    og.copy_field(0, 1)
+    t2 = time.perf_counter()
+    og.compute(w.alpha[0] * w.dt)
+    t3 = time.perf_counter()
     return dict(value=cells * len(w.alpha) / (t1 - t0) / 1e6, unit="Mcell-updates/s", cores=threads,
                 kind="port", sample="1 RK step (3 stages + 6 exchanges) of the same 64-patch jet, %s oracle, "
-                                    "OpenMP over patches, %.1f s" % (np.dtype(dtype).name, t1 - t0))
+                                    "OpenMP over patches, %.1f s" % (np.dtype(dtype).name, t1 - t0),
+                single_thread=dict(value=cells / (t3 - t2) / 1e6, cores=1,
+                                   sample="1 stage of the same grid, 1 thread, %.1f s" % (t3 - t2)))
 
 
 def main():
