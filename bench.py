@@ -84,6 +84,7 @@ def run_form(w, dtype, steps, warmup, launch, label, tiling=None):
     cells = w.total_cells
     esize = np.dtype(dtype).itemsize
     S = len(w.alpha)
+    ex_bytes, ls_bytes = exchange_fill_bytes(hg, w, esize)
     if tiling is None:
         # the library's autotuner picks the stage work-list (i-range, XCD order, kernel) for this form
         tiling = ctx.autotune(launch)
@@ -96,6 +97,7 @@ def run_form(w, dtype, steps, warmup, launch, label, tiling=None):
     ctx.sync()
     ctx.timing(0)
     ctx.timing(1)
+    ctx.timing(2)
     ctx.timing_enable(True)
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -105,6 +107,7 @@ def run_form(w, dtype, steps, warmup, launch, label, tiling=None):
     ctx.timing_enable(False)
     st_ms, st_n = ctx.timing(0)
     ex_ms, ex_n = ctx.timing(1)
+    ls_ms, ls_n = ctx.timing(2)
     wall = t1 - t0
     diag = ctx.diagnostics(0, w.dt)
     log("%s: %d steps %.3f s, stage kernel avg %.4f ms (%d), exchange avg %.4f ms (%d), diag %s"
@@ -112,7 +115,8 @@ def run_form(w, dtype, steps, warmup, launch, label, tiling=None):
     if diag["nonfinite"] > 0:
         raise RuntimeError("%s: non-finite values after the timed steps" % label)
     res = dict(wall=wall, ms_per_step=1e3 * wall / steps, mcups=cells * S * steps / wall / 1e6,
-               stage_ms=st_ms, stage_n=st_n, exch_ms=ex_ms, exch_n=ex_n, tiling=tiling)
+               stage_ms=st_ms, stage_n=st_n, exch_ms=ex_ms, exch_n=ex_n, tiling=tiling,
+               exch_bytes=ex_bytes, ls_ms=ls_ms, ls_n=ls_n, ls_bytes=ls_bytes)
     # per launch the stage kernel advances all stored cells once
     bytes_step = stage_bytes_per_step(cells, esize, S)
     res["stage_bytes_per_launch"] = bytes_step / S
@@ -143,6 +147,55 @@ def run_form(w, dtype, steps, warmup, launch, label, tiling=None):
     res["copy_gbs"] = nbytes / (cp_ms * 1e-3) / 1e9 if cp_ms > 0 else 0.0
     ctx.close()
     return res
+
+
+def exchange_fill_bytes(hg, w, esize):
+    """Algorithmic bytes of one exchange and of one level-set fill in the tables'
+    device layout.  Exchange: per receiver its 4-byte index and 5 values written,
+    per (donor index, weight) entry 4 B + one weight + 5 donor values read.
+    Fill, per layer cell: 8 x (4 B index + weight + 5 values) gathered and 5
+    values written to the accumulator, then the cell index, the normal (3), the
+    accumulator (5) and the current momentum (3) read and 5 values written."""
+    n_rec = n_pairs = 0
+    for p in range(len(w.patches)):
+        t = hg.tables(p)
+        n_rec += len(t["receivers"])
+        n_pairs += len(t["donor_index"])
+    ex = n_rec * (4 + 5 * esize) + n_pairs * (4 + 6 * esize)
+    n_ls = 0
+    if w.spheres is not None:
+        for p in range(len(w.patches)):
+            for layer in range(w.ls_inner_layers):
+                n_ls += len(hg.ls_cells(p, layer)["cells"])
+    ls = n_ls * (8 * (4 + 6 * esize) + 5 * esize + 4 + 3 * esize + 5 * esize + 3 * esize + 5 * esize)
+    return ex, ls
+
+
+def kernel_roofline(ms, nbytes, traffic=None):
+    """achieved GB/s of a launch moving nbytes algorithmic bytes in ms, against HBM peak"""
+    if not ms or not nbytes:
+        return None
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "ms": round(ms, 4), "algorithmic_bytes_per_launch": int(nbytes),
+            "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic}
+
+
+def measured_exchange_traffic():
+    """HBM bytes per exchange of the 64-patch jet from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibration applied)."""
+    try:
+        lines = open(TRAFFIC_TABLE).read().splitlines()
+    except OSError:
+        return None
+    for ln in lines:
+        cols = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if len(cols) == 5 and cols[0].startswith("k_exchange_single<float>"):
+            try:
+                return float(cols[3])
+            except ValueError:
+                return None
+    return None
 
 
 def _workloads():
@@ -434,6 +487,9 @@ def main():
         },
         "stage_kernel_ms": round(f32["stage_ms"], 4),
         "exchange_ms": round(f32["exch_ms"], 4),
+        "exchange_roofline": kernel_roofline(f32["exch_ms"], f32["exch_bytes"],
+                                             measured_exchange_traffic() if default_workload else None),
+        "levelset_fill_roofline": kernel_roofline(f32["ls_ms"], f32["ls_bytes"]),
         "stream_copy_gbs": round(f32["copy_gbs"], 1),
         "cpu_baseline": cpu,
     }
