@@ -297,13 +297,17 @@ static void launch_range(drn_ctx* ctx, int kind, const void* tiles, int ntiles, 
                          ctx->d_pd, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
   } else if (kind == K_CELLZP) {
     if (ntiles > 0) {
-      // resident workgroups per CU of this instantiation, queried once
-      static int per_cu = 0;
-      if (per_cu <= 0 &&
-          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stage_cell_persist<T, RECON, LIM, FAST>, 256, 0) !=
-              hipSuccess)
-        per_cu = 1;
-      if (per_cu <= 0) per_cu = 1;
+      // resident workgroups per CU of this instantiation (a property of the
+      // kernel and the target), queried once; the initialisation of a local
+      // static is thread-safe, so contexts driven from several threads agree
+      static const int per_cu = [] {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage_cell_persist<T, RECON, LIM, FAST>, 256, 0) !=
+                hipSuccess ||
+            n <= 0)
+          n = 1;
+        return n;
+      }();
       // a multiple of 8 keeps every workgroup on the XCD of its first item
       int g = std::max(8, (ctx->n_cu * per_cu) / 8 * 8);
       if (g > ntiles) g = ntiles;

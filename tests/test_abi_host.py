@@ -139,6 +139,30 @@ def test_bench_reads_the_committed_traffic_figure():
     assert b.measured_traffic(999) is None
 
 
+def test_jet_exchange_bytes_follow_from_the_grid():
+    """Every receiver of the benchmark jet is a seek cell with exactly one
+    unit-weight donor, so one FP32 exchange moves 52 B per receiver (index and
+    5 values written, index, weight and 5 values read): the bench's figure."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    b = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(b)
+    w = configs.jet(core=64, blocks=4)
+    hg = host.HostGrid.from_workload(w, fp64_geometry=True)
+    hg.compute_dependencies(w.periodic)
+    n_rec = n_pairs = 0
+    for p in range(len(w.patches)):
+        t = hg.tables(p)
+        n_rec += len(t["receivers"])
+        n_pairs += len(t["donor_index"])
+        assert np.all(t["donor_weight"] == 1.0)
+    assert n_rec == n_pairs == w.total_cells - w.core_cells
+    ex, ls = b.exchange_fill_bytes(hg, w, 4)
+    assert ex == 52 * n_rec == 128_524_032 and ls == 0
+    assert b.measured_exchange_traffic() == pytest.approx(418.1e6)
+
+
 def test_dropin_app_reports_missing_device(tmp_path):
     """Without a GPU the C++ application stops with the device error of
     drn_ctx_create (the C ABI's return code, turned into drnum::DeviceError)."""
