@@ -370,7 +370,20 @@ def cpu_baseline(w, dtype, threads):
                 kind="port", sample="1 RK step (3 stages + 6 exchanges) of the same 64-patch jet, %s oracle, "
                                     "OpenMP over patches, %.1f s" % (np.dtype(dtype).name, t1 - t0),
                 single_thread=dict(value=cells / (t3 - t2) / 1e6, cores=1,
-                                   sample="1 stage of the same grid, 1 thread, %.1f s" % (t3 - t2)))
+                                   sample="1 stage of the same grid, 1 thread, %.1f s" % (t3 - t2)),
+                host_cpu=host_cpu_model(), host_logical_cpus=os.cpu_count())
+
+
+def host_cpu_model():
+    """the host CPU's model name as the kernel reports it (first processor)"""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():
