@@ -16,6 +16,15 @@
 
 namespace drn {
 
+// cells [i0,i1) x [j0,j1) x [k0,k1) of one patch
+struct CellBox {
+  int32_t i0, i1, j0, j1, k0, k1;
+};
+
+__device__ __forceinline__ bool in_box(const CellBox& b, int i, int j, int k) {
+  return i >= b.i0 && i < b.i1 && j >= b.j0 && j < b.j1 && k >= b.k0 && k < b.k1;
+}
+
 struct PatchDev {
   int32_t ni, nj, nk, njk;
   uint32_t off;   // first global cell of the patch in a state buffer
@@ -24,6 +33,16 @@ struct PatchDev {
   int32_t bc[6];
   int32_t seek[6];
   double par[6][16];
+  // Exchange copies of the z-face seek layers done by the stage kernel itself
+  // (used only when a launch asks for it): a cell in push[s] also writes its
+  // new value, as the exchange would copy it, to global cell
+  // push_base[s] + i*push_si[s] + j*push_sj[s] + k of the receiving patch, and
+  // a cell in recv[s] is such a receiver and does not store its own value.
+  CellBox push[2];
+  int64_t push_base[2];
+  uint32_t push_si[2], push_sj[2];
+  CellBox recv[2];
+  int32_t npush, nrecv;
 };
 
 // work item of the cell-per-thread stage kernel: patch, plane i, first (j,k) cell
@@ -253,7 +272,7 @@ __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, 
 template <typename T, int RECON, int LIM, bool FAST, int BS, bool ZS, bool VISC = false>
 __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds, const TileCell t,
                                                 const T* __restrict__ src, const T* base, T* dst, uint64_t vs,
-                                                T factor) {
+                                                T factor, int push) {
   const PatchDev& P = pds[t.patch];
   const int ni = P.ni, nj = P.nj, nk = P.nk, njk = P.njk;
   const int lane = (int)(threadIdx.x & 63);
@@ -390,19 +409,40 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
   }
   // cartesianiterator.h:310-322
   const T pf = factor / (dx * dy * dz);
+  T nv[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     const char* bv = reinterpret_cast<const char*>(base + (uint64_t)v * vs);
-    char* dv = reinterpret_cast<char*>(dst + (uint64_t)v * vs);
-    *reinterpret_cast<T*>(dv + c) = ldv<T>(bv, c) + pf * res[v];
+    nv[v] = ldv<T>(bv, c) + pf * res[v];
+  }
+  bool store = true;
+  if (push) {
+    // the exchange of the z-face seek layers fused into the stage: a receiver
+    // keeps the value its donor writes, a donor also writes 0 + value (the
+    // exchange's unit-weight sum, which turns -0 into +0) into its receiver
+    for (int s = 0; s < P.nrecv; ++s) store = store && !in_box(P.recv[s], i, j, k);
+    for (int s = 0; s < P.npush; ++s) {
+      if (in_box(P.push[s], i, j, k)) {
+        // a receiver cell of this layout: its byte offset stays below 2^32
+        const int64_t tcell = P.push_base[s] + (int64_t)i * P.push_si[s] + (int64_t)j * P.push_sj[s] + k;
+        const uint32_t tb = (uint32_t)tcell * W;
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+          *reinterpret_cast<T*>(reinterpret_cast<char*>(dst + (uint64_t)v * vs) + tb) = T(0) + nv[v];
+      }
+    }
+  }
+  if (store) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) *reinterpret_cast<T*>(reinterpret_cast<char*>(dst + (uint64_t)v * vs) + c) = nv[v];
   }
 }
 
 template <typename T, int RECON, int LIM, bool FAST, int BS, bool ZS = false, bool VISC = false>
 __global__ __launch_bounds__(BS) void k_stage_cell(const PatchDev* __restrict__ pds, const TileCell* __restrict__ tiles,
                                                    const T* __restrict__ src, const T* base, T* dst, uint64_t vs,
-                                                   T factor) {
-  stage_cell_item<T, RECON, LIM, FAST, BS, ZS, VISC>(pds, tiles[blockIdx.x], src, base, dst, vs, factor);
+                                                   T factor, int push) {
+  stage_cell_item<T, RECON, LIM, FAST, BS, ZS, VISC>(pds, tiles[blockIdx.x], src, base, dst, vs, factor, push);
 }
 
 // The FP32 viscous form of k_stage_cell<..., ZS = true>.  Left to itself the
@@ -412,8 +452,8 @@ __global__ __launch_bounds__(BS) void k_stage_cell(const PatchDev* __restrict__ 
 template <int RECON, int LIM, bool FAST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5))) void k_stage_cell_visc_f32(
     const PatchDev* __restrict__ pds, const TileCell* __restrict__ tiles, const float* __restrict__ src,
-    const float* base, float* dst, uint64_t vs, float factor) {
-  stage_cell_item<float, RECON, LIM, FAST, 256, true, true>(pds, tiles[blockIdx.x], src, base, dst, vs, factor);
+    const float* base, float* dst, uint64_t vs, float factor, int push) {
+  stage_cell_item<float, RECON, LIM, FAST, 256, true, true>(pds, tiles[blockIdx.x], src, base, dst, vs, factor, push);
 }
 
 // Persistent form of k_stage_cell<..., ZS = true>: a grid sized to the resident
@@ -424,9 +464,9 @@ template <typename T, int RECON, int LIM, bool FAST>
 __global__ __launch_bounds__(256) void k_stage_cell_persist(const PatchDev* __restrict__ pds,
                                                             const TileCell* __restrict__ tiles, int n_items,
                                                             const T* __restrict__ src, const T* base, T* dst,
-                                                            uint64_t vs, T factor) {
+                                                            uint64_t vs, T factor, int push) {
   for (int it = (int)blockIdx.x; it < n_items; it += (int)gridDim.x)
-    stage_cell_item<T, RECON, LIM, FAST, 256, true>(pds, tiles[it], src, base, dst, vs, factor);
+    stage_cell_item<T, RECON, LIM, FAST, 256, true>(pds, tiles[it], src, base, dst, vs, factor, push);
 }
 
 // ---------------------------------------------------------------------------

@@ -101,19 +101,27 @@ struct drn_ctx {
   std::vector<TileCell> thin_tiles;
   TileCell* d_thin_tiles = nullptr;
 
-  // exchange
+  // exchange: the full receiver lists, and the lists without the z-face
+  // receivers that the stage kernel fills itself when the exchange is fused
   std::map<int, ExchangeTables> xt;
-  struct SingleClass {
-    int stride;
-    uint32_t n;
-    uint32_t *rec, *idx;
-    void* w;
+  struct ExchangeLists {
+    struct SingleClass {
+      int stride;
+      uint32_t n;
+      uint32_t *rec, *idx;
+      void* w;
+    };
+    std::vector<SingleClass> single;
+    uint32_t multi_n = 0;
+    uint32_t *m_rec = nullptr, *m_gptr = nullptr, *m_gstride = nullptr, *m_gwi = nullptr, *m_idx = nullptr;
+    void* m_w = nullptr;
   };
-  std::vector<SingleClass> xsingle;
-  uint32_t xmulti_n = 0;
-  uint32_t *xm_rec = nullptr, *xm_gptr = nullptr, *xm_gstride = nullptr, *xm_gwi = nullptr, *xm_idx = nullptr;
-  void* xm_w = nullptr;
+  ExchangeLists xfull, xpart;
   bool have_exchange = false;
+  std::vector<PatchDev> pd_host;  // the device patch descriptors as uploaded
+  bool fuse = true;               // drn_set_exchange_fusion
+  bool push_now = false;          // the stage being launched fills the fused receivers
+  int64_t n_fused = 0;            // receivers the stage kernel fills when fused
 
   // level set
   std::map<int, LsTables> lt;
@@ -254,7 +262,7 @@ static void launch_cell_t(drn_ctx* ctx, const TileCell* tiles, int ntiles, const
   }
   {
     hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, FAST, 256>), dim3(ntiles), dim3(256), 0, ctx->stream, ctx->d_pd,
-                       tiles, src, base, dst, ctx->vstride, factor);
+                       tiles, src, base, dst, ctx->vstride, factor, (int)ctx->push_now);
   }
 }
 
@@ -288,13 +296,14 @@ static void launch_range(drn_ctx* ctx, int kind, const void* tiles, int ntiles, 
     if (ntiles > 0 && ctx->visc) {
       if constexpr (sizeof(T) == 4)
         hipLaunchKernelGGL((k_stage_cell_visc_f32<RECON, LIM, FAST>), dim3(ntiles), dim3(256), 0, ctx->stream,
-                           ctx->d_pd, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
+                           ctx->d_pd, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor, (int)ctx->push_now);
       else
         hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, FAST, 256, true, true>), dim3(ntiles), dim3(256), 0,
-                           ctx->stream, ctx->d_pd, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
+                           ctx->stream, ctx->d_pd, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor,
+                           (int)ctx->push_now);
     } else if (ntiles > 0)
       hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, FAST, 256, true>), dim3(ntiles), dim3(256), 0, ctx->stream,
-                         ctx->d_pd, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
+                         ctx->d_pd, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor, (int)ctx->push_now);
   } else if (kind == K_CELLZP) {
     if (ntiles > 0) {
       // resident workgroups per CU of this instantiation (a property of the
@@ -312,7 +321,7 @@ static void launch_range(drn_ctx* ctx, int kind, const void* tiles, int ntiles, 
       int g = std::max(8, (ctx->n_cu * per_cu) / 8 * 8);
       if (g > ntiles) g = ntiles;
       hipLaunchKernelGGL((k_stage_cell_persist<T, RECON, LIM, FAST>), dim3(g), dim3(256), 0, ctx->stream, ctx->d_pd,
-                         (const TileCell*)tiles, ntiles, src, base, dst, ctx->vstride, factor);
+                         (const TileCell*)tiles, ntiles, src, base, dst, ctx->vstride, factor, (int)ctx->push_now);
     }
   }
   else
@@ -413,13 +422,26 @@ static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int l
   }
 }
 
-static int do_stage(drn_ctx* ctx, int psrc, int pbase, int pdst, double factor, int launch) {
+// The cell kernels (choices of kind K_CELL, K_CELLZ, K_CELLZP) can fill the
+// fused z-face receivers themselves; the other kinds leave them to the exchange.
+static bool push_kind(int kind) { return kind == K_CELL || kind == K_CELLZ || kind == K_CELLZP; }
+
+// whether a stage of this launch form fills the fused receivers (and the
+// exchange after it then runs on the lists without them)
+static bool push_active(const drn_ctx* ctx, int launch) {
+  if (!ctx->fuse || !ctx->have_exchange || ctx->n_fused == 0) return false;
+  return push_kind(effective_kind(ctx, launch == DRN_LAUNCH_BATCHED ? ctx->tiling : ctx->tiling_pp));
+}
+
+static int do_stage(drn_ctx* ctx, int psrc, int pbase, int pdst, double factor, int launch, bool push = false) {
   hipEvent_t stop;
   timer_begin(ctx, 0, &stop);
+  ctx->push_now = push;
   if (ctx->prec == DRN_F32)
     stage_t<float>(ctx, psrc, pbase, pdst, (float)factor, launch);
   else
     stage_t<double>(ctx, psrc, pbase, pdst, factor, launch);
+  ctx->push_now = false;
   timer_end(ctx, stop);
   HIPCHK(ctx, hipGetLastError());
   return DRN_OK;
@@ -554,27 +576,28 @@ static int build_march(drn_ctx* ctx) {
 }
 
 template <typename T>
-static void launch_exchange_t(drn_ctx* ctx, T* U) {
-  for (const auto& c : ctx->xsingle) {
+static void launch_exchange_t(drn_ctx* ctx, const drn_ctx::ExchangeLists& x, T* U) {
+  for (const auto& c : x.single) {
     if (c.n == 0) continue;
     hipLaunchKernelGGL(k_exchange_single<T>, dim3((c.n + 255) / 256), dim3(256), 0, ctx->stream, c.n, c.stride, c.rec,
                        c.idx, (const T*)c.w, U, ctx->vstride);
   }
-  if (ctx->xmulti_n > 0) {
-    hipLaunchKernelGGL(k_exchange_multi<T>, dim3((ctx->xmulti_n + 255) / 256), dim3(256), 0, ctx->stream,
-                       ctx->xmulti_n, ctx->xm_rec, ctx->xm_gptr, ctx->xm_gstride, ctx->xm_gwi, ctx->xm_idx,
-                       (const T*)ctx->xm_w, U, ctx->vstride);
+  if (x.multi_n > 0) {
+    hipLaunchKernelGGL(k_exchange_multi<T>, dim3((x.multi_n + 255) / 256), dim3(256), 0, ctx->stream, x.multi_n,
+                       x.m_rec, x.m_gptr, x.m_gstride, x.m_gwi, x.m_idx, (const T*)x.m_w, U, ctx->vstride);
   }
 }
 
-static int do_exchange(drn_ctx* ctx, int pbuf) {
+// partial: the stage just launched filled the fused z-face receivers itself
+static int do_exchange(drn_ctx* ctx, int pbuf, bool partial = false) {
   if (!ctx->have_exchange) return DRN_OK;
+  const drn_ctx::ExchangeLists& x = partial ? ctx->xpart : ctx->xfull;
   hipEvent_t stop;
   timer_begin(ctx, 1, &stop);
   if (ctx->prec == DRN_F32)
-    launch_exchange_t<float>(ctx, (float*)ctx->buf[pbuf]);
+    launch_exchange_t<float>(ctx, x, (float*)ctx->buf[pbuf]);
   else
-    launch_exchange_t<double>(ctx, (double*)ctx->buf[pbuf]);
+    launch_exchange_t<double>(ctx, x, (double*)ctx->buf[pbuf]);
   timer_end(ctx, stop);
   HIPCHK(ctx, hipGetLastError());
   return DRN_OK;
@@ -619,13 +642,15 @@ static int do_rk_step(drn_ctx* ctx, double dt, int n_alpha, const double* alpha,
   }
   ctx->first_step = false;
   int cur = X;
+  // the stage fills the fused z-face receivers, the exchange after it the rest
+  const bool push = push_active(ctx, launch);
   for (int s = 0; s < n_alpha; ++s) {
     if (s > 0 && post) {
       if ((rc = do_exchange(ctx, cur))) return rc;
     }
     const int dst = (s % 2 == 0) ? Y : Z;
-    if ((rc = do_stage(ctx, cur, X, dst, alpha[s] * dt, launch))) return rc;
-    if ((rc = do_exchange(ctx, dst))) return rc;
+    if ((rc = do_stage(ctx, cur, X, dst, alpha[s] * dt, launch, push))) return rc;
+    if ((rc = do_exchange(ctx, dst, push))) return rc;
     if (post) {
       if ((rc = do_ls_fill(ctx, dst, 0.5))) return rc;
     }
@@ -663,23 +688,29 @@ int drn_ctx_create(int device, int precision, drn_ctx** out) {
   return DRN_OK;
 }
 
-static void free_exchange(drn_ctx* ctx) {
-  for (auto& c : ctx->xsingle) {
+static void free_lists(drn_ctx::ExchangeLists& x) {
+  for (auto& c : x.single) {
     dfree(c.rec);
     dfree(c.idx);
     dfree(c.w);
   }
-  ctx->xsingle.clear();
-  dfree(ctx->xm_rec);
-  dfree(ctx->xm_gptr);
-  dfree(ctx->xm_gstride);
-  dfree(ctx->xm_gwi);
-  dfree(ctx->xm_idx);
-  dfree(ctx->xm_w);
-  ctx->xm_rec = ctx->xm_gptr = ctx->xm_gstride = ctx->xm_gwi = ctx->xm_idx = nullptr;
-  ctx->xm_w = nullptr;
-  ctx->xmulti_n = 0;
+  x.single.clear();
+  dfree(x.m_rec);
+  dfree(x.m_gptr);
+  dfree(x.m_gstride);
+  dfree(x.m_gwi);
+  dfree(x.m_idx);
+  dfree(x.m_w);
+  x.m_rec = x.m_gptr = x.m_gstride = x.m_gwi = x.m_idx = nullptr;
+  x.m_w = nullptr;
+  x.multi_n = 0;
+}
+
+static void free_exchange(drn_ctx* ctx) {
+  free_lists(ctx->xfull);
+  free_lists(ctx->xpart);
   ctx->have_exchange = false;
+  ctx->n_fused = 0;
 }
 
 static void free_ls(drn_ctx* ctx) {
@@ -882,6 +913,7 @@ int drn_layout(drn_ctx* ctx) {
   }
   HIPCHK(ctx, dalloc(&ctx->d_pd, np));
   HIPCHK(ctx, hipMemcpy(ctx->d_pd, pd.data(), np * sizeof(PatchDev), hipMemcpyHostToDevice));
+  ctx->pd_host = pd;
   HIPCHK(ctx, dalloc(&ctx->d_tiles, ctx->tiles.size()));
   HIPCHK(ctx, hipMemcpy(ctx->d_tiles, ctx->tiles.data(), ctx->tiles.size() * sizeof(TileCell), hipMemcpyHostToDevice));
   {
@@ -1080,12 +1112,30 @@ static int upload_weights(drn_ctx* ctx, void** dptr, const std::vector<double>& 
   return rc;
 }
 
+// the patch that holds global cell g
+static int patch_of(const drn_ctx* ctx, uint64_t g) {
+  return (int)(std::upper_bound(ctx->offs.begin(), ctx->offs.end(), g) - ctx->offs.begin()) - 1;
+}
+
+static void drop_graph(drn_ctx* ctx) {
+  if (ctx->gexec) {
+    (void)hipGraphExecDestroy(ctx->gexec);
+    ctx->gexec = nullptr;
+  }
+}
+
 int drn_exchange_commit(drn_ctx* ctx) {
   if (!ctx) return DRN_E_ARG;
   if (!ctx->laid_out) return fail(ctx, DRN_E_STATE, "drn_layout not called");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  drop_graph(ctx);  // a captured step holds the old lists
   free_exchange(ctx);
+  for (PatchDev& P : ctx->pd_host) {
+    P.npush = P.nrecv = 0;
+    memset(P.push, 0, sizeof(P.push));
+    memset(P.recv, 0, sizeof(P.recv));
+  }
   // per unique receiver: its contribution groups in donor order
   struct Group {
     int stride;
@@ -1094,6 +1144,8 @@ int drn_exchange_commit(drn_ctx* ctx) {
   };
   std::map<int, std::vector<std::pair<uint32_t, std::vector<Group>>>> single_by_stride;
   std::vector<std::pair<uint32_t, std::vector<Group>>> multi;
+  std::vector<bool> fused(ctx->vstride, false);  // receivers the stage kernel fills when fused
+  int64_t n_fused = 0;
   for (auto& kv : ctx->xt) {
     const int p = kv.first;
     const ExchangeTables& t = kv.second;
@@ -1111,6 +1163,58 @@ int drn_exchange_commit(drn_ctx* ctx) {
         }
         groups[t.rec_concat[r0 + ll]].push_back(std::move(g));
       }
+    }
+    // The z-face seek layers of the x/y core columns, one side at a time, are
+    // filled by the donors' stage threads when every cell of the side is a
+    // receiver with one unit-weight donor, all in one donor patch at one
+    // (i, j, k) offset: the aligned patch-to-patch copy of a block grid.
+    const drn_patch_desc& d = ctx->descs[p];
+    const int64_t njk = (int64_t)d.nj * d.nk;
+    const int i0 = d.seek[0], i1 = d.ni - d.seek[1], j0 = d.seek[2], j1 = d.nj - d.seek[3];
+    for (int side = 0; side < 2; ++side) {
+      const int k0 = side == 0 ? 0 : d.nk - d.seek[5];
+      const int k1 = side == 0 ? d.seek[4] : d.nk;
+      if (k1 <= k0 || i1 <= i0 || j1 <= j0 || k0 < 0) continue;
+      int q = -1;
+      int64_t oi = 0, oj = 0, ok = 0;
+      auto aligned = [&]() {
+        for (int i = i0; i < i1; ++i)
+          for (int j = j0; j < j1; ++j)
+            for (int k = k0; k < k1; ++k) {
+              const auto it = groups.find((int64_t)i * njk + (int64_t)j * d.nk + k);
+              if (it == groups.end() || it->second.size() != 1) return false;
+              const Group& g = it->second[0];
+              if (g.stride != 1 || g.w[0] != 1.0) return false;
+              const int dq = patch_of(ctx, g.idx[0]);
+              const drn_patch_desc& e = ctx->descs[dq];
+              const int64_t enjk = (int64_t)e.nj * e.nk, dl = (int64_t)(g.idx[0] - ctx->offs[dq]);
+              const int64_t iq = dl / enjk, jq = (dl % enjk) / e.nk, kq = dl % e.nk;
+              if (q < 0) {
+                q = dq;
+                oi = iq - i;
+                oj = jq - j;
+                ok = kq - k;
+              } else if (dq != q || iq - i != oi || jq - j != oj || kq - k != ok) {
+                return false;
+              }
+            }
+        return q >= 0;
+      };
+      if (!aligned()) continue;
+      PatchDev& Q = ctx->pd_host[q];
+      PatchDev& P = ctx->pd_host[p];
+      if (Q.npush >= 2 || P.nrecv >= 2) continue;
+      const int sl = Q.npush++;
+      Q.push[sl] = CellBox{(int32_t)(i0 + oi), (int32_t)(i1 + oi), (int32_t)(j0 + oj), (int32_t)(j1 + oj),
+                           (int32_t)(k0 + ok), (int32_t)(k1 + ok)};
+      Q.push_base[sl] = (int64_t)ctx->offs[p] - oi * njk - oj * d.nk - ok;
+      Q.push_si[sl] = (uint32_t)njk;
+      Q.push_sj[sl] = (uint32_t)d.nk;
+      P.recv[P.nrecv++] = CellBox{i0, i1, j0, j1, k0, k1};
+      for (int i = i0; i < i1; ++i)
+        for (int j = j0; j < j1; ++j)
+          for (int k = k0; k < k1; ++k) fused[ctx->offs[p] + (uint64_t)((int64_t)i * njk + (int64_t)j * d.nk + k)] = true;
+      n_fused += (int64_t)(i1 - i0) * (j1 - j0) * (k1 - k0);
     }
     for (int64_t r : t.rec_unique) {
       auto it = groups.find(r);
@@ -1134,40 +1238,41 @@ int drn_exchange_commit(drn_ctx* ctx) {
     auto donor_is_recv = [&](const std::vector<std::pair<uint32_t, std::vector<Group>>>& list) {
       for (const auto& e : list)
         for (const auto& g : e.second)
-          for (uint32_t d : g.idx)
-            if (is_recv[d]) return true;
+          for (uint32_t dd : g.idx)
+            if (is_recv[dd]) return true;
       return false;
     };
     bool clash = donor_is_recv(multi);
     for (const auto& kv : single_by_stride) clash = clash || donor_is_recv(kv.second);
     if (clash) return fail(ctx, DRN_E_RANGE, "a donor cell is also a receiver cell of the exchange");
   }
-  for (auto& kv : single_by_stride) {
-    drn_ctx::SingleClass c{};
-    c.stride = kv.first;
-    c.n = (uint32_t)kv.second.size();
-    std::vector<uint32_t> rec, idx;
-    std::vector<double> w;
-    rec.reserve(c.n);
-    idx.reserve((size_t)c.n * c.stride);
-    w.reserve((size_t)c.n * c.stride);
-    for (auto& e : kv.second) {
-      rec.push_back(e.first);
-      idx.insert(idx.end(), e.second[0].idx.begin(), e.second[0].idx.end());
-      w.insert(w.end(), e.second[0].w.begin(), e.second[0].w.end());
+  // the full lists, and the lists without the fused receivers
+  auto upload_lists = [&](drn_ctx::ExchangeLists& x, bool skip_fused) -> int {
+    for (auto& kv : single_by_stride) {
+      drn_ctx::ExchangeLists::SingleClass c{};
+      c.stride = kv.first;
+      std::vector<uint32_t> rec, idx;
+      std::vector<double> w;
+      for (const auto& e : kv.second) {
+        if (skip_fused && fused[e.first]) continue;
+        rec.push_back(e.first);
+        idx.insert(idx.end(), e.second[0].idx.begin(), e.second[0].idx.end());
+        w.insert(w.end(), e.second[0].w.begin(), e.second[0].w.end());
+      }
+      c.n = (uint32_t)rec.size();
+      if (c.n == 0) continue;
+      const int rc = upload_vec(ctx, &c.rec, rec) || upload_vec(ctx, &c.idx, idx) || upload_weights(ctx, &c.w, w)
+                         ? DRN_E_NOMEM
+                         : DRN_OK;
+      x.single.push_back(c);  // freed with the lists, also after a failed upload
+      if (rc) return fail(ctx, DRN_E_NOMEM, "exchange list upload failed");
     }
-    int rc;
-    if ((rc = upload_vec(ctx, &c.rec, rec)) || (rc = upload_vec(ctx, &c.idx, idx)) ||
-        (rc = upload_weights(ctx, &c.w, w)))
-      return rc;
-    ctx->xsingle.push_back(c);
-  }
-  if (!multi.empty()) {
     std::vector<uint32_t> rec, gptr{0}, gstride, gwi, idx;
     std::vector<double> w;
-    for (auto& e : multi) {
+    for (const auto& e : multi) {
+      if (skip_fused && fused[e.first]) continue;
       rec.push_back(e.first);
-      for (auto& g : e.second) {
+      for (const auto& g : e.second) {
         gstride.push_back((uint32_t)g.stride);
         gwi.push_back((uint32_t)idx.size());
         idx.insert(idx.end(), g.idx.begin(), g.idx.end());
@@ -1175,14 +1280,36 @@ int drn_exchange_commit(drn_ctx* ctx) {
       }
       gptr.push_back((uint32_t)gstride.size());
     }
-    ctx->xmulti_n = (uint32_t)rec.size();
-    int rc;
-    if ((rc = upload_vec(ctx, &ctx->xm_rec, rec)) || (rc = upload_vec(ctx, &ctx->xm_gptr, gptr)) ||
-        (rc = upload_vec(ctx, &ctx->xm_gstride, gstride)) || (rc = upload_vec(ctx, &ctx->xm_gwi, gwi)) ||
-        (rc = upload_vec(ctx, &ctx->xm_idx, idx)) || (rc = upload_weights(ctx, &ctx->xm_w, w)))
-      return rc;
-  }
-  ctx->have_exchange = !ctx->xsingle.empty() || ctx->xmulti_n > 0;
+    if (!rec.empty()) {
+      x.multi_n = (uint32_t)rec.size();
+      if (upload_vec(ctx, &x.m_rec, rec) || upload_vec(ctx, &x.m_gptr, gptr) || upload_vec(ctx, &x.m_gstride, gstride) ||
+          upload_vec(ctx, &x.m_gwi, gwi) || upload_vec(ctx, &x.m_idx, idx) || upload_weights(ctx, &x.m_w, w))
+        return fail(ctx, DRN_E_NOMEM, "exchange list upload failed");
+    }
+    return DRN_OK;
+  };
+  int rc;
+  if ((rc = upload_lists(ctx->xfull, false)) || (rc = upload_lists(ctx->xpart, true))) return rc;
+  ctx->have_exchange = !ctx->xfull.single.empty() || ctx->xfull.multi_n > 0;
+  ctx->n_fused = n_fused;
+  HIPCHK(ctx, hipMemcpy(ctx->d_pd, ctx->pd_host.data(), ctx->pd_host.size() * sizeof(PatchDev), hipMemcpyHostToDevice));
+  return DRN_OK;
+}
+
+int drn_set_exchange_fusion(drn_ctx* ctx, int on) {
+  if (!ctx) return DRN_E_ARG;
+  if (on != 0 && on != 1) return fail(ctx, DRN_E_ARG, "exchange fusion switch must be 0 or 1");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->fuse = on != 0;
+  drop_graph(ctx);
+  return DRN_OK;
+}
+
+int drn_exchange_fused_cells(const drn_ctx* ctx, int launch, int64_t* n) {
+  if (!ctx || !n) return DRN_E_ARG;
+  if (launch != DRN_LAUNCH_BATCHED && launch != DRN_LAUNCH_PER_PATCH) return DRN_E_ARG;
+  *n = push_active(ctx, launch) ? ctx->n_fused : 0;
   return DRN_OK;
 }
 
@@ -1215,6 +1342,7 @@ int drn_levelset_set(drn_ctx* ctx, int patch, int64_t n, const int64_t* cells, c
 int drn_levelset_commit(drn_ctx* ctx) {
   if (!ctx) return DRN_E_ARG;
   if (!ctx->laid_out) return fail(ctx, DRN_E_STATE, "drn_layout not called");
+  drop_graph(ctx);  // a captured step holds the old fill lists
   HIPCHK(ctx, hipSetDevice(ctx->device));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   free_ls(ctx);
@@ -1467,11 +1595,18 @@ int drn_autotune(drn_ctx* ctx, int launch, int* choice) {
       tried.push_back(key);
     }
     if (drn_set_tiling(ctx, launch, c) != DRN_OK) continue;
-    rc = do_stage(ctx, ctx->map[0], ctx->map[1], ctx->map[2], 0.0, launch);  // warm-up
+    // a stage and the exchange after it: the cell kernels fill the fused
+    // receivers and leave the exchange less to do
+    const bool push = push_active(ctx, launch);
+    auto trial = [&]() {
+      int r = do_stage(ctx, ctx->map[0], ctx->map[1], ctx->map[2], 0.0, launch, push);
+      return r ? r : do_exchange(ctx, ctx->map[2], push);
+    };
+    rc = trial();  // warm-up
     if (rc) break;
     HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
     const int reps = 3;
-    for (int r = 0; r < reps && rc == DRN_OK; ++r) rc = do_stage(ctx, ctx->map[0], ctx->map[1], ctx->map[2], 0.0, launch);
+    for (int r = 0; r < reps && rc == DRN_OK; ++r) rc = trial();
     HIPCHK(ctx, hipEventRecord(e1, ctx->stream));
     HIPCHK(ctx, hipEventSynchronize(e1));
     float ms = 0;

@@ -50,6 +50,8 @@ _SIGS = {
                                    C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     "drn_exchange_commit": (C.c_int, [C.c_void_p]),
     "drn_exchange": (C.c_int, [C.c_void_p, C.c_int]),
+    "drn_set_exchange_fusion": (C.c_int, [C.c_void_p, C.c_int]),
+    "drn_exchange_fused_cells": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "drn_levelset_set": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_void_p]),
     "drn_levelset_commit": (C.c_int, [C.c_void_p]),
@@ -203,6 +205,16 @@ class Context:
 
     def exchange(self, field=0):
         self._chk(self.L.drn_exchange(self.h, field), "drn_exchange")
+
+    def set_exchange_fusion(self, on=True):
+        """the cell stage kernels fill the aligned z-face receivers inside rk_step (default on)"""
+        self._chk(self.L.drn_set_exchange_fusion(self.h, int(bool(on))), "drn_set_exchange_fusion")
+
+    def exchange_fused_cells(self, launch=DRN_LAUNCH_BATCHED) -> int:
+        """receivers the stage kernel fills itself in rk_step for this launch form"""
+        n = C.c_int64(0)
+        self._chk(self.L.drn_exchange_fused_cells(self.h, launch, C.byref(n)), "drn_exchange_fused_cells")
+        return int(n.value)
 
     def levelset_fill(self, field=0, relax=0.5):
         self._chk(self.L.drn_levelset_fill(self.h, field, relax), "drn_levelset_fill")

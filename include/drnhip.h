@@ -149,6 +149,23 @@ int drn_exchange_set(drn_ctx* ctx, int recv_id, int n_donors, const int32_t* don
 int drn_exchange_commit(drn_ctx* ctx);
 /* PatchIterator::copyDonorData(field) (patchiterator.h:103-108, gpu_patchiterator.h:191-233). */
 int drn_exchange(drn_ctx* ctx, int field);
+/* No counterpart in the reference, whose copyDonorData copies every receiver
+ * after the stage.  With fusion on (the default), drn_rk_step and
+ * drn_rk_graph_run let the cell stage kernels (the tiling kinds of choices 11,
+ * 20, 27, 28, 33, 34) fill part of the exchange themselves: the z-face seek
+ * layers of a patch's x/y core columns, one side at a time, where
+ * drn_exchange_commit found that side to be an aligned unit-weight copy of
+ * one donor patch.  A donor thread writes 0 + its new value into its receiver
+ * (the exchange's own sum), the receiver thread skips its store, and the
+ * exchange after the stage runs on the other receivers.  The results are
+ * bit-identical to fusion off; drn_stage and drn_exchange never fuse.
+ * Switching drops a captured graph. */
+int drn_set_exchange_fusion(drn_ctx* ctx, int on);
+/* Receivers that the stage kernels of drn_rk_step fill themselves for this
+ * launch form (DRN_LAUNCH_*) under the current tiling and fusion switch: 0 when
+ * fusion is off, the tiling's kernel does not fuse or no side qualified at
+ * drn_exchange_commit. */
+int drn_exchange_fused_cells(const drn_ctx* ctx, int launch, int64_t* n);
 
 /* ---- level-set boundary fill -------------------------------------------- */
 /* Inner layer cells of one patch in the flat order of

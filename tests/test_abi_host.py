@@ -142,7 +142,9 @@ def test_bench_reads_the_committed_traffic_figure():
 def test_jet_exchange_bytes_follow_from_the_grid():
     """Every receiver of the benchmark jet is a seek cell with exactly one
     unit-weight donor, so one FP32 exchange moves 52 B per receiver (index and
-    5 values written, index, weight and 5 values read): the bench's figure."""
+    5 values written, index, weight and 5 values read): the bench's figure.
+    With the z-face exchange fused into the stage kernel, the exchange keeps
+    the receivers outside the z-face seek layers of the x/y core columns."""
     import importlib.util
 
     spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
@@ -151,16 +153,24 @@ def test_jet_exchange_bytes_follow_from_the_grid():
     w = configs.jet(core=64, blocks=4)
     hg = host.HostGrid.from_workload(w, fp64_geometry=True)
     hg.compute_dependencies(w.periodic)
-    n_rec = n_pairs = 0
-    for p in range(len(w.patches)):
+    n_rec = n_pairs = n_zface = 0
+    for p, P in enumerate(w.patches):
         t = hg.tables(p)
         n_rec += len(t["receivers"])
         n_pairs += len(t["donor_index"])
         assert np.all(t["donor_weight"] == 1.0)
+        r = t["receivers"]
+        i, j, k = r // (P.nj * P.nk), (r // P.nk) % P.nj, r % P.nk
+        s = P.seek
+        core_xy = (i >= s[0]) & (i < P.ni - s[1]) & (j >= s[2]) & (j < P.nj - s[3])
+        n_zface += int(np.sum(core_xy & ((k < s[4]) | (k >= P.nk - s[5]))))
     assert n_rec == n_pairs == w.total_cells - w.core_cells
     ex, ls = b.exchange_fill_bytes(hg, w, 4)
     assert ex == 52 * n_rec == 128_524_032 and ls == 0
-    assert b.measured_exchange_traffic() == pytest.approx(418.1e6)
+    # 48 interior z faces, both sides, 64 x 64 core columns, 2 layers
+    assert n_zface == 48 * 2 * 64 * 64 * 2 == 786_432
+    assert 52 * (n_rec - n_zface) == 87_629_568   # the exchange after a fused stage
+    assert b.measured_exchange_traffic() == pytest.approx(110.7e6)
 
 
 def test_dropin_app_reports_missing_device(tmp_path):
