@@ -21,8 +21,10 @@ struct CellBox {
   int32_t i0, i1, j0, j1, k0, k1;
 };
 
+// each range test as one unsigned compare: (unsigned)(x - lo) < (unsigned)(hi - lo)
 __device__ __forceinline__ bool in_box(const CellBox& b, int i, int j, int k) {
-  return i >= b.i0 && i < b.i1 && j >= b.j0 && j < b.j1 && k >= b.k0 && k < b.k1;
+  return (uint32_t)(i - b.i0) < (uint32_t)(b.i1 - b.i0) && (uint32_t)(j - b.j0) < (uint32_t)(b.j1 - b.j0) &&
+         (uint32_t)(k - b.k0) < (uint32_t)(b.k1 - b.k0);
 }
 
 struct PatchDev {
@@ -423,9 +425,10 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
     for (int s = 0; s < P.nrecv; ++s) store = store && !in_box(P.recv[s], i, j, k);
     for (int s = 0; s < P.npush; ++s) {
       if (in_box(P.push[s], i, j, k)) {
-        // a receiver cell of this layout: its byte offset stays below 2^32
-        const int64_t tcell = P.push_base[s] + (int64_t)i * P.push_si[s] + (int64_t)j * P.push_sj[s] + k;
-        const uint32_t tb = (uint32_t)tcell * W;
+        // a receiver cell of this layout, whose byte offset is below 2^32: the
+        // 32-bit sum, exact modulo 2^32, is that offset
+        const uint32_t tb =
+            ((uint32_t)P.push_base[s] + (uint32_t)i * P.push_si[s] + (uint32_t)j * P.push_sj[s] + (uint32_t)k) * W;
 #pragma unroll
         for (int v = 0; v < 5; ++v)
           *reinterpret_cast<T*>(reinterpret_cast<char*>(dst + (uint64_t)v * vs) + tb) = T(0) + nv[v];
