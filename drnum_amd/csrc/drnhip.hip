@@ -65,6 +65,7 @@ struct drn_ctx {
   std::vector<int> tz_first;
   TileCell* d_tz = nullptr;
   TileCell* d_tz_xcd = nullptr;
+  TileCell* d_tz_chain = nullptr;  // z-shared list interleaved along the fused push chains (commit)
   TileCell* d_tz_xcd_pp = nullptr;
   // 2-plane x 252-cell items of the two-plane cell kernel
   std::vector<TileCell> tx2;
@@ -211,7 +212,9 @@ enum {
 };
 struct TilingChoice {
   int irange;  // planes per work item, 0 = whole patch
-  int xcd;     // 1 = segment the work list so that neighbouring chunks share an XCD
+  int xcd;     // 1 = segment the work list so that neighbouring chunks share an XCD;
+               // 2 = the same over patch planes interleaved along the chains of
+               //     fused z-face pushes (z-shared cell kernel, batched launch)
   int kind;    // K_MARCH (LDS planes, or the slope form in fast mode), K_CELL, K_COLUMN
 };
 static const TilingChoice kTilings[] = {
@@ -221,7 +224,8 @@ static const TilingChoice kTilings[] = {
     {16, 0, K_COLUMN}, {16, 1, K_COLUMN}, {8, 1, K_COLUMN}, {4, 1, K_COLUMN}, {0, 0, K_CELL_SLOPE},
     {0, 1, K_CELL},    {0, 1, K_CELL_SLOPE}, {0, 1, K_WTILE}, {16, 1, K_WTILE}, {8, 1, K_WTILE},
     {4, 1, K_WTILE},   {8, 0, K_WTILE},      {0, 1, K_CELLZ},  {0, 0, K_CELLZ},  {0, 1, K_CELLYZ},
-    {0, 0, K_CELLYZ},  {0, 1, K_CELLX2},     {0, 0, K_CELLX2}, {0, 1, K_CELLZP}, {0, 0, K_CELLZP}};
+    {0, 0, K_CELLYZ},  {0, 1, K_CELLX2},     {0, 0, K_CELLX2}, {0, 1, K_CELLZP}, {0, 0, K_CELLZP},
+    {0, 2, K_CELLZ}};
 static const int kNumTilings = sizeof(kTilings) / sizeof(kTilings[0]);
 
 // the column kernel covers the fast MUSCL forms with a symmetric limited slope
@@ -381,7 +385,9 @@ static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int l
     } else if (kind == K_CELLYZ) {
       go(K_CELLYZ, kTilings[ctx->tiling].xcd ? ctx->d_tyz_xcd : ctx->d_tyz, (int)ctx->tyz.size());
     } else if (kind == K_CELLZ || kind == K_CELLZP) {
-      go(kind, kTilings[ctx->tiling].xcd ? ctx->d_tz_xcd : ctx->d_tz, (int)ctx->tz.size());
+      const int xo = kTilings[ctx->tiling].xcd;
+      const TileCell* tl = xo == 2 && ctx->d_tz_chain ? ctx->d_tz_chain : (xo ? ctx->d_tz_xcd : ctx->d_tz);
+      go(kind, tl, (int)ctx->tz.size());
     } else if (kind == K_CELL || kind == K_CELL_SLOPE) {
       go(kind, kTilings[ctx->tiling].xcd ? ctx->d_tiles_xcd : ctx->d_tiles, (int)ctx->tiles.size());
     } else if (kind == K_WTILE) {
@@ -738,6 +744,7 @@ void drn_ctx_destroy(drn_ctx* ctx) {
   dfree(ctx->d_tiles_xcd_pp);
   dfree(ctx->d_tz);
   dfree(ctx->d_tz_xcd);
+  dfree(ctx->d_tz_chain);
   dfree(ctx->d_tz_xcd_pp);
   dfree(ctx->d_tyz);
   dfree(ctx->d_tyz_xcd);
@@ -1124,6 +1131,61 @@ static void drop_graph(drn_ctx* ctx) {
   }
 }
 
+// The z-shared list of tiling value xcd = 2: patches joined by fused z-face
+// pushes form chains, and a chain's patches are walked plane by plane
+// together (plane i of every patch of the chain, then plane i + 1), so that
+// a donor's pushed rows and the receiver's own rows of the same lines are
+// written close together, by one XCD; then segmented per XCD as the others.
+static int build_chain_list(drn_ctx* ctx, const std::vector<std::pair<int, int>>& pairs) {
+  dfree(ctx->d_tz_chain);
+  ctx->d_tz_chain = nullptr;
+  const int np = (int)ctx->descs.size();
+  if (pairs.empty() || np == 0) return DRN_OK;
+  std::vector<int> up(np);
+  for (int p = 0; p < np; ++p) up[p] = p;
+  auto root = [&](int p) {
+    while (up[p] != p) p = up[p] = up[up[p]];
+    return p;
+  };
+  for (const auto& pr : pairs) {
+    const int a = root(pr.first), b = root(pr.second);
+    if (a != b) up[std::max(a, b)] = std::min(a, b);
+  }
+  std::vector<std::vector<int>> chains;
+  std::vector<int> chain_of(np, -1);
+  for (int p = 0; p < np; ++p) {
+    const int r = root(p);
+    if (chain_of[r] < 0) {
+      chain_of[r] = (int)chains.size();
+      chains.emplace_back();
+    }
+    chains[chain_of[r]].push_back(p);
+  }
+  std::vector<TileCell> nat;
+  nat.reserve(ctx->tz.size());
+  for (const auto& ch : chains) {
+    int ni_max = 0;
+    for (int p : ch) ni_max = std::max(ni_max, ctx->descs[p].ni);
+    for (int i = 0; i < ni_max; ++i)
+      for (int p : ch) {
+        const auto& d = ctx->descs[p];
+        if (i >= d.ni) continue;
+        const int njk = d.nj * d.nk;
+        for (int c2 = 0; c2 < njk; c2 += 252) nat.push_back(TileCell{p, i, c2, 0});
+      }
+  }
+  if (nat.size() != ctx->tz.size()) return fail(ctx, DRN_E_STATE, "chain list does not cover the z-shared list");
+  const int n = (int)nat.size(), seg = (n + 7) / 8;
+  std::vector<TileCell> out;
+  out.reserve(n);
+  for (int r = 0; r < seg; ++r)
+    for (int x = 0; x < 8; ++x)
+      if (x * seg + r < n) out.push_back(nat[x * seg + r]);
+  HIPCHK(ctx, dalloc(&ctx->d_tz_chain, n));
+  HIPCHK(ctx, hipMemcpy(ctx->d_tz_chain, out.data(), n * sizeof(TileCell), hipMemcpyHostToDevice));
+  return DRN_OK;
+}
+
 int drn_exchange_commit(drn_ctx* ctx) {
   if (!ctx) return DRN_E_ARG;
   if (!ctx->laid_out) return fail(ctx, DRN_E_STATE, "drn_layout not called");
@@ -1146,6 +1208,7 @@ int drn_exchange_commit(drn_ctx* ctx) {
   std::vector<std::pair<uint32_t, std::vector<Group>>> multi;
   std::vector<bool> fused(ctx->vstride, false);  // receivers the stage kernel fills when fused
   int64_t n_fused = 0;
+  std::vector<std::pair<int, int>> push_pairs;    // (donor patch, receiving patch) of each fused side
   for (auto& kv : ctx->xt) {
     const int p = kv.first;
     const ExchangeTables& t = kv.second;
@@ -1211,6 +1274,7 @@ int drn_exchange_commit(drn_ctx* ctx) {
       Q.push_si[sl] = (uint32_t)njk;
       Q.push_sj[sl] = (uint32_t)d.nk;
       P.recv[P.nrecv++] = CellBox{i0, i1, j0, j1, k0, k1};
+      push_pairs.push_back({q, p});
       for (int i = i0; i < i1; ++i)
         for (int j = j0; j < j1; ++j)
           for (int k = k0; k < k1; ++k) fused[ctx->offs[p] + (uint64_t)((int64_t)i * njk + (int64_t)j * d.nk + k)] = true;
@@ -1293,7 +1357,7 @@ int drn_exchange_commit(drn_ctx* ctx) {
   ctx->have_exchange = !ctx->xfull.single.empty() || ctx->xfull.multi_n > 0;
   ctx->n_fused = n_fused;
   HIPCHK(ctx, hipMemcpy(ctx->d_pd, ctx->pd_host.data(), ctx->pd_host.size() * sizeof(PatchDev), hipMemcpyHostToDevice));
-  return DRN_OK;
+  return build_chain_list(ctx, push_pairs);
 }
 
 int drn_set_exchange_fusion(drn_ctx* ctx, int on) {

@@ -152,7 +152,7 @@ int drn_exchange(drn_ctx* ctx, int field);
 /* No counterpart in the reference, whose copyDonorData copies every receiver
  * after the stage.  With fusion on (the default), drn_rk_step and
  * drn_rk_graph_run let the cell stage kernels (the tiling kinds of choices 11,
- * 20, 27, 28, 33, 34) fill part of the exchange themselves: the z-face seek
+ * 20, 27, 28, 33, 34, 35) fill part of the exchange themselves: the z-face seek
  * layers of a patch's x/y core columns, one side at a time, where
  * drn_exchange_commit found that side to be an aligned unit-weight copy of
  * one donor patch.  A donor thread writes 0 + its new value into its receiver
@@ -235,13 +235,18 @@ int drn_post_variables(drn_ctx* ctx, int field, int patch, int mask, double ref_
  *       them evaluated once;
  * 33, 34 the work list of 27 / 28 in one persistent launch: as many
  *       workgroups as the device holds at once (a multiple of 8), each
- *       walking the list with a stride of the grid size.
+ *       walking the list with a stride of the grid size;
+ * 35    the kernel of 27 on a list that walks the patches joined by fused
+ *       z-face pushes (drn_set_exchange_fusion) plane by plane together, then
+ *       XCD segmented: a donor's pushed rows and the receiver's own rows meet
+ *       in one L2.  Without fused pushes it is 27; one launch per patch runs 27.
  * Default for both forms: 11; drn_autotune picks per device. */
 int drn_num_tilings(void);
 int drn_set_tiling(drn_ctx* ctx, int launch, int choice);
 int drn_get_tiling(const drn_ctx* ctx, int launch);
-/* Time every choice on one stage of the launch form on the current state (the
- * state is not modified) and keep the fastest; writes it to *choice if not NULL. */
+/* Time every choice on one stage and the exchange after it, for the launch
+ * form, on the current state (the state is not modified), and keep the
+ * fastest; writes it to *choice if not NULL. */
 int drn_autotune(drn_ctx* ctx, int launch, int* choice);
 /* Arithmetic of the flux kernels: IEEE division and square root, or the fast
  * form (hardware reciprocal, single-division van Albada, exact 0/1 for the

@@ -27,8 +27,9 @@ CASES = {
     # overlapping and nested patches of 12-24 cells: no z side is an aligned unit-weight copy
     "uneven": (lambda: configs.uneven_batch(n_patches=30, size_range=(12, 24)), 0),
 }
-# (tiling, launch form) of each fusing kernel: cell, z-shared, persistent, per-patch z-shared
-FORMS = [(11, B), (27, B), (33, B), (28, PP)]
+# (tiling, launch form) of each fusing kernel: cell, z-shared, persistent,
+# z-shared on the push-chain list, per-patch z-shared
+FORMS = [(11, B), (27, B), (33, B), (35, B), (28, PP)]
 
 
 def setup(w, dtype, fuse):
