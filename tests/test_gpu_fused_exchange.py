@@ -22,6 +22,10 @@ CASES = {
     # 27 periodic patches, both z sides of every patch: 8 x 8 x 2 x 2
     "kh": (lambda: configs.kelvin_helmholtz(core=8, blocks=3), 27 * 8 * 8 * 2 * 2),
     "shear": (lambda: configs.viscous_shear(core=8, blocks=(2, 2, 2)), 8 * 8 * 8 * 2 * 2),
+    # one periodic patch, its own z neighbour: it pushes into its own seek layers
+    "kh1": (lambda: configs.kelvin_helmholtz(core=8, blocks=1), 1 * 8 * 8 * 2 * 2),
+    # 2-cell cores: the same donor cells push to both z neighbours (one patch, periodic)
+    "kh_thin": (lambda: configs.kelvin_helmholtz(core=2, blocks=2), 8 * 2 * 2 * 2 * 2),
     # 12 patches of 40^3 core (3 x 2 x 2): the z sides between the two layers
     "spheres": (lambda: configs.levelset_spheres(core=40), 38400),
     # overlapping and nested patches of 12-24 cells: no z side is an aligned unit-weight copy
