@@ -6,7 +6,7 @@
 // operator AusmPlus<5, Upwind2<5, VanAlbada>, PerfectGas>, add the RK
 // coefficients, run the time steps, copy the result back to the host.
 //
-// usage: drnum_run <grid file> <case file> <out file> <steps> <dt> <mode> [f64] [levelset] [visc]
+// usage: drnum_run <grid file> <case file> <out file> <steps> <dt> <mode> [f64] [levelset] [swall] [visc]
 //                  [restart=<file.dnd>] [dnd=<base name>] [codes=<code string>] [vtk=<base name>]
 //   case file (binary, little endian), per patch in grid order:
 //     int32 bc[6], float64 par[6][16], then 5*N initial values in `real`
@@ -50,6 +50,7 @@ struct Options {
   int steps = 0;
   double dt = 0;
   bool levelset = false;
+  bool swall = false;  // the level-set fill with LSOBCCompressibleSWallOp instead of the slip op
 };
 
 template <typename real, class OP>
@@ -132,8 +133,11 @@ static int run(const Options& o) {
     printf("feed: %zu patches for \"%s\"\n", iterator.numPatches(), iterator.getCodeString().c_str());
   }
   iterator.updateDevice();
-  HIP_LevelSetBC<5, OP, real>* ls = nullptr;
-  if (levelset) ls = new HIP_LevelSetBC<5, OP, real>(&iterator);
+  GenericOperation* ls = nullptr;
+  if (levelset && o.swall)
+    ls = new HIP_LevelSetBC<5, OP, real, LSOBCCompressibleSWallOp>(&iterator);
+  else if (levelset)
+    ls = new HIP_LevelSetBC<5, OP, real>(&iterator);
 
   RungeKutta<real> rk;
   HIP_RungeKutta<5, OP, real> hrk(&iterator);
@@ -177,7 +181,7 @@ static int run(const Options& o) {
 int main(int argc, char** argv) {
   if (argc < 7) {
     fprintf(stderr,
-            "usage: %s <grid> <case> <out> <steps> <dt> <rk|hip|graph> [f64] [levelset] [visc] [restart=F] [dnd=B] "
+            "usage: %s <grid> <case> <out> <steps> <dt> <rk|hip|graph> [f64] [levelset] [swall] [visc] [restart=F] [dnd=B] "
             "[codes=S] [vtk=B]\n",
             argv[0]);
     return 2;
@@ -193,6 +197,7 @@ int main(int argc, char** argv) {
   for (int a = 7; a < argc; ++a) {
     if (!strcmp(argv[a], "f64")) f64 = true;
     if (!strcmp(argv[a], "levelset")) o.levelset = true;
+    if (!strcmp(argv[a], "swall")) o.swall = true;
     if (!strcmp(argv[a], "visc")) visc = true;
     if (!strncmp(argv[a], "restart=", 8)) o.restart = argv[a] + 8;
     if (!strncmp(argv[a], "dnd=", 4)) o.dnd = argv[a] + 4;

@@ -1981,6 +1981,27 @@ __global__ __launch_bounds__(256) void k_ls_operate(uint32_t n, const uint32_t* 
   U[4 * vs + c] = acc[4 * (uint64_t)n + e];
 }
 
+// lsobccompressibleswallop.h:82-116: density and total energy from the mirror
+// sum, the momentum reversed and relaxed.  In the reference's mixed precision
+// relax * (-acc) is a product in the state type and (1. - relax) * rhou a
+// double, whose sum is stored in the state type.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ls_operate_swall(uint32_t n, const uint32_t* __restrict__ cells,
+                                                          const T* __restrict__ acc, T* U, uint64_t vs, T relax) {
+#pragma clang fp contract(off)
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const uint64_t c = cells[e];
+  U[0 * vs + c] = acc[0 * (uint64_t)n + e];
+#pragma unroll
+  for (int q = 1; q <= 3; ++q) {
+    const T a = acc[(uint64_t)q * n + e];
+    const T r = relax * (-a);
+    U[q * vs + c] = T(double(r) + (1. - double(relax)) * double(U[q * vs + c]));
+  }
+  U[4 * vs + c] = acc[4 * (uint64_t)n + e];
+}
+
 // ---------------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------------

@@ -127,6 +127,7 @@ struct drn_ctx {
   // level set
   std::map<int, LsTables> lt;
   uint32_t ls_n = 0;
+  int ls_op = DRN_LS_SLIP;  // the operator of the fill
   uint32_t *ls_cells = nullptr, *ls_donors = nullptr;
   void *ls_w = nullptr, *ls_nrm = nullptr, *ls_acc = nullptr;
 
@@ -614,8 +615,12 @@ static void launch_ls_t(drn_ctx* ctx, T* U, T relax) {
   const uint32_t n = ctx->ls_n;
   hipLaunchKernelGGL(k_ls_access<T>, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, ctx->ls_donors,
                      (const T*)ctx->ls_w, (const T*)U, ctx->vstride, (T*)ctx->ls_acc);
-  hipLaunchKernelGGL(k_ls_operate<T>, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, ctx->ls_cells,
-                     (const T*)ctx->ls_nrm, (const T*)ctx->ls_acc, U, ctx->vstride, relax);
+  if (ctx->ls_op == DRN_LS_SWALL)
+    hipLaunchKernelGGL(k_ls_operate_swall<T>, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, ctx->ls_cells,
+                       (const T*)ctx->ls_acc, U, ctx->vstride, relax);
+  else
+    hipLaunchKernelGGL(k_ls_operate<T>, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, ctx->ls_cells,
+                       (const T*)ctx->ls_nrm, (const T*)ctx->ls_acc, U, ctx->vstride, relax);
 }
 
 static int do_ls_fill(drn_ctx* ctx, int pbuf, double relax) {
@@ -1427,6 +1432,16 @@ int drn_levelset_commit(drn_ctx* ctx) {
       (rc = upload_weights(ctx, &ctx->ls_nrm, nrm)) || (rc = upload_weights(ctx, &ctx->ls_w, w)))
     return rc;
   HIPCHK(ctx, hipMalloc(&ctx->ls_acc, (size_t)5 * ctx->ls_n * ctx->esize() + 16));
+  return DRN_OK;
+}
+
+int drn_set_levelset_op(drn_ctx* ctx, int op) {
+  if (!ctx) return DRN_E_ARG;
+  if (op != DRN_LS_SLIP && op != DRN_LS_SWALL) return fail(ctx, DRN_E_ARG, "unknown level-set operator");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->ls_op = op;
+  drop_graph(ctx);  // a captured step holds the fill kernels
   return DRN_OK;
 }
 

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libdrnhip.so")
 DRN_F32, DRN_F64 = 0, 1
 DRN_LAUNCH_BATCHED, DRN_LAUNCH_PER_PATCH = 0, 1
 DRN_ARITH_IEEE, DRN_ARITH_FAST = 0, 1
+DRN_LS_SLIP, DRN_LS_SWALL = 0, 1
 NPAR = 16
 
 
@@ -56,6 +57,7 @@ _SIGS = {
                                    C.c_void_p]),
     "drn_levelset_commit": (C.c_int, [C.c_void_p]),
     "drn_levelset_fill": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "drn_set_levelset_op": (C.c_int, [C.c_void_p, C.c_int]),
     "drn_rk_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_double), C.c_int]),
     "drn_rk_graph_run": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_double), C.c_int]),
     "drn_core_sums": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
@@ -218,6 +220,10 @@ class Context:
 
     def levelset_fill(self, field=0, relax=0.5):
         self._chk(self.L.drn_levelset_fill(self.h, field, relax), "drn_levelset_fill")
+
+    def set_levelset_op(self, op):
+        """DRN_LS_SLIP (LSOBCCompressibleEulerOp, default) or DRN_LS_SWALL (LSOBCCompressibleSWallOp)"""
+        self._chk(self.L.drn_set_levelset_op(self.h, op), "drn_set_levelset_op")
 
     def rk_step(self, dt, alpha, launch=DRN_LAUNCH_BATCHED):
         a = (C.c_double * len(alpha))(*alpha)

@@ -510,10 +510,26 @@ class GenericOperation {
   virtual void operator()() = 0;
 };
 
-// The slip level-set fill (CPU_LevelSetObjectBC<LSOBCCompressibleEulerOp>,
-// cpu_levelsetobjectbc.h:58-90, relax 0.5) on the inner layers the grid's
-// level-set builder produced, for the patches of one HIP iterator.
-template <unsigned DIM, class OP, typename real>
+// The OP of CPU_LevelSetObjectBC<DIM, OP>: the slip wall
+// (lsobccompressibleeulerop.h:49-144) or the simple wall
+// (lsobccompressibleswallop.h:48-116), as type tags
+struct LSOBCCompressibleEulerOp {};
+struct LSOBCCompressibleSWallOp {};
+template <class L>
+struct ls_op_code;
+template <>
+struct ls_op_code<LSOBCCompressibleEulerOp> {
+  static constexpr int value = DRN_LS_SLIP;
+};
+template <>
+struct ls_op_code<LSOBCCompressibleSWallOp> {
+  static constexpr int value = DRN_LS_SWALL;
+};
+
+// The level-set fill (CPU_LevelSetObjectBC<LSOP>, cpu_levelsetobjectbc.h:58-90,
+// relax 0.5) on the inner layers the grid's level-set builder produced, for the
+// patches of one HIP iterator; LSOP is one of the tags above.
+template <unsigned DIM, class OP, typename real, class LSOP = LSOBCCompressibleEulerOp>
 class HIP_LevelSetBC : public GenericOperation {
  public:
   HIP_LevelSetBC(HIP_CartesianIterator<DIM, OP, real>* it, size_t field = 0, double relax = 0.5)
@@ -541,6 +557,7 @@ class HIP_LevelSetBC : public GenericOperation {
                  "drn_levelset_set");
     }
     it_->check(drn_levelset_commit(ctx), "drn_levelset_commit");
+    it_->check(drn_set_levelset_op(ctx, ls_op_code<LSOP>::value), "drn_set_levelset_op");
   }
   void operator()() override { it_->check(drn_levelset_fill(it_->context(), (int)field_, relax_), "drn_levelset_fill"); }
 

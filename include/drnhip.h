@@ -174,10 +174,16 @@ int drn_exchange_fused_cells(const drn_ctx* ctx, int launch, int64_t* n);
 int drn_levelset_set(drn_ctx* ctx, int patch, int64_t n, const int64_t* cells, const double* normals,
                      const int64_t* donors, const double* weights);
 int drn_levelset_commit(drn_ctx* ctx);
-/* CPU_LevelSetObjectBC<LSOBCCompressibleEulerOp>::operator() (cpu_levelsetobjectbc.h:58-90,
- * lsobccompressibleeulerop.h:49-144) on `field`; the mirror gather goes to
- * device scratch (the reference's abuse field). relax is 0.5 in the reference. */
+/* CPU_LevelSetObjectBC<OP>::operator() (cpu_levelsetobjectbc.h:58-90) on `field`,
+ * with OP LSOBCCompressibleEulerOp (lsobccompressibleeulerop.h:49-144, the slip
+ * wall) or LSOBCCompressibleSWallOp (lsobccompressibleswallop.h:48-116, the
+ * simple wall) as drn_set_levelset_op chose; the mirror gather goes to device
+ * scratch (the reference's abuse field). relax is 0.5 in the reference. */
 int drn_levelset_fill(drn_ctx* ctx, int field, double relax);
+/* The OP template argument of CPU_LevelSetObjectBC<DIM, OP> for drn_levelset_fill
+ * and the fill inside drn_rk_step.  Default DRN_LS_SLIP.  Drops a captured graph. */
+enum { DRN_LS_SLIP = 0, DRN_LS_SWALL = 1 };
+int drn_set_levelset_op(drn_ctx* ctx, int op);
 
 /* ---- time step ------------------------------------------------------------ */
 /* RungeKutta::operator()(dt) (rungekutta.cpp:27-42): one low-storage RK step

@@ -139,6 +139,7 @@ struct O_GRID {
   int visc; /* add CompressibleViscFlux to every interior face flux */
   int nthreads;
   int have_ls;
+  int ls_op;  /* 0: LSOBCCompressibleEulerOp, 1: LSOBCCompressibleSWallOp */
   int error;
 };
 
@@ -1492,9 +1493,12 @@ void F(grid_ls_cells)(const GRID* g, int p, int layer, int outer, int64_t* cells
   }
 }
 
+void F(grid_set_levelset_op)(GRID* g, int op) { g->ls_op = op == 1 ? 1 : 0; }
+
 /* cpu_levelsetobjectbc.h:58-90 + lsobccompressibleeulerop.h:49-144 (inner cells;
- * operateOuter is a no-op).  The flat order is patch, layer, list position
- * (levelsetobjectbc.cpp:38-134). */
+ * operateOuter is a no-op), or with ls_op 1 lsobccompressibleswallop.h:48-116,
+ * whose access is the same mirror sum.  The flat order is patch, layer, list
+ * position (levelsetobjectbc.cpp:38-134). */
 void F(grid_levelset_fill)(GRID* g, int field, int abuse_field) {
   const REAL relax = 0.5;
   for (int p = 0; p < g->np; ++p) {
@@ -1524,6 +1528,15 @@ void F(grid_levelset_fill)(GRID* g, int field, int abuse_field) {
       const lsvec_t* L = &P->inner[layer];
       for (size_t ll = 0; ll < L->n; ++ll) {
         size_t l = L->v[ll].cell;
+        if (g->ls_op == 1) {
+          /* lsobccompressibleswallop.h:82-116: density and total energy from the
+           * mirror sum, momentum reversed and relaxed; the double literal makes
+           * (1. - relax) * rhou a double sum, stored in REAL */
+          f[0 * N + l] = acc[0 * N + l];
+          for (int a = 1; a <= 3; ++a) f[a * N + l] = relax * (-acc[a * N + l]) + (1. - relax) * f[a * N + l];
+          f[4 * N + l] = acc[4 * N + l];
+          continue;
+        }
         REAL gxn = L->v[ll].gx, gyn = L->v[ll].gy, gzn = L->v[ll].gz;
         REAL rho_uvw_n = acc[1 * N + l] * gxn + acc[2 * N + l] * gyn + acc[3 * N + l] * gzn;
         REAL eps = 1.e-6;

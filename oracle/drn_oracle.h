@@ -105,8 +105,10 @@ typedef struct o64_grid o64_grid;
   /* per layer cell: cell, then (as REAL) G, gx, gy, gz; 8 donors, 8 weights */               \
   void  PFX##grid_ls_cells(const GRID* g, int p, int layer, int outer, int64_t* cells,          \
                            int64_t* donors, REAL* gvals, REAL* weights);                        \
-  /* cpu_levelsetobjectbc.h:58-90 with LSOBCCompressibleEulerOp */                              \
+  /* cpu_levelsetobjectbc.h:58-90 with LSOBCCompressibleEulerOp (op 0, the default) or     */ \
+  /* LSOBCCompressibleSWallOp (op 1, lsobccompressibleswallop.h:48-116)                    */ \
   void  PFX##grid_levelset_fill(GRID* g, int field, int abuse_field);                           \
+  void  PFX##grid_set_levelset_op(GRID* g, int op);                                             \
   /* rungekutta.cpp:27-42 (+ level-set fill as post operation when set up) */                   \
   void  PFX##grid_rk_step(GRID* g, REAL dt, int n_alpha, const REAL* alpha);                    \
   void  PFX##grid_set_threads(GRID* g, int n);                                                  \

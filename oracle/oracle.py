@@ -78,6 +78,7 @@ def lib():
                 "grid_ls_cells": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64),
                                          C.POINTER(C.c_int64), rp, rp]),
                 "grid_levelset_fill": (None, [C.c_void_p, C.c_int, C.c_int]),
+                "grid_set_levelset_op": (None, [C.c_void_p, C.c_int]),
                 "grid_rk_step": (None, [C.c_void_p, rt, C.c_int, rp]),
                 "grid_set_threads": (None, [C.c_void_p, C.c_int]),
                 "grid_core_sums": (None, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
@@ -279,3 +280,7 @@ class OracleGrid:
 
     def levelset_fill(self, field=0, abuse_field=2):
         self._f("grid_levelset_fill")(self.h, field, abuse_field)
+
+    def set_levelset_op(self, op):
+        """0: LSOBCCompressibleEulerOp (slip, the default), 1: LSOBCCompressibleSWallOp"""
+        self._f("grid_set_levelset_op")(self.h, op)

@@ -53,12 +53,13 @@ def run_app(tmp_path, w, dtype, steps, mode, levelset=None, extra=(), tag=""):
     return states
 
 
-def oracle_after(w, dtype, steps, levelset=None):
+def oracle_after(w, dtype, steps, levelset=None, ls_op=0):
     og = O.OracleGrid(w, dtype, threads=8)
     og.init_state()
     og.compute_dependencies()
     if levelset is not None:
         og.levelset(levelset[0], n_inner=levelset[1], n_outer=0, min_inner_rel=levelset[2])
+        og.set_levelset_op(ls_op)
     for _ in range(steps):
         og.rk_step()
     return og
@@ -78,15 +79,19 @@ def test_dropin_jet_all_modes(gpu, tmp_path, dtype):
             assert np.array_equal(a, b), m
 
 
+@pytest.mark.parametrize("swall", [False, True])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_dropin_spheres_levelset(gpu, tmp_path, dtype):
+def test_dropin_spheres_levelset(gpu, tmp_path, dtype, swall):
+    """HIP_LevelSetBC<5, OP, real> (the slip wall) and
+    HIP_LevelSetBC<5, OP, real, LSOBCCompressibleSWallOp> (the simple wall)"""
     w = configs.levelset_spheres(core=16, blocks=(2, 1, 1), n_spheres=2, r_range=(2.5, 3.5))
     ls = (w.spheres, 3, 0.1)
-    og = oracle_after(w, dtype, 5, ls)
-    res = {m: run_app(tmp_path, w, dtype, 5, m, ls) for m in ("rk", "hip")}
+    og = oracle_after(w, dtype, 5, ls, ls_op=1 if swall else 0)
+    extra = ("swall",) if swall else ()
+    res = {m: run_app(tmp_path, w, dtype, 5, m, ls, extra=extra, tag="sw" if swall else "") for m in ("rk", "hip")}
     for m, st in res.items():
         err = rel_linf(st, og)
-        print("drop-in spheres", m, np.dtype(dtype).name, err, flush=True)
+        print("drop-in spheres", m, "swall" if swall else "slip", np.dtype(dtype).name, err, flush=True)
         assert err <= TOL[dtype]
     for a, b in zip(res["rk"], res["hip"]):
         assert np.array_equal(a, b)

@@ -42,12 +42,17 @@ def small_spheres():
     return configs.levelset_spheres(core=16, blocks=(2, 1, 1), n_spheres=2, r_range=(2.5, 3.5))
 
 
+@pytest.mark.parametrize("op", [hip.DRN_LS_SLIP, hip.DRN_LS_SWALL])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_levelset_fill_bitwise(gpu, dtype):
+def test_levelset_fill_bitwise(gpu, dtype, op):
+    """the slip wall (LSOBCCompressibleEulerOp) and the simple wall
+    (LSOBCCompressibleSWallOp) fills, bit for bit against the oracle"""
     w = small_spheres()
     og, gg = make(w, dtype)
     og.levelset(w.spheres, n_inner=3, n_outer=0, min_inner_rel=0.1)
+    og.set_levelset_op(op)
     gg.set_levelset(ls_tables(og))
+    gg.ctx.set_levelset_op(op)
     rng = np.random.default_rng(21)
     states = []
     for p in range(og.np):
@@ -65,17 +70,20 @@ def test_levelset_fill_bitwise(gpu, dtype):
     assert n_changed > 0
 
 
+@pytest.mark.parametrize("op", [hip.DRN_LS_SLIP, hip.DRN_LS_SWALL])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_levelset_rk_steps(gpu, dtype):
+def test_levelset_rk_steps(gpu, dtype, op):
     w = small_spheres()
     og, gg = make(w, dtype)
     og.levelset(w.spheres, n_inner=3, n_outer=0, min_inner_rel=0.1)
+    og.set_levelset_op(op)
     gg.set_levelset(ls_tables(og))
+    gg.ctx.set_levelset_op(op)
     for _ in range(8):
         og.rk_step()
         gg.rk_step()
     err = rel_linf(gg.state(0), og)
-    print("spheres 8 steps", np.dtype(dtype).name, err, flush=True)
+    print("spheres 8 steps op", op, np.dtype(dtype).name, err, flush=True)
     assert err <= TOL[dtype]
 
 

@@ -243,3 +243,82 @@ def test_host_levelset_tables_match_oracle_bitwise(fp64, case):
             np.testing.assert_array_equal(b["donors"], a["donors"])
             np.testing.assert_array_equal(b["g"].astype(dtype), a["g"])
             np.testing.assert_array_equal(b["weights"].astype(dtype), a["weights"])
+
+
+# ---- the simple-wall operator (LSOBCCompressibleSWallOp) --------------------
+
+def linear_wall_workload(u_of, v_of):
+    """The plane wall of plane_wall_workload with velocities u_of(x, y), v_of(x, y)."""
+    w, h = plane_wall_workload(0.0, 0.0)
+
+    def init(x, y, z):
+        one = np.ones_like(x)
+        return np.stack([one * 1.0, one / R, u_of(x, y), v_of(x, y), 0 * x])
+
+    w.init = init
+    return w, h
+
+
+def test_swall_fill_stops_normal_and_tangential_flow():
+    """The simple wall reverses the whole mirrored momentum and blends it half
+    and half with the current one (lsobccompressibleswallop.h:103-106): in a
+    uniform flow every layer cell comes to rest, where the slip wall keeps
+    the tangential flow.  Density and total energy come from the mirror sum."""
+    for u, v in ((0.4, 0.0), (0.0, 0.4), (0.3, -0.2)):
+        w, h = plane_wall_workload(u, v)
+        g = oracle_ls(w, np.float64)
+        g.set_levelset_op(1)
+        g.init_state()
+        U0 = g.field(0, 0).copy()
+        g.levelset_fill()
+        U = g.field(0, 0)
+        cells = np.concatenate([g.ls_cells(0, L)["cells"] for L in range(3)])
+        assert len(cells) > 0
+        np.testing.assert_allclose(U[0, cells], U0[0, cells], rtol=1e-14)
+        np.testing.assert_allclose(U[4, cells], U0[4, cells], rtol=1e-14)
+        assert np.abs(U[1:4, cells]).max() < 1e-12
+        mask = np.ones(U.shape[1], bool)
+        mask[cells] = False
+        np.testing.assert_array_equal(U[:, mask], U0[:, mask])
+
+
+def test_swall_fill_of_linear_momentum_is_the_mirror_formula():
+    """Linear momentum fields are reproduced by the trilinear mirror weights,
+    so each layer cell must hold 1/2 (-m(x_m)) + 1/2 m(x_c), with the mirror
+    point x_m = x_c - 2 G n of the cell (levelsetobject.cpp:300-330)."""
+    u_of = lambda x, y: 0.2 + 0.3 * y - 0.1 * x
+    v_of = lambda x, y: -0.1 + 0.2 * x
+    w, h = linear_wall_workload(u_of, v_of)
+    g = oracle_ls(w, np.float64)
+    g.set_levelset_op(1)
+    g.init_state()
+    g.levelset_fill()
+    U = g.field(0, 0)
+    P = w.patches[0]
+    checked = 0
+    for L in range(3):
+        t = g.ls_cells(0, L)
+        i, j, k = ijk(P, t["cells"])
+        xc = np.array(P.origin)[:, None] + (np.stack([i, j, k]) + 0.5) * h
+        G, n = t["g"][:, 0], t["g"][:, 1:4].T
+        xm = xc - 2.0 * G * n
+        full = np.abs(t["weights"].sum(axis=1) - 1.0) < 1e-12
+        for q, f in ((1, u_of), (2, v_of)):
+            expect = 0.5 * (-f(xm[0], xm[1])) + 0.5 * f(xc[0], xc[1])
+            np.testing.assert_allclose(U[q, t["cells"]][full], expect[full], rtol=0, atol=2e-3 * 0.3 * h)
+        checked += int(full.sum())
+    assert checked > 50
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_swall_and_slip_differ_only_in_momentum(dtype):
+    w, _ = plane_wall_workload(0.3, -0.2)
+    out = {}
+    for op in (0, 1):
+        g = oracle_ls(w, dtype)
+        g.set_levelset_op(op)
+        g.init_state()
+        g.levelset_fill()
+        out[op] = g.field(0, 0).copy()
+    np.testing.assert_array_equal(out[0][[0, 4]], out[1][[0, 4]])
+    assert np.abs(out[0][1:4] - out[1][1:4]).max() > 0.05
