@@ -29,6 +29,8 @@
 //   vtk=B:     at the end, B_<steps, 6 digits>.vtm and B_<steps>/ with one .vts
 //              per patch (PatchGrid::writeToVtk with CompressibleVariables)
 // Prints one line per phase and exits non-zero on any error.
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -160,6 +162,19 @@ static int run(const Options& o) {
   }
   iterator.sync();
   printf("ran %d steps (%s)\n", steps, mode.c_str());
+  {
+    // the density change of the last step, as the jet demo logs it
+    // (applications/drnumJetDemo/main.h:216-225): max over patches, L2 as the
+    // root mean square of the patches' L2 norms
+    real max_all = 0, ql2 = 0;
+    for (size_t p = 0; p < iterator.numPatches(); ++p) {
+      real mx = 0, l2 = 0;
+      iterator.computeVariableDifference(p, 0, 0, 1, 0, mx, l2);
+      max_all = std::max(max_all, mx);
+      ql2 += l2 * l2;
+    }
+    printf("drho: max %.6e L2 %.6e\n", (double)max_all, std::sqrt((double)ql2 / std::max<size_t>(1, iterator.numPatches())));
+  }
   iterator.updateHost();
   std::ofstream out(out_file, std::ios::binary);
   for (auto& P : patches)

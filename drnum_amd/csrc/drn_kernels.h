@@ -2163,4 +2163,43 @@ __global__ __launch_bounds__(256) void k_diag_partial(const PatchDev* __restrict
   }
 }
 
+// Patch::computeVariableDifference (patch.cpp:733-746) over the cells of one
+// patch's tiles: per cell (b - a)^2 in the state type; per block its maximum
+// (exactly a state-type value) and its FP64 sum.
+template <typename T>
+__global__ __launch_bounds__(256) void k_vardiff_partial(const PatchDev* __restrict__ pds,
+                                                         const TileCell* __restrict__ tiles, const T* __restrict__ a,
+                                                         const T* __restrict__ b, double* __restrict__ part) {
+#pragma clang fp contract(off)
+  __shared__ double red[4][2];
+  const TileCell t = tiles[blockIdx.x];
+  const PatchDev& P = pds[t.patch];
+  const int c2 = t.c2 + (int)threadIdx.x;
+  double mx = 0, sum = 0;
+  if (c2 < P.njk) {
+    const uint64_t c = (uint64_t)P.off + (uint64_t)t.i * P.njk + c2;
+    const T d = b[c] - a[c];
+    const T sq = d * d;
+    mx = double(sq);
+    sum = double(sq);
+  }
+  mx = wave_max(mx);
+  sum = wave_sum(sum);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[wv][0] = mx;
+    red[wv][1] = sum;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double o0 = red[0][0], o1 = red[0][1];
+    for (int q = 1; q < 4; ++q) {
+      o0 = fmax(o0, red[q][0]);
+      o1 += red[q][1];
+    }
+    part[(uint64_t)blockIdx.x * 2 + 0] = o0;
+    part[(uint64_t)blockIdx.x * 2 + 1] = o1;
+  }
+}
+
 }  // namespace drn

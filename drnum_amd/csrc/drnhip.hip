@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -1579,6 +1580,45 @@ int drn_diagnostics(drn_ctx* ctx, int field, double dt, double* out) {
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const void* U = ctx->buf[ctx->map[field]];
   return ctx->prec == DRN_F32 ? diag_t(ctx, (const float*)U, dt, out) : diag_t(ctx, (const double*)U, dt, out);
+}
+
+template <typename T>
+static int vardiff_t(drn_ctx* ctx, int patch, const T* a, const T* b, double* max_norm, double* l2_norm) {
+  const int t0 = ctx->tile_first[patch], nt = ctx->tile_first[patch + 1] - t0;
+  if (nt > 0)
+    hipLaunchKernelGGL(k_vardiff_partial<T>, dim3(nt), dim3(256), 0, ctx->stream, ctx->d_pd, ctx->d_tiles + t0, a, b,
+                       ctx->d_part);
+  HIPCHK(ctx, hipGetLastError());
+  std::vector<double> h((size_t)nt * 2);
+  if (nt > 0)
+    HIPCHK(ctx, hipMemcpyAsync(h.data(), ctx->d_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  double mx = 0, sum = 0;
+  for (int q = 0; q < nt; ++q) {
+    mx = std::max(mx, h[2 * q]);
+    sum += h[2 * q + 1];
+  }
+  // the maximum is a state-type value: its square root in the state type, as
+  // the reference takes it; the sum was accumulated in FP64
+  *max_norm = (double)std::sqrt((T)mx);
+  *l2_norm = std::sqrt(sum);
+  return DRN_OK;
+}
+
+int drn_variable_difference(drn_ctx* ctx, int patch, int field1, int var1, int field2, int var2, double* max_norm,
+                            double* l2_norm) {
+  if (!ctx || !max_norm || !l2_norm) return DRN_E_ARG;
+  if (!ctx->laid_out) return fail(ctx, DRN_E_STATE, "drn_layout not called");
+  if (patch < 0 || patch >= (int)ctx->descs.size()) return fail(ctx, DRN_E_ARG, "patch id out of range");
+  if (var1 < 0 || var1 > 4 || var2 < 0 || var2 > 4) return fail(ctx, DRN_E_ARG, "variable index out of range");
+  int rc;
+  if ((rc = check_field(ctx, field1)) || (rc = check_field(ctx, field2))) return rc;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t es = ctx->esize();
+  const char* A = (const char*)ctx->buf[ctx->map[field1]] + (size_t)var1 * ctx->vstride * es;
+  const char* Bv = (const char*)ctx->buf[ctx->map[field2]] + (size_t)var2 * ctx->vstride * es;
+  return ctx->prec == DRN_F32 ? vardiff_t(ctx, patch, (const float*)A, (const float*)Bv, max_norm, l2_norm)
+                              : vardiff_t(ctx, patch, (const double*)A, (const double*)Bv, max_norm, l2_norm);
 }
 
 template <typename T>

@@ -58,6 +58,8 @@ _SIGS = {
     "drn_levelset_commit": (C.c_int, [C.c_void_p]),
     "drn_levelset_fill": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "drn_set_levelset_op": (C.c_int, [C.c_void_p, C.c_int]),
+    "drn_variable_difference": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                          C.c_void_p]),
     "drn_rk_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_double), C.c_int]),
     "drn_rk_graph_run": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_double), C.c_int]),
     "drn_core_sums": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
@@ -220,6 +222,13 @@ class Context:
 
     def levelset_fill(self, field=0, relax=0.5):
         self._chk(self.L.drn_levelset_fill(self.h, field, relax), "drn_levelset_fill")
+
+    def variable_difference(self, patch, field1, var1, field2, var2):
+        """Patch::computeVariableDifference on the device: (max_norm, l2_norm)"""
+        mx, l2 = C.c_double(0), C.c_double(0)
+        self._chk(self.L.drn_variable_difference(self.h, patch, field1, var1, field2, var2, C.byref(mx), C.byref(l2)),
+                  "drn_variable_difference")
+        return mx.value, l2.value
 
     def set_levelset_op(self, op):
         """DRN_LS_SLIP (LSOBCCompressibleEulerOp, default) or DRN_LS_SWALL (LSOBCCompressibleSWallOp)"""

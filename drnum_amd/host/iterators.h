@@ -368,6 +368,18 @@ class HIP_CartesianIterator : public PatchIterator<real> {
     return choice;
   }
   void sync() { check(drn_sync(ctx_), "drn_sync"); }
+  // Patch::computeVariableDifference (patch.cpp:733-746) for the iterator's
+  // patch i_patch, on the device: max_norm as the reference computes it, the
+  // sum of squares accumulated in FP64
+  void computeVariableDifference(size_t i_patch, size_t i_field1, size_t i_var1, size_t i_field2, size_t i_var2,
+                                 real& max_norm, real& l2_norm) {
+    setup();
+    double mx = 0, l2 = 0;
+    check(drn_variable_difference(ctx_, (int)i_patch, (int)i_field1, (int)i_var1, (int)i_field2, (int)i_var2, &mx, &l2),
+          "drn_variable_difference");
+    max_norm = (real)mx;
+    l2_norm = (real)l2;
+  }
   int launchForm() const { return launch_; }
 
   // PatchGrid::writeToVtk (patchgrid.cpp:673-695): the donor data of field 0 are

@@ -202,6 +202,13 @@ int drn_core_sums(drn_ctx* ctx, int field, double* out);
 /* out[0] = max CFL number |u|+a times dt/dx (jet main.h:188-210), out[1] = rho min,
  * out[2] = rho max, out[3] = number of non-finite values.  Synchronous. */
 int drn_diagnostics(drn_ctx* ctx, int field, double dt, double* out);
+/* Patch::computeVariableDifference(field1, var1, field2, var2, max_norm, l2_norm)
+ * (patch.cpp:733-746) over all cells of `patch`: max_norm = sqrt(max (b - a)^2),
+ * l2_norm = sqrt(sum (b - a)^2), each square in the state type.  max_norm is the
+ * reference's value bit for bit; the sum is accumulated in FP64 (the reference
+ * accumulates in the state type).  Synchronous. */
+int drn_variable_difference(drn_ctx* ctx, int patch, int field1, int var1, int field2, int var2, double* max_norm,
+                            double* l2_norm);
 
 /* ---- post-processing ----------------------------------------------------- */
 /* CompressibleVariables<PerfectGas>::getScalar / getVector (compressiblevariables.h:81-105)

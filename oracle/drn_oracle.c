@@ -1495,6 +1495,24 @@ void F(grid_ls_cells)(const GRID* g, int p, int layer, int outer, int64_t* cells
 
 void F(grid_set_levelset_op)(GRID* g, int op) { g->ls_op = op == 1 ? 1 : 0; }
 
+/* patch.cpp:733-746: squares, their maximum and their sum in REAL, over every
+ * cell of the patch, then the square roots in REAL */
+void F(grid_variable_difference)(const GRID* g, int p, int f1, int v1, int f2, int v2, double* max_norm,
+                                 double* l2_norm) {
+  const opatch* P = &g->p[p];
+  const REAL* a = fieldp((opatch*)P, f1) + (size_t)v1 * P->N;
+  const REAL* b = fieldp((opatch*)P, f2) + (size_t)v2 * P->N;
+  REAL mx = 0., l2 = 0.;
+  for (size_t i = 0; i < P->N; ++i) {
+    REAL d = b[i] - a[i];
+    REAL diff = d * d;
+    mx = mx > diff ? mx : diff;
+    l2 += diff;
+  }
+  *max_norm = (double)RSQRT(mx);
+  *l2_norm = (double)RSQRT(l2);
+}
+
 /* cpu_levelsetobjectbc.h:58-90 + lsobccompressibleeulerop.h:49-144 (inner cells;
  * operateOuter is a no-op), or with ls_op 1 lsobccompressibleswallop.h:48-116,
  * whose access is the same mirror sum.  The flat order is patch, layer, list

@@ -109,6 +109,9 @@ typedef struct o64_grid o64_grid;
   /* LSOBCCompressibleSWallOp (op 1, lsobccompressibleswallop.h:48-116)                    */ \
   void  PFX##grid_levelset_fill(GRID* g, int field, int abuse_field);                           \
   void  PFX##grid_set_levelset_op(GRID* g, int op);                                             \
+  /* Patch::computeVariableDifference (patch.cpp:733-746), in REAL as the reference */         \
+  void  PFX##grid_variable_difference(const GRID* g, int p, int f1, int v1, int f2, int v2,      \
+                                      double* max_norm, double* l2_norm);                        \
   /* rungekutta.cpp:27-42 (+ level-set fill as post operation when set up) */                   \
   void  PFX##grid_rk_step(GRID* g, REAL dt, int n_alpha, const REAL* alpha);                    \
   void  PFX##grid_set_threads(GRID* g, int n);                                                  \

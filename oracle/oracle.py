@@ -79,6 +79,8 @@ def lib():
                                          C.POINTER(C.c_int64), rp, rp]),
                 "grid_levelset_fill": (None, [C.c_void_p, C.c_int, C.c_int]),
                 "grid_set_levelset_op": (None, [C.c_void_p, C.c_int]),
+                "grid_variable_difference": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                    C.c_void_p, C.c_void_p]),
                 "grid_rk_step": (None, [C.c_void_p, rt, C.c_int, rp]),
                 "grid_set_threads": (None, [C.c_void_p, C.c_int]),
                 "grid_core_sums": (None, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
@@ -280,6 +282,12 @@ class OracleGrid:
 
     def levelset_fill(self, field=0, abuse_field=2):
         self._f("grid_levelset_fill")(self.h, field, abuse_field)
+
+    def variable_difference(self, p, f1, v1, f2, v2):
+        """Patch::computeVariableDifference: (max_norm, l2_norm) in the state type"""
+        mx, l2 = C.c_double(0), C.c_double(0)
+        self._f("grid_variable_difference")(self.h, p, f1, v1, f2, v2, C.byref(mx), C.byref(l2))
+        return mx.value, l2.value
 
     def set_levelset_op(self, op):
         """0: LSOBCCompressibleEulerOp (slip, the default), 1: LSOBCCompressibleSWallOp"""
