@@ -8,6 +8,7 @@
 //
 // usage: drnum_run <grid file> <case file> <out file> <steps> <dt> <mode> [f64] [levelset] [swall] [visc]
 //                  [restart=<file.dnd>] [dnd=<base name>] [codes=<code string>] [vtk=<base name>]
+//                  [cfl=<target>]  (dt rescaled to the target CFL number after each step)
 //   case file (binary, little endian), per patch in grid order:
 //     int32 bc[6], float64 par[6][16], then 5*N initial values in `real`
 //     (variable-major, the layout of field 0 in Patch::m_Data);
@@ -53,6 +54,7 @@ struct Options {
   double dt = 0;
   bool levelset = false;
   bool swall = false;  // the level-set fill with LSOBCCompressibleSWallOp instead of the slip op
+  double cfl = 0;      // > 0: dt rescaled to this CFL number after every step (rk and hip modes)
 };
 
 template <typename real, class OP>
@@ -60,7 +62,8 @@ static int run(const Options& o) {
   const std::string &grid_file = o.grid_file, &case_file = o.case_file, &out_file = o.out_file, &mode = o.mode;
   const std::string &restart = o.restart, &dnd = o.dnd, &codes = o.codes, &vtk = o.vtk;
   const int steps = o.steps;
-  const double dt = o.dt;
+  double dt = o.dt;
+  double t_run = 0;  // simulated time of this run
   const bool levelset = o.levelset;
   PatchGrid<double> grid;
   std::string err;
@@ -150,12 +153,34 @@ static int run(const Options& o) {
   rk.addIterator(&iterator);
   if (ls) rk.addPostOperation(ls);
 
+  // with cfl=<target>, dt *= target / CFL_max after every step, the rescale of
+  // the jet demo (applications/drnumJetDemo/main.h:233), which applies it at
+  // every write interval
+  auto rescale = [&]() {
+    if (o.cfl <= 0) return;
+    const double c = iterator.cflMax(0, dt);
+    if (c > 0) dt *= o.cfl / c;
+    printf("dt %.12e CFL %.12e\n", dt, c);
+  };
+  if (o.cfl > 0 && mode == "graph") {
+    fprintf(stderr, "cfl= needs mode rk or hip (a graph replays one dt)\n");
+    return 2;
+  }
   if (mode == "rk") {
-    for (int s = 0; s < steps; ++s) rk(real(dt));
+    for (int s = 0; s < steps; ++s) {
+      rk(real(dt));
+      t_run += dt;
+      rescale();
+    }
   } else if (mode == "hip") {
-    for (int s = 0; s < steps; ++s) hrk(real(dt));
+    for (int s = 0; s < steps; ++s) {
+      hrk(real(dt));
+      t_run += dt;
+      rescale();
+    }
   } else if (mode == "graph") {
     hrk.graph(real(dt), steps);
+    t_run = steps * dt;
   } else {
     fprintf(stderr, "unknown mode %s\n", mode.c_str());
     return 2;
@@ -182,7 +207,7 @@ static int run(const Options& o) {
   delete ls;
   printf("wrote %s\n", out_file.c_str());
   if (!dnd.empty()) {
-    writeData<real>(pp, 0, dnd, real(t0 + steps * dt), steps);
+    writeData<real>(pp, 0, dnd, real(t0 + t_run), steps);
     printf("wrote restart file %s_%06d.dnd\n", dnd.c_str(), steps);
   }
   if (!vtk.empty()) {
@@ -197,7 +222,7 @@ int main(int argc, char** argv) {
   if (argc < 7) {
     fprintf(stderr,
             "usage: %s <grid> <case> <out> <steps> <dt> <rk|hip|graph> [f64] [levelset] [swall] [visc] [restart=F] [dnd=B] "
-            "[codes=S] [vtk=B]\n",
+            "[codes=S] [vtk=B] [cfl=C]\n",
             argv[0]);
     return 2;
   }
@@ -213,6 +238,7 @@ int main(int argc, char** argv) {
     if (!strcmp(argv[a], "f64")) f64 = true;
     if (!strcmp(argv[a], "levelset")) o.levelset = true;
     if (!strcmp(argv[a], "swall")) o.swall = true;
+    if (!strncmp(argv[a], "cfl=", 4)) o.cfl = atof(argv[a] + 4);
     if (!strcmp(argv[a], "visc")) visc = true;
     if (!strncmp(argv[a], "restart=", 8)) o.restart = argv[a] + 8;
     if (!strncmp(argv[a], "dnd=", 4)) o.dnd = argv[a] + 4;

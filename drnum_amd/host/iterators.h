@@ -368,6 +368,15 @@ class HIP_CartesianIterator : public PatchIterator<real> {
     return choice;
   }
   void sync() { check(drn_sync(ctx_), "drn_sync"); }
+  // the largest CFL number of field i_field for time step dt, max over cells
+  // and axes of (|u_axis| + a) dt / d_axis (the CFL_max loop of the jet demo,
+  // applications/drnumJetDemo/main.h:188-210, which takes the x axis only)
+  double cflMax(size_t i_field, double dt) {
+    setup();
+    double out[4] = {0, 0, 0, 0};
+    check(drn_diagnostics(ctx_, (int)i_field, dt, out), "drn_diagnostics");
+    return out[0];
+  }
   // Patch::computeVariableDifference (patch.cpp:733-746) for the iterator's
   // patch i_patch, on the device: max_norm as the reference computes it, the
   // sum of squares accumulated in FP64

@@ -28,7 +28,7 @@ APP = os.path.join(ROOT, "drnum_amd", "lib", "drnum_run")
 pytestmark = pytest.mark.gpu
 
 
-def run_app(tmp_path, w, dtype, steps, mode, levelset=None, extra=(), tag=""):
+def run_app(tmp_path, w, dtype, steps, mode, levelset=None, extra=(), tag="", stdout=None):
     grid = tmp_path / "grid"
     case = tmp_path / "case.bin"
     out = tmp_path / ("out_%s%s.bin" % (mode, tag))
@@ -43,6 +43,8 @@ def run_app(tmp_path, w, dtype, steps, mode, levelset=None, extra=(), tag=""):
     print("drnum_run", mode, np.dtype(dtype).name, *extra, flush=True)
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
+    if stdout is not None:
+        stdout.append(r.stdout)
     raw = np.frombuffer(out.read_bytes(), dtype=dtype)
     states, o = [], 0
     for P in w.patches:
@@ -196,3 +198,42 @@ def test_dropin_vtk_output(gpu, tmp_path, dtype):
         np.testing.assert_allclose(r["points"][-1, -1, -1],
                                    [P.origin[a] + (n - s[2 * a + 1]) * h[a]
                                     for a, n in enumerate((P.ni, P.nj, P.nk))], rtol=1e-6, atol=1e-7)
+
+
+def cfl_numpy(w, states, dt, dtype):
+    """max over cells and axes of (|u_axis| + a) dt / d_axis, from the conserved state"""
+    from drnum_amd import solver
+
+    best = 0.0
+    for P, U in zip(w.patches, states):
+        U = U.astype(np.float64)
+        u = U[1:4] / U[0]
+        T = (U[4] / U[0] - 0.5 * (u * u).sum(axis=0)) / 717.5
+        a = np.sqrt(1.4 * 287.2 * T)
+        d = np.array(solver.patch_spacing(P, dtype), dtype=np.float64)
+        best = max(best, float(((np.abs(u) + a) * dt / d[:, None]).max()))
+    return best
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_dropin_cfl_time_step(gpu, tmp_path, dtype):
+    """cfl=<target>: after every step dt *= target / CFL_max (the rescale of the
+    jet demo, main.h:233).  The printed CFL of the last step is the CFL of the
+    written state at the dt of that step, recomputed here with numpy; rk and hip
+    modes take the same dt sequence and end in the same state."""
+    w = configs.jet(core=8, blocks=2)
+    outs = {}
+    for m in ("rk", "hip"):
+        log = []
+        st = run_app(tmp_path, w, dtype, 4, m, extra=("cfl=0.3",), tag="cfl", stdout=log)
+        lines = [ln.split() for ln in log[0].splitlines() if ln.startswith("dt ")]
+        assert len(lines) == 4
+        dts = [float(w.dt)] + [float(ln[1]) for ln in lines]
+        cfls = [float(ln[3]) for ln in lines]
+        for n in range(4):
+            assert dts[n + 1] == pytest.approx(dts[n] * 0.3 / cfls[n], rel=1e-8)
+        assert cfls[-1] == pytest.approx(cfl_numpy(w, st, dts[-2], dtype), rel=1e-5)
+        outs[m] = (dts, st)
+    assert outs["rk"][0] == outs["hip"][0]
+    for a, b in zip(outs["rk"][1], outs["hip"][1]):
+        assert np.array_equal(a, b)
