@@ -291,3 +291,21 @@ def test_autotune_keeps_state(gpu):
         og.rk_step()
         gg.rk_step()
     assert rel_linf(gg.state(0), og) <= TOL[np.float32]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("tiling", [11, 27, 35, 33, 0, 12, 22, 29, 31])
+def test_upwind1_rk_steps(gpu, dtype, tiling):
+    """First-order Upwind1 (upwind1.h:29-39) through every kernel family: the Sod
+    tube on one patch and the smooth shear layer over 8 patches (exchange every
+    stage, fused where the kernel fuses), 10 RK steps against the oracle."""
+    for w in (configs.sod(n=16), smooth_shear()):
+        w.recon = configs.RECON_UPWIND1
+        og, gg = make(w, dtype)
+        gg.ctx.set_tiling(tiling)
+        for s in range(10):
+            og.rk_step()
+            gg.rk_step()
+        err = rel_linf(gg.state(0), og)
+        print("upwind1", w.name, "tiling", tiling, np.dtype(dtype).name, err, flush=True)
+        assert err <= TOL[dtype]
