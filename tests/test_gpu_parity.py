@@ -309,3 +309,42 @@ def test_upwind1_rk_steps(gpu, dtype, tiling):
         err = rel_linf(gg.state(0), og)
         print("upwind1", w.name, "tiling", tiling, np.dtype(dtype).name, err, flush=True)
         assert err <= TOL[dtype]
+
+
+FAMILY_TILINGS = [11, 27, 35, 33, 0, 12, 22, 29, 31]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("limiter", [configs.LIM_MINMOD, configs.LIM_SECONDORDER])
+@pytest.mark.parametrize("tiling", FAMILY_TILINGS)
+def test_limiters_through_every_kernel_family(gpu, dtype, limiter, tiling):
+    """minmod (minmod.h:28-31) and the unlimited second-order slopes
+    (secondorder.h:28-31) through one tiling of each kernel family, 10 RK
+    steps of the 8-patch smooth shear (and of the Sod tube for minmod)."""
+    cases = [smooth_shear()] + ([configs.sod(n=16)] if limiter == configs.LIM_MINMOD else [])
+    for w in cases:
+        w.limiter = limiter
+        og, gg = make(w, dtype)
+        gg.ctx.set_tiling(tiling)
+        for s in range(10):
+            og.rk_step()
+            gg.rk_step()
+        err = rel_linf(gg.state(0), og)
+        print("limiter", limiter, w.name, "tiling", tiling, np.dtype(dtype).name, err, flush=True)
+        assert err <= TOL[dtype]
+
+
+@pytest.mark.parametrize("tiling", FAMILY_TILINGS)
+def test_fp32_ieee_arithmetic_through_every_kernel_family(gpu, tiling):
+    """FP32 with IEEE division and square root (DRN_ARITH_IEEE) through one tiling
+    of each kernel family (the families without an IEEE form run the cell kernel)."""
+    w = smooth_shear()
+    og, gg = make(w, np.float32)
+    gg.ctx.set_arith(hip.DRN_ARITH_IEEE)
+    gg.ctx.set_tiling(tiling)
+    for s in range(10):
+        og.rk_step()
+        gg.rk_step()
+    err = rel_linf(gg.state(0), og)
+    print("ieee fp32 tiling", tiling, err, flush=True)
+    assert err <= TOL[np.float32]
