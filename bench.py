@@ -19,6 +19,9 @@ is one RK step over all patches.
               instructions per launch (committed rocprofv3 summary) / CUs /
               launch time, against the FP32 VALU issue rate measured by
               drnum_amd/lib/valu_probe after the timed region.
+  pcie_inclusive the same steps with one upload of the whole state before them
+              and one download after them (drn_patch_upload/download, pageable
+              host buffers), as a caller that keeps the state on the host runs.
   cpu_baseline the oracle (C restatement of the reference iterator, OpenMP over
               patches) on this host, one RK step of the same workload; its
               single_thread entry is one stage of the same grid on one thread.
@@ -140,6 +143,24 @@ def run_form(w, dtype, steps, warmup, launch, label, tiling=None):
         tg1 = time.perf_counter()
         res["graph_ms_per_step"] = 1e3 * (tg1 - tg0) / steps
         res["graph_mcups"] = cells * S * steps / (tg1 - tg0) / 1e6
+    # host hand-over through PCIe (drn_patch_upload / drn_patch_download of every
+    # patch, pageable host buffers): one upload of the whole state before a run
+    # and one download after it, as a caller that keeps the state on the host does
+    if launch == hip.DRN_LAUNCH_BATCHED:
+        host_states = [w.initial_state(p, dtype) for p in range(len(w.patches))]
+        ctx.sync()
+        tu0 = time.perf_counter()
+        for p, a in enumerate(host_states):
+            ctx.upload(p, 0, a)
+        ctx.sync()
+        tu1 = time.perf_counter()
+        for p in range(len(w.patches)):
+            ctx.download(p, 0)
+        ctx.sync()
+        tu2 = time.perf_counter()
+        res["upload_s"], res["download_s"] = tu1 - tu0, tu2 - tu1
+        res["state_bytes"] = sum(a.nbytes for a in host_states)
+        del host_states
     # STREAM-style copy of one state buffer on the same device
     ctx.copy_field(0, 2)
     ctx.sync()
@@ -503,6 +524,18 @@ def main():
             "ms_per_step": round(f32pp["ms_per_step"], 4),
             "tiling": f32pp["tiling"],
             "batched_over_per_patch": round(f32["mcups"] / f32pp["mcups"], 3),
+        },
+        "pcie_inclusive": {
+            # never the value: the same K steps with one upload of the whole
+            # state before them and one download after them
+            "value": round(len(w.alpha) * w.total_cells * args.steps
+                           / (f32["wall"] + f32["upload_s"] + f32["download_s"]) / 1e6, 2),
+            "unit": "Mcell-updates/s",
+            "upload_ms": round(1e3 * f32["upload_s"], 3),
+            "download_ms": round(1e3 * f32["download_s"], 3),
+            "state_bytes": f32["state_bytes"],
+            "sample": "%d steps between one upload and one download of the FP32 state (pageable host buffers)"
+                      % args.steps,
         },
         "stage_kernel_ms": round(f32["stage_ms"], 4),
         "exchange_ms": round(f32["exch_ms"], 4),
