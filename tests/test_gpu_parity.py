@@ -348,3 +348,43 @@ def test_fp32_ieee_arithmetic_through_every_kernel_family(gpu, tiling):
     err = rel_linf(gg.state(0), og)
     print("ieee fp32 tiling", tiling, err, flush=True)
     assert err <= TOL[np.float32]
+
+
+def z_wave(zc):
+    """Two periodic patches stacked along z, 5 x 4 x zc core cells each (nk =
+    zc + 4 stored cells with the seek layers), smooth in all three directions
+    (Mach about 0.4).  A wavefront of the z-shared kernels holds 63 consecutive
+    cells of the flattened (j,k) plane and takes the z neighbours of its cells
+    from the neighbouring lanes; nk decides where the wavefront edges (lanes 0,
+    1 and 63, which load the neighbours outside it) fall along the rows."""
+    patches = configs.block_grid([5, 4, 2 * zc], [1, 1, 2], [1.0] * 3, 2, [True] * 3, [configs.BC_SLIP] * 6)
+
+    def init(x, y, z):
+        p = configs.P_AMB * (1.0 + 0.05 * np.sin(2.0 * np.pi * z))
+        T = p / ((1.0 + 0.1 * np.cos(2.0 * np.pi * (x + z))) * configs.gas.R)
+        u = 0.2 * np.sin(2.0 * np.pi * z)
+        v = 0.1 * np.cos(2.0 * np.pi * x)
+        w = 0.1 + 0.2 * np.sin(2.0 * np.pi * y)
+        return np.stack([p, T, u, v, w])
+
+    dt = 0.4 * (1.0 / (2 * zc)) / 1.5
+    return configs.Workload("zwave%d" % zc, patches, configs.RECON_UPWIND2, configs.LIM_VANALBADA, [1.0] * 3,
+                            [0.25, 0.5, 1.0], dt, init)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("zc", [3, 57, 59, 60, 61, 62, 63, 123])
+def test_z_neighbours_across_wavefront_edges(gpu, dtype, zc):
+    """The z-shared kernels (cell, push-chain list, persistent) against the
+    oracle, 5 RK steps, for z lengths that put the wavefront edges at every
+    position relative to the rows and the patch ends."""
+    w = z_wave(zc)
+    for tiling in (27, 35, 33):
+        og, gg = make(w, dtype)
+        gg.ctx.set_tiling(tiling)
+        for s in range(5):
+            og.rk_step()
+            gg.rk_step()
+        err = rel_linf(gg.state(0), og)
+        print("z lanes nk", zc + 4, "tiling", tiling, np.dtype(dtype).name, err, flush=True)
+        assert err <= TOL[dtype]
