@@ -409,14 +409,16 @@ __device__ __forceinline__ void visc_face(const T* vl, const T* vr, const T* gla
   }
 }
 
-// perfectgas.h:83-92
-template <typename T>
+// perfectgas.h:83-92; the fast form takes the hardware reciprocal of rho and
+// multiplies by 1/cv, as the interior faces do
+template <typename T, bool FAST = false>
 __device__ __forceinline__ void cons_to_prim(const T* var, T& p, T& Tt, T& u, T& v, T& w) {
-  T ir = T(1) / var[0];
+  T ir = drcp<T, FAST>(var[0]);
   u = var[1] * ir;
   v = var[2] * ir;
   w = var[3] * ir;
-  Tt = (var[4] * ir - T(0.5) * (u * u + v * v + w * w)) / Gas<T>::CV;
+  const T e = var[4] * ir - T(0.5) * (u * u + v * v + w * w);
+  Tt = FAST ? e * (T(1) / Gas<T>::CV) : e / Gas<T>::CV;
   p = var[0] * Gas<T>::R * Tt;
 }
 
@@ -441,12 +443,13 @@ __device__ __forceinline__ void wall_slip(const T* var, int dir, T A, T* f) {
 }
 
 // zero-gradient outflow: Euler flux of the boundary cell (no counterpart in the reference)
-template <typename T>
+template <typename T, bool FAST = false>
 __device__ __forceinline__ void wall_transmissive(const T* var, int dir, T A, T* f) {
   T r = var[0];
-  T ir = T(1) / r;
+  T ir = drcp<T, FAST>(r);
   T vel[3] = {var[1] * ir, var[2] * ir, var[3] * ir};
-  T Tt = (var[4] * ir - T(0.5) * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2])) / Gas<T>::CV;
+  const T e = var[4] * ir - T(0.5) * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
+  T Tt = FAST ? e * (T(1) / Gas<T>::CV) : e / Gas<T>::CV;
   T p = r * Gas<T>::R * Tt;
   T m = A * var[1 + dir];
   f[0] += m;
@@ -456,27 +459,27 @@ __device__ __forceinline__ void wall_transmissive(const T* var, int dir, T A, T*
 }
 
 // compressiblefarfieldflux.h:101-105 / jetflux.h:186-190
-template <typename T>
+template <typename T, bool FAST = false>
 __device__ __forceinline__ void wall_state_flux(T p, T Tt, const T* vel, int dir, T A, T* f) {
   T un = vel[dir];
-  f[0] = A * p / (Gas<T>::R * Tt) * un;
+  f[0] = (FAST ? A * p * drcp<T, FAST>(Gas<T>::R * Tt) : A * p / (Gas<T>::R * Tt)) * un;
 #pragma unroll
   for (int c = 0; c < 3; ++c) f[1 + c] = f[0] * vel[c] + (c == dir ? A * p : T(0));
   f[4] = f[0] * (Gas<T>::CP * Tt + T(0.5) * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]));
 }
 
 // compressiblefarfieldflux.h:26-65,93-181 with the Upwind1 projection (var1 = boundary cell)
-template <typename T>
+template <typename T, bool FAST = false>
 __device__ __forceinline__ void wall_farfield(const double* par, const T* var1, int dir, bool plus, T A, T* f) {
   T far[5] = {T(par[0]), T(par[1]), T(par[2]), T(par[3]), T(par[4])};
   T p0, T0, u0, v0, w0, p1, T1, u1, v1, w1;
-  cons_to_prim(far, p0, T0, u0, v0, w0);
-  cons_to_prim(var1, p1, T1, u1, v1, w1);
+  cons_to_prim<T, FAST>(far, p0, T0, u0, v0, w0);
+  cons_to_prim<T, FAST>(var1, p1, T1, u1, v1, w1);
   T a0 = dsqrt(Gas<T>::GAMMA * Gas<T>::R * T0);
   T a1 = dsqrt(Gas<T>::GAMMA * Gas<T>::R * T1);
   T un0 = dir == 0 ? u0 : (dir == 1 ? v0 : w0);
   T un1 = dir == 0 ? u1 : (dir == 1 ? v1 : w1);
-  T M0 = un0 / a0, M1 = un1 / a1;
+  T M0 = ddiv<T, FAST>(un0, a0), M1 = ddiv<T, FAST>(un1, a1);
   if (plus) {
     M0 = -M0;
     M1 = -M1;
@@ -496,7 +499,7 @@ __device__ __forceinline__ void wall_farfield(const double* par, const T* var1, 
     vel[1] = v1;
     vel[2] = w1;
   }
-  wall_state_flux(p, Tt, vel, dir, A, f);
+  wall_state_flux<T, FAST>(p, Tt, vel, dir, A, f);
 }
 
 __device__ __forceinline__ uint64_t smix64(uint64_t x) {
@@ -517,11 +520,11 @@ __device__ __forceinline__ T jet_noise(uint64_t seed, int64_t ga, int64_t gb, in
 }
 
 // jetflux.h:158-192 generalised: top-hat inflow inside the radius, far state outside
-template <typename T>
+template <typename T, bool FAST = false>
 __device__ __forceinline__ void wall_jet(const double* par, const T* var, int dir, bool plus, T ta, T tb,
                                          int64_t la, int64_t lb, T A, T* f) {
   T p, Tt, u, v, w;
-  cons_to_prim(var, p, Tt, u, v, w);
+  cons_to_prim<T, FAST>(var, p, Tt, u, v, w);
   T ca = T(par[0]), cb = T(par[1]), rad = T(par[2]);
   T rr = (ta - ca) * (ta - ca) + (tb - cb) * (tb - cb);
   T sgn = plus ? T(-1) : T(1);
@@ -541,7 +544,7 @@ __device__ __forceinline__ void wall_jet(const double* par, const T* var, int di
     vel[dir] = sgn * T(par[5]);
     Tt = T(par[6]);
   }
-  wall_state_flux(p, Tt, vel, dir, A, f);
+  wall_state_flux<T, FAST>(p, Tt, vel, dir, A, f);
 }
 
 // one face functor call; `var` is the cell inside the face
@@ -550,9 +553,9 @@ __device__ __forceinline__ void wall_flux(int code, const double* par, const T* 
                                           int64_t la, int64_t lb, T A, T* f) {
   switch (code) {
     case BC_SLIP: wall_slip<T, FAST>(var, dir, A, f); break;
-    case BC_TRANSMISSIVE: wall_transmissive(var, dir, A, f); break;
-    case BC_FARFIELD: wall_farfield(par, var, dir, plus, A, f); break;
-    case BC_JET: wall_jet(par, var, dir, plus, ta, tb, la, lb, A, f); break;
+    case BC_TRANSMISSIVE: wall_transmissive<T, FAST>(var, dir, A, f); break;
+    case BC_FARFIELD: wall_farfield<T, FAST>(par, var, dir, plus, A, f); break;
+    case BC_JET: wall_jet<T, FAST>(par, var, dir, plus, ta, tb, la, lb, A, f); break;
     default: break;
   }
 }
