@@ -151,7 +151,10 @@ static int run(const Options& o) {
     hrk.addAlpha(a);
   }
   rk.addIterator(&iterator);
-  if (ls) rk.addPostOperation(ls);
+  if (ls) {
+    rk.addPostOperation(ls);
+    hrk.addPostOperation(ls);
+  }
 
   // with cfl=<target>, dt *= target / CFL_max after every step, the rescale of
   // the jet demo (applications/drnumJetDemo/main.h:233), which applies it at

@@ -162,7 +162,7 @@ T_AMB = P_AMB / (1.0 * gas.R)   # rho = 1, so a = sqrt(1.4 R T) = 1
 
 
 def jet(core: int = 64, blocks: int = 4, steps: int = 1000, mach: float = 0.9, radius_cells: float = 16.0,
-        noise: float = 0.01, seed: int = 1234, coflow: float = 0.1) -> Workload:
+        noise: float = 0.01, seed: int = 1234, coflow: float = 0.1, p_jet: float = P_AMB) -> Workload:
     """Subsonic top-hat jet entering through the z=0 face of a (core*blocks)^3 box.
     Ambient rho=1, p=1/1.4 (a=1) at rest; inflow |w| = mach inside radius_cells
     of the face centre with seeded +/-noise velocity perturbation (fixed in
@@ -171,7 +171,10 @@ def jet(core: int = 64, blocks: int = 4, steps: int = 1000, mach: float = 0.9, r
     jetflux.h:180-185); all other domain faces transmissive; 2 seek layers on
     interior patch faces.  With coflow = 0 the rest of the face is a wall with
     zero normal velocity, and the cells beside the nozzle lip are drained to
-    negative density within a few dozen steps of the impulsive start."""
+    negative density within a few dozen steps of the impulsive start.  As in
+    the jet demo (main.h:128, jetflux.h:176-178), a jet or co-flow faster than
+    sound imposes p_jet inside the radius; a subsonic jet takes p from the
+    boundary cell."""
     nc = core * blocks
     h = 1.0 / nc
     dom_bc = [BC_TRANSMISSIVE] * 6
@@ -191,6 +194,8 @@ def jet(core: int = 64, blocks: int = 4, steps: int = 1000, mach: float = 0.9, r
             par[8] = float(seed)
             par[9] = P.gidx[0]
             par[10] = P.gidx[1]
+            par[11] = p_jet
+            par[12] = 1.0 if max(mach, coflow) > 1.0 else 0.0
 
     def init(x, y, z):
         one = np.ones_like(x)

@@ -2129,55 +2129,82 @@ __global__ __launch_bounds__(256) void k_sums_final(uint32_t nblk, const double*
   }
 }
 
-// diagnostics partials: max CFL, min rho, max rho, count of non-finite values
+// The CFL number of one cell as the jet demo computes it
+// (applications/drnumJetDemo/main.h:196-205): PerfectGas::conservativeToPrimitive
+// (perfectgas.h:84-92) and a = sqrt(gamma R T) in the state type, with the
+// reference's double-literal promotions (1.0/rho, 0.5*|u|^2, /cv), then
+// max(|u|, |u+a|, |u-a|) * dt / dx with the x velocity and spacing only.
+template <typename T>
+__device__ __forceinline__ T cfl_x_reference(const T* var, T dt, T dx) {
+#pragma clang fp contract(off)
+  const T ir = T(1.0 / double(var[0]));
+  const T u = var[1] * ir, v = var[2] * ir, w = var[3] * ir;
+  const T Tt = T((double(var[4] * ir) - 0.5 * double(u * u + v * v + w * w)) / double(Gas<T>::CV));
+  const T a = dsqrt(Gas<T>::GAMMA * Gas<T>::R * Tt);
+  T c = dabs(u) * dt / dx;
+  c = fmax(c, dabs(u + a) * dt / dx);
+  c = fmax(c, dabs(u - a) * dt / dx);
+  return c;
+}
+
+// diagnostics partials per tile: [0] the reference CFL (x axis only), [1] min
+// rho, [2] max rho, [3] count of non-finite values, [4] the CFL over all three
+// axes, max_d (|u_d| + a) dt / d_d, evaluated in FP64
 template <typename T>
 __global__ __launch_bounds__(256) void k_diag_partial(const PatchDev* __restrict__ pds, const TileCell* __restrict__ tiles,
                                                       const T* __restrict__ U, uint64_t vs, double dt,
                                                       double* __restrict__ part) {
-  __shared__ double red[4][4];
+  __shared__ double red[4][5];
   const TileCell t = tiles[blockIdx.x];
   const PatchDev& P = pds[t.patch];
   const int c2 = t.c2 + (int)threadIdx.x;
-  double cfl = 0, rmin = 1e300, rmax = -1e300, bad = 0;
+  double cflx = 0, cfl3 = 0, rmin = 1e300, rmax = -1e300, bad = 0;
   if (c2 < P.njk) {
     const uint64_t c = (uint64_t)P.off + (uint64_t)t.i * P.njk + c2;
+    T tv[5];
     double var[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      var[v] = double(U[v * vs + c]);
+      tv[v] = U[v * vs + c];
+      var[v] = double(tv[v]);
       if (!isfinite(var[v])) bad += 1;
     }
+    cflx = double(cfl_x_reference<T>(tv, T(dt), T(P.dx)));
+    if (!isfinite(cflx)) cflx = 0;
     double ir = 1.0 / var[0];
     double u = var[1] * ir, v = var[2] * ir, w = var[3] * ir;
     double Tt = (var[4] * ir - 0.5 * (u * u + v * v + w * w)) / 717.5;
     double a = sqrt(1.4 * 287.2 * Tt);
-    cfl = fmax(fmax((fabs(u) + a) * dt / P.dx, (fabs(v) + a) * dt / P.dy), (fabs(w) + a) * dt / P.dz);
-    if (!isfinite(cfl)) cfl = 0;
+    cfl3 = fmax(fmax((fabs(u) + a) * dt / P.dx, (fabs(v) + a) * dt / P.dy), (fabs(w) + a) * dt / P.dz);
+    if (!isfinite(cfl3)) cfl3 = 0;
     rmin = var[0];
     rmax = var[0];
   }
-  cfl = wave_max(cfl);
+  cflx = wave_max(cflx);
+  cfl3 = wave_max(cfl3);
   rmin = wave_min(rmin);
   rmax = wave_max(rmax);
   bad = wave_sum(bad);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) {
-    red[wv][0] = cfl;
+    red[wv][0] = cflx;
     red[wv][1] = rmin;
     red[wv][2] = rmax;
     red[wv][3] = bad;
+    red[wv][4] = cfl3;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double o[4] = {red[0][0], red[0][1], red[0][2], red[0][3]};
+    double o[5] = {red[0][0], red[0][1], red[0][2], red[0][3], red[0][4]};
     for (int q = 1; q < 4; ++q) {
       o[0] = fmax(o[0], red[q][0]);
       o[1] = fmin(o[1], red[q][1]);
       o[2] = fmax(o[2], red[q][2]);
       o[3] += red[q][3];
+      o[4] = fmax(o[4], red[q][4]);
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) part[(uint64_t)blockIdx.x * 4 + q] = o[q];
+    for (int q = 0; q < 5; ++q) part[(uint64_t)blockIdx.x * 5 + q] = o[q];
   }
 }
 

@@ -519,7 +519,8 @@ __device__ __forceinline__ T jet_noise(uint64_t seed, int64_t ga, int64_t gb, in
   return T(q) * T(1.0 / 8388608.0);
 }
 
-// jetflux.h:158-192 generalised: top-hat inflow inside the radius, far state outside
+// jetflux.h:158-192 generalised to any face: top-hat inflow inside the radius
+// (p_jet when the jet is supersonic, else p of the cell), far state outside
 template <typename T, bool FAST = false>
 __device__ __forceinline__ void wall_jet(const double* par, const T* var, int dir, bool plus, T ta, T tb,
                                          int64_t la, int64_t lb, T A, T* f) {
@@ -539,6 +540,7 @@ __device__ __forceinline__ void wall_jet(const double* par, const T* var, int di
     vel[dir] = sgn * (uj * (T(1) + amp * nn));
     vel[axa] = uj * (amp * na);
     vel[axb] = uj * (amp * nb);
+    if (par[12] != 0.0) p = T(par[11]);  // jetflux.h:176-178: a supersonic jet imposes p_jet
     Tt = T(par[4]);
   } else {
     vel[dir] = sgn * T(par[5]);

@@ -136,12 +136,23 @@ struct drn_ctx {
   bool timing = false;
   TimerSlot timers[4];
 
-  // graph
-  hipGraphExec_t gexec = nullptr;
-  double g_dt = 0;
-  std::vector<double> g_alpha;
-  int g_steps = 0;
-  bool first_step = true;
+  // Captured step cycles, one per starting field map: a graph holds the
+  // physical buffers of the map it was captured from, so it is replayed only
+  // from that map.
+  struct GraphEntry {
+    int map[NBUF];
+    double dt;
+    std::vector<double> alpha;
+    int steps;      // steps per replay (the cycle of the field-map permutation)
+    int exch_end;   // exch_buf after one replay
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
+  // The physical buffer whose receivers hold the exchange of its current
+  // donor values (-1: none).  Every write to a buffer clears it for that
+  // buffer; a step without post operation skips its leading exchange
+  // (rungekutta.cpp:30) only when it is set for the state buffer.
+  int exch_buf = -1;
 
   size_t esize() const { return prec == DRN_F32 ? 4 : 8; }
 };
@@ -154,6 +165,16 @@ struct drn_ctx {
       return DRN_E_HIP;                                                              \
     }                                                                                \
   } while (0)
+
+static void drop_graph(drn_ctx* ctx) {
+  for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
+  ctx->graphs.clear();
+}
+
+// buffer b is written: its receivers no longer hold the exchange
+static void wrote(drn_ctx* ctx, int b) {
+  if (ctx->exch_buf == b) ctx->exch_buf = -1;
+}
 
 static int fail(drn_ctx* ctx, int code, const std::string& msg) {
   if (ctx) ctx->err = msg;
@@ -442,6 +463,7 @@ static bool push_active(const drn_ctx* ctx, int launch) {
 }
 
 static int do_stage(drn_ctx* ctx, int psrc, int pbase, int pdst, double factor, int launch, bool push = false) {
+  wrote(ctx, pdst);
   hipEvent_t stop;
   timer_begin(ctx, 0, &stop);
   ctx->push_now = push;
@@ -608,6 +630,8 @@ static int do_exchange(drn_ctx* ctx, int pbuf, bool partial = false) {
     launch_exchange_t<double>(ctx, x, (double*)ctx->buf[pbuf]);
   timer_end(ctx, stop);
   HIPCHK(ctx, hipGetLastError());
+  // after a stage that filled the fused receivers, the partial lists complete the exchange
+  ctx->exch_buf = pbuf;
   return DRN_OK;
 }
 
@@ -626,6 +650,7 @@ static void launch_ls_t(drn_ctx* ctx, T* U, T relax) {
 
 static int do_ls_fill(drn_ctx* ctx, int pbuf, double relax) {
   if (ctx->ls_n == 0) return DRN_OK;
+  wrote(ctx, pbuf);
   hipEvent_t stop;
   timer_begin(ctx, 2, &stop);
   if (ctx->prec == DRN_F32)
@@ -649,10 +674,9 @@ static int do_rk_step(drn_ctx* ctx, double dt, int n_alpha, const double* alpha,
   int Z = 0;
   while (Z == X || Z == Y) ++Z;
   int rc;
-  if (ctx->first_step || post) {
+  if (post || ctx->exch_buf != X) {
     if ((rc = do_exchange(ctx, X))) return rc;
   }
-  ctx->first_step = false;
   int cur = X;
   // the stage fills the fused z-face receivers, the exchange after it the rest
   const bool push = push_active(ctx, launch);
@@ -740,7 +764,7 @@ void drn_ctx_destroy(drn_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  drop_graph(ctx);
   free_exchange(ctx);
   free_ls(ctx);
   for (int b = 0; b < NBUF; ++b) dfree(ctx->buf[b]);
@@ -795,10 +819,7 @@ int drn_set_operator(drn_ctx* ctx, int recon, int limiter) {
     int rc = build_march(ctx);
     if (rc) return rc;
   }
-  if (ctx->gexec) {
-    (void)hipGraphExecDestroy(ctx->gexec);
-    ctx->gexec = nullptr;
-  }
+  drop_graph(ctx);
   return DRN_OK;
 }
 
@@ -812,10 +833,7 @@ int drn_set_viscous(drn_ctx* ctx, int on) {
     int rc = build_march(ctx);
     if (rc) return rc;
   }
-  if (ctx->gexec) {
-    (void)hipGraphExecDestroy(ctx->gexec);
-    ctx->gexec = nullptr;
-  }
+  drop_graph(ctx);
   return DRN_OK;
 }
 
@@ -1008,6 +1026,7 @@ int drn_patch_upload(drn_ctx* ctx, int id, int field, const void* host) {
   if (id < 0 || id >= (int)ctx->descs.size()) return fail(ctx, DRN_E_ARG, "patch id out of range");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const size_t n = drn_patch_cells(ctx, id), es = ctx->esize();
+  wrote(ctx, ctx->map[field]);
   char* b = (char*)ctx->buf[ctx->map[field]];
   for (int v = 0; v < 5; ++v)
     HIPCHK(ctx, hipMemcpyAsync(b + (v * ctx->vstride + ctx->offs[id]) * es, (const char*)host + v * n * es, n * es,
@@ -1039,6 +1058,7 @@ int drn_copy_field(drn_ctx* ctx, int src, int dst) {
   HIPCHK(ctx, hipSetDevice(ctx->device));
   // buffers hold 5*vstride values, vstride a multiple of 64 cells: a whole number of 16-byte units
   const uint64_t n16 = 5 * ctx->vstride * ctx->esize() / 16;
+  wrote(ctx, ctx->map[dst]);
   hipEvent_t stop;
   timer_begin(ctx, 3, &stop);
   hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, ctx->stream, n16, (const uint4*)ctx->buf[ctx->map[src]],
@@ -1128,13 +1148,6 @@ static int upload_weights(drn_ctx* ctx, void** dptr, const std::vector<double>& 
 // the patch that holds global cell g
 static int patch_of(const drn_ctx* ctx, uint64_t g) {
   return (int)(std::upper_bound(ctx->offs.begin(), ctx->offs.end(), g) - ctx->offs.begin()) - 1;
-}
-
-static void drop_graph(drn_ctx* ctx) {
-  if (ctx->gexec) {
-    (void)hipGraphExecDestroy(ctx->gexec);
-    ctx->gexec = nullptr;
-  }
 }
 
 // The z-shared list of tiling value xcd = 2: patches joined by fused z-face
@@ -1467,19 +1480,17 @@ int drn_rk_graph_run(drn_ctx* ctx, double dt, int n_alpha, const double* alpha, 
   if (!ctx->laid_out) return fail(ctx, DRN_E_STATE, "drn_layout not called");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   if (n_steps == 0) return DRN_OK;
-  if (ctx->first_step) {
-    // the leading exchange of the very first step runs eagerly
-    int rc = do_exchange(ctx, ctx->map[0]);
-    if (rc) return rc;
-    ctx->first_step = false;
+  int rc;
+  // A replay starts from an exchanged state, so that the captured steps may
+  // skip the leading exchange as drn_rk_step does; it runs eagerly when needed.
+  if (ctx->exch_buf != ctx->map[0]) {
+    if ((rc = do_exchange(ctx, ctx->map[0]))) return rc;
   }
-  // A step permutes the three buffers; three steps (or one, when the stage
-  // count brings the mapping back) form a cycle that is captured once.
-  std::vector<double> al(alpha, alpha + n_alpha);
-  const bool same = ctx->gexec && ctx->g_dt == dt && ctx->g_alpha == al;
+  // A step permutes the three buffers; the steps until the map returns to the
+  // start (two for an odd stage count, one for an even one) form the cycle
+  // that is captured, once per starting map.
   int map0[NBUF];
   memcpy(map0, ctx->map, sizeof(map0));
-  // find cycle length of the permutation
   int cyc = 0;
   {
     int m[NBUF];
@@ -1496,38 +1507,56 @@ int drn_rk_graph_run(drn_ctx* ctx, double dt, int n_alpha, const double* alpha, 
       if (memcmp(m, map0, sizeof(m)) == 0) break;
     }
   }
-  if (!same || ctx->g_steps != cyc) {
-    if (ctx->gexec) {
-      (void)hipGraphExecDestroy(ctx->gexec);
-      ctx->gexec = nullptr;
-    }
-    const bool was_timing = ctx->timing;
-    ctx->timing = false;
-    hipGraph_t graph;
-    HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int rc = DRN_OK;
-    for (int s = 0; s < cyc && rc == DRN_OK; ++s) rc = do_rk_step(ctx, dt, n_alpha, alpha, DRN_LAUNCH_BATCHED);
-    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
-    ctx->timing = was_timing;
-    memcpy(ctx->map, map0, sizeof(map0));
-    if (rc) return rc;
-    if (e != hipSuccess) return fail(ctx, DRN_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-    e = hipGraphInstantiate(&ctx->gexec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) return fail(ctx, DRN_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-    ctx->g_dt = dt;
-    ctx->g_alpha = al;
-    ctx->g_steps = cyc;
-  }
   int done = 0;
-  while (done + cyc <= n_steps) {
-    HIPCHK(ctx, hipGraphLaunch(ctx->gexec, ctx->stream));
-    done += cyc;
+  if (n_steps >= cyc) {
+    const std::vector<double> al(alpha, alpha + n_alpha);
+    int gi = -1;
+    for (size_t q = 0; q < ctx->graphs.size(); ++q) {
+      const drn_ctx::GraphEntry& e = ctx->graphs[q];
+      if (memcmp(e.map, map0, sizeof(map0)) == 0 && e.dt == dt && e.alpha == al && e.steps == cyc) gi = (int)q;
+    }
+    if (gi < 0) {
+      // graphs of another step size or other coefficients are not replayed again
+      if (!ctx->graphs.empty() && (ctx->graphs[0].dt != dt || ctx->graphs[0].alpha != al)) drop_graph(ctx);
+      const bool was_timing = ctx->timing;
+      const int exch0 = ctx->exch_buf;
+      ctx->timing = false;
+      hipGraph_t graph;
+      HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      rc = DRN_OK;
+      for (int s = 0; s < cyc && rc == DRN_OK; ++s) rc = do_rk_step(ctx, dt, n_alpha, alpha, DRN_LAUNCH_BATCHED);
+      hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+      const int exch_end = ctx->exch_buf;
+      ctx->timing = was_timing;
+      memcpy(ctx->map, map0, sizeof(map0));
+      ctx->exch_buf = exch0;
+      if (rc) {
+        if (e == hipSuccess) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      if (e != hipSuccess) return fail(ctx, DRN_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+      drn_ctx::GraphEntry g;
+      memcpy(g.map, map0, sizeof(map0));
+      g.dt = dt;
+      g.alpha = al;
+      g.steps = cyc;
+      g.exch_end = exch_end;
+      g.exec = nullptr;
+      e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) return fail(ctx, DRN_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+      ctx->graphs.push_back(g);
+      gi = (int)ctx->graphs.size() - 1;
+    }
+    const drn_ctx::GraphEntry& g = ctx->graphs[(size_t)gi];
+    for (; done + cyc <= n_steps; done += cyc) {
+      HIPCHK(ctx, hipGraphLaunch(g.exec, ctx->stream));
+      ctx->exch_buf = g.exch_end;
+    }
   }
-  // the remainder runs eagerly and advances the mapping
+  // the remainder runs eagerly and advances the map
   for (; done < n_steps; ++done) {
-    int rc = do_rk_step(ctx, dt, n_alpha, alpha, DRN_LAUNCH_BATCHED);
-    if (rc) return rc;
+    if ((rc = do_rk_step(ctx, dt, n_alpha, alpha, DRN_LAUNCH_BATCHED))) return rc;
   }
   return DRN_OK;
 }
@@ -1553,33 +1582,53 @@ int drn_core_sums(drn_ctx* ctx, int field, double* out) {
   return ctx->prec == DRN_F32 ? sums_t(ctx, (const float*)U, out) : sums_t(ctx, (const double*)U, out);
 }
 
+// out[0..4] as k_diag_partial's partials, reduced over all tiles
 template <typename T>
 static int diag_t(drn_ctx* ctx, const T* U, double dt, double* out) {
   const int nt = (int)ctx->tiles.size();
   hipLaunchKernelGGL(k_diag_partial<T>, dim3(nt), dim3(256), 0, ctx->stream, ctx->d_pd, ctx->d_tiles, U, ctx->vstride,
                      dt, ctx->d_part);
   HIPCHK(ctx, hipGetLastError());
-  std::vector<double> h((size_t)nt * 4);
+  std::vector<double> h((size_t)nt * 5);
   HIPCHK(ctx, hipMemcpyAsync(h.data(), ctx->d_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  double o[4] = {0, 1e300, -1e300, 0};
+  double o[5] = {0, 1e300, -1e300, 0, 0};
   for (int b = 0; b < nt; ++b) {
-    o[0] = std::max(o[0], h[4 * b]);
-    o[1] = std::min(o[1], h[4 * b + 1]);
-    o[2] = std::max(o[2], h[4 * b + 2]);
-    o[3] += h[4 * b + 3];
+    o[0] = std::max(o[0], h[5 * b]);
+    o[1] = std::min(o[1], h[5 * b + 1]);
+    o[2] = std::max(o[2], h[5 * b + 2]);
+    o[3] += h[5 * b + 3];
+    o[4] = std::max(o[4], h[5 * b + 4]);
   }
   memcpy(out, o, sizeof(o));
   return DRN_OK;
 }
 
-int drn_diagnostics(drn_ctx* ctx, int field, double dt, double* out) {
-  if (!ctx || !out) return DRN_E_ARG;
+static int diag(drn_ctx* ctx, int field, double dt, double* o5) {
   int rc;
   if ((rc = check_field(ctx, field))) return rc;
+  if (!ctx->laid_out) return fail(ctx, DRN_E_STATE, "drn_layout not called");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const void* U = ctx->buf[ctx->map[field]];
-  return ctx->prec == DRN_F32 ? diag_t(ctx, (const float*)U, dt, out) : diag_t(ctx, (const double*)U, dt, out);
+  return ctx->prec == DRN_F32 ? diag_t(ctx, (const float*)U, dt, o5) : diag_t(ctx, (const double*)U, dt, o5);
+}
+
+int drn_diagnostics(drn_ctx* ctx, int field, double dt, double* out) {
+  if (!ctx || !out) return DRN_E_ARG;
+  double o[5];
+  int rc = diag(ctx, field, dt, o);
+  if (rc) return rc;
+  memcpy(out, o, 4 * sizeof(double));
+  return DRN_OK;
+}
+
+int drn_cfl_axes(drn_ctx* ctx, int field, double dt, double* cfl) {
+  if (!ctx || !cfl) return DRN_E_ARG;
+  double o[5];
+  int rc = diag(ctx, field, dt, o);
+  if (rc) return rc;
+  *cfl = o[4];
+  return DRN_OK;
 }
 
 template <typename T>
@@ -1672,10 +1721,7 @@ int drn_set_tiling(drn_ctx* ctx, int launch, int choice) {
     build_march(ctx);
     return rc;
   }
-  if (ctx->gexec) {
-    (void)hipGraphExecDestroy(ctx->gexec);
-    ctx->gexec = nullptr;
-  }
+  drop_graph(ctx);
   return DRN_OK;
 }
 
@@ -1753,10 +1799,7 @@ int drn_set_arith(drn_ctx* ctx, int mode) {
     int rc = build_march(ctx);
     if (rc) return rc;
   }
-  if (ctx->gexec) {
-    (void)hipGraphExecDestroy(ctx->gexec);
-    ctx->gexec = nullptr;
-  }
+  drop_graph(ctx);
   return DRN_OK;
 }
 

@@ -64,6 +64,7 @@ _SIGS = {
     "drn_rk_graph_run": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_double), C.c_int]),
     "drn_core_sums": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "drn_diagnostics": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double)]),
+    "drn_cfl_axes": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double)]),
     "drn_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "drn_set_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "drn_get_tiling": (C.c_int, [C.c_void_p, C.c_int]),
@@ -251,6 +252,12 @@ class Context:
         out = (C.c_double * 4)()
         self._chk(self.L.drn_diagnostics(self.h, field, float(dt), out), "drn_diagnostics")
         return dict(cfl=out[0], rho_min=out[1], rho_max=out[2], nonfinite=out[3])
+
+    def cfl_axes(self, field=0, dt=0.0):
+        """the CFL number over all three axes (no counterpart in the reference)"""
+        out = C.c_double()
+        self._chk(self.L.drn_cfl_axes(self.h, field, float(dt), C.byref(out)), "drn_cfl_axes")
+        return out.value
 
     def autotune(self, launch=DRN_LAUNCH_BATCHED):
         c = C.c_int()

@@ -285,6 +285,13 @@ class IteratorFeeder {
   std::vector<PatchIterator<real>*> iterators_;
 };
 
+// GenericOperation (genericoperation.h:24-31): a post operation of the time integration
+class GenericOperation {
+ public:
+  virtual ~GenericOperation() {}
+  virtual void operator()() = 0;
+};
+
 template <typename real>
 struct real_precision;
 template <>
@@ -368,14 +375,22 @@ class HIP_CartesianIterator : public PatchIterator<real> {
     return choice;
   }
   void sync() { check(drn_sync(ctx_), "drn_sync"); }
-  // the largest CFL number of field i_field for time step dt, max over cells
-  // and axes of (|u_axis| + a) dt / d_axis (the CFL_max loop of the jet demo,
-  // applications/drnumJetDemo/main.h:188-210, which takes the x axis only)
+  // CFL_max of field i_field for time step dt as the jet demo's diagnostics
+  // loop computes it (applications/drnumJetDemo/main.h:196-205): the maximum
+  // over cells of max(|u|, |u+a|, |u-a|) dt/dx, x axis only, in `real`
   double cflMax(size_t i_field, double dt) {
     setup();
     double out[4] = {0, 0, 0, 0};
     check(drn_diagnostics(ctx_, (int)i_field, dt, out), "drn_diagnostics");
     return out[0];
+  }
+  // not in the reference: the CFL number over all three axes, max over cells
+  // and axes of (|u_d| + a) dt / delta_d
+  double cflAxes(size_t i_field, double dt) {
+    setup();
+    double c = 0;
+    check(drn_cfl_axes(ctx_, (int)i_field, dt, &c), "drn_cfl_axes");
+    return c;
   }
   // Patch::computeVariableDifference (patch.cpp:733-746) for the iterator's
   // patch i_patch, on the device: max_norm as the reference computes it, the
@@ -390,6 +405,13 @@ class HIP_CartesianIterator : public PatchIterator<real> {
     l2_norm = (real)l2;
   }
   int launchForm() const { return launch_; }
+  // the level-set fill registered on this iterator's context (HIP_LevelSetBC),
+  // which drn_rk_step runs as its post operation
+  const GenericOperation* levelSetOperation() const { return ls_op_; }
+  void setLevelSetOperation(const GenericOperation* op) {
+    if (ls_op_ && ls_op_ != op) throw DeviceError("HIP_CartesianIterator: a second level-set operation");
+    ls_op_ = op;
+  }
 
   // PatchGrid::writeToVtk (patchgrid.cpp:673-695): the donor data of field 0 are
   // exchanged first (accessAllDonorData(0)), then the core cells of every patch
@@ -519,17 +541,12 @@ class HIP_CartesianIterator : public PatchIterator<real> {
   drn_ctx* ctx_ = nullptr;
   bool laid_out_ = false;
   int launch_ = DRN_LAUNCH_BATCHED;
+  const GenericOperation* ls_op_ = nullptr;
 };
 
 // ---------------------------------------------------------------------------
 // post operations and time integration
 // ---------------------------------------------------------------------------
-
-class GenericOperation {
- public:
-  virtual ~GenericOperation() {}
-  virtual void operator()() = 0;
-};
 
 // The OP of CPU_LevelSetObjectBC<DIM, OP>: the slip wall
 // (lsobccompressibleeulerop.h:49-144) or the simple wall
@@ -579,6 +596,7 @@ class HIP_LevelSetBC : public GenericOperation {
     }
     it_->check(drn_levelset_commit(ctx), "drn_levelset_commit");
     it_->check(drn_set_levelset_op(ctx, ls_op_code<LSOP>::value), "drn_set_levelset_op");
+    it_->setLevelSetOperation(this);
   }
   void operator()() override { it_->check(drn_levelset_fill(it_->context(), (int)field_, relax_), "drn_levelset_fill"); }
 
@@ -643,21 +661,43 @@ class RungeKutta : public TimeIntegration<real> {
 // field 1, and the exchange before a stage is skipped when nothing changed the
 // field since the last one.  A level-set fill set on the iterator's context runs
 // after every stage's exchange, as the post operation does.  graph(dt, n) runs
-// n steps from a captured HIP graph.
+// n steps from captured HIP graphs.
+//
+// The library call runs exactly the one iterator and the level-set fill
+// registered on its context, so the step refuses (DeviceError) any other
+// iterator or post operation added with addIterator / addPostOperation, and a
+// registered fill that was not added as a post operation: run such a setup
+// with RungeKutta, the reference's stage loop over the virtual interface.
 template <unsigned DIM, class OP, typename real>
 class HIP_RungeKutta : public RungeKutta<real> {
  public:
   explicit HIP_RungeKutta(HIP_CartesianIterator<DIM, OP, real>* it) : it_(it) { this->addIterator(it); }
   void operator()(real dt) override {
+    checkSetup();
     std::vector<double> a(this->alpha_.begin(), this->alpha_.end());
     it_->check(drn_rk_step(it_->context(), (double)dt, (int)a.size(), a.data(), it_->launchForm()), "drn_rk_step");
   }
   void graph(real dt, int n_steps) {
+    checkSetup();
     std::vector<double> a(this->alpha_.begin(), this->alpha_.end());
     it_->check(drn_rk_graph_run(it_->context(), (double)dt, (int)a.size(), a.data(), n_steps), "drn_rk_graph_run");
   }
 
  private:
+  void checkSetup() const {
+    if (this->iterators_.size() != 1 || this->iterators_.front() != it_)
+      throw DeviceError("HIP_RungeKutta runs its one HIP iterator only; use RungeKutta for several iterators");
+    const GenericOperation* ls = it_->levelSetOperation();
+    size_t n_ls = 0;
+    for (const GenericOperation* op : this->post_) {
+      if (op != ls || ls == nullptr)
+        throw DeviceError("HIP_RungeKutta runs no post operation but the iterator's level-set fill; use RungeKutta");
+      ++n_ls;
+    }
+    if (ls != nullptr && n_ls != 1)
+      throw DeviceError("HIP_RungeKutta: the iterator's level-set fill must be added once with addPostOperation");
+  }
+
   HIP_CartesianIterator<DIM, OP, real>* it_;
 };
 

@@ -76,7 +76,10 @@ typedef struct drn_patch_desc {
   double bc_par[6][DRN_BC_NPAR]; /* FARFIELD: [0..4] far state in CONSERVED variables;       *
                                   * JET: [0] c_a [1] c_b [2] radius [3] jet speed [4] T_jet     *
                                   *      [5] far normal speed [6] T_far [7] noise amplitude    *
-                                  *      [8] seed [9],[10] global index offsets (axes a, b)    */
+                                  *      [8] seed [9],[10] global index offsets (axes a, b)    *
+                                  *      [11] p_jet [12] supersonic: nonzero takes p = p_jet    *
+                                  *      inside the radius (jetflux.h:176-178), else p of the   *
+                                  *      boundary cell                                          */
 } drn_patch_desc;
 
 /* ---- context ------------------------------------------------------------ */
@@ -190,18 +193,30 @@ int drn_set_levelset_op(drn_ctx* ctx, int op);
  * with coefficients alpha[0..n-1], the exchange after every stage and the
  * level-set fill (if tables are set) as the post operation.  The state is in
  * field 0 before and after; fields 1 and 2 are used as stage storage.  The
- * exchange before stages 2..n is repeated only when a post operation exists
- * (without one it reproduces the same values). */
+ * exchange before a stage is repeated only when a post operation exists or
+ * when the state was written since its last exchange (an upload, a copy, a
+ * drn_stage or a fill into it); otherwise it would reproduce the same values. */
 int drn_rk_step(drn_ctx* ctx, double dt, int n_alpha, const double* alpha, int launch);
-/* Capture `n_steps` drn_rk_step calls in a HIP graph and replay it. */
+/* `n_steps` drn_rk_step calls (batched launch form) replayed from HIP graphs.
+ * A graph holds the steps until the field map returns to its start (two for an
+ * odd stage count) and the physical buffers of the map it was captured from,
+ * so one is captured per starting map and per (dt, alpha); a remainder of
+ * fewer steps runs eagerly.  Results are bit-identical to drn_rk_step. */
 int drn_rk_graph_run(drn_ctx* ctx, double dt, int n_alpha, const double* alpha, int n_steps);
 
 /* ---- reductions (FP64 accumulators) -------------------------------------- */
 /* out[0..4]: sum over core cells of (rho, rhou, rhov, rhow, rhoE) * dV. Synchronous. */
 int drn_core_sums(drn_ctx* ctx, int field, double* out);
-/* out[0] = max CFL number |u|+a times dt/dx (jet main.h:188-210), out[1] = rho min,
- * out[2] = rho max, out[3] = number of non-finite values.  Synchronous. */
+/* The diagnostics loop of the jet demo (applications/drnumJetDemo/main.h:188-210)
+ * over all cells: out[0] = CFL_max = max over cells of max(|u|, |u+a|, |u-a|)
+ * dt/dx with the x velocity and spacing only, evaluated in the state precision
+ * as the reference does; out[1] = rho min, out[2] = rho max, out[3] = number of
+ * non-finite values.  Synchronous. */
 int drn_diagnostics(drn_ctx* ctx, int field, double dt, double* out);
+/* No counterpart in the reference: the CFL number over all three axes,
+ * max over cells and axes d of (|u_d| + a) dt/delta_d, evaluated in FP64, for
+ * flows that are not aligned with x.  Synchronous. */
+int drn_cfl_axes(drn_ctx* ctx, int field, double dt, double* cfl);
 /* Patch::computeVariableDifference(field1, var1, field2, var2, max_norm, l2_norm)
  * (patch.cpp:733-746) over all cells of `patch`: max_norm = sqrt(max (b - a)^2),
  * l2_norm = sqrt(sum (b - a)^2), each square in the state type.  max_norm is the

@@ -381,8 +381,10 @@ static REAL jet_noise(uint64_t seed, int64_t ga, int64_t gb, int c) {
 }
 
 /* jetflux.h:158-192 (JetFlux::xWallM), generalised to any face: inside the jet
- * radius the normal velocity is the jet speed (with seeded velocity noise),
- * outside it is the far speed; p is taken from the boundary cell (subsonic). */
+ * radius the normal velocity is the jet speed (with seeded velocity noise) and
+ * p is p_jet when the jet is supersonic (par[12] != 0), else the boundary
+ * cell's; outside the radius the normal velocity is the far speed and p the
+ * cell's. */
 static void wall_jet(const double* par, const REAL* var, int dir, int plus, REAL ta, REAL tb, int64_t la,
                      int64_t lb, REAL A, REAL* flux) {
   REAL p, T, u, v, w;
@@ -401,6 +403,7 @@ static void wall_jet(const double* par, const REAL* var, int dir, int plus, REAL
     vel[dir] = sgn * (uj * (1 + amp * nn));
     vel[axa] = uj * (amp * na);
     vel[axb] = uj * (amp * nb);
+    if (par[12] != 0.0) p = (REAL)par[11]; /* jetflux.h:176-178 (m_SuperSonic) */
     T = (REAL)par[4];
   } else {
     vel[dir] = sgn * (REAL)par[5];

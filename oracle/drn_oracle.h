@@ -40,7 +40,8 @@ enum { O_BC_SLIP = 0, O_BC_FARFIELD = 1, O_BC_TRANSMISSIVE = 2, O_BC_JET = 3, O_
  *  JET:      [0] c_a, [1] c_b   jet centre in the two tangential patch-local coordinates
  *            [2] radius, [3] normal jet speed (positive into the domain), [4] T_jet,
  *            [5] normal far speed, [6] T_far, [7] noise amplitude (fraction of jet speed),
- *            [8] noise seed, [9] global index offset of tangential axis a, [10] of axis b  */
+ *            [8] noise seed, [9] global index offset of tangential axis a, [10] of axis b,
+ *            [11] p_jet, [12] supersonic flag (nonzero: p = p_jet inside the radius)  */
 #define O_BC_NPAR 16
 
 typedef struct o_patch_desc {

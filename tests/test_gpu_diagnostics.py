@@ -37,3 +37,59 @@ def test_variable_difference_matches_the_oracle(gpu, dtype):
     with pytest.raises(RuntimeError, match="variable index"):
         gg.ctx.variable_difference(0, 0, 5, 2, 0)
     gg.close()
+
+
+def cfl_reference_x(U, dt, dx, dtype):
+    """The jet demo's CFL of every cell (applications/drnumJetDemo/main.h:196-205)
+    evaluated in the state type as the reference evaluates it:
+    PerfectGas::conservativeToPrimitive (perfectgas.h:84-92, whose 1.0/rho,
+    0.5*|u|^2 and /cv are double expressions), a = sqrt(gamma R T), then
+    max(|u|*dt/dx, |u+a|*dt/dx, |u-a|*dt/dx) with the x velocity only."""
+    f = np.dtype(dtype).type
+    r, m1, m2, m3, e = (U[v].astype(dtype) for v in range(5))
+    ir = (1.0 / r.astype(np.float64)).astype(dtype)
+    u, v, w = m1 * ir, m2 * ir, m3 * ir
+    ke = (u * u + v * v + w * w).astype(dtype)
+    T = ((((e * ir).astype(np.float64)) - 0.5 * ke.astype(np.float64)) / 717.5).astype(dtype)
+    a = np.sqrt(f(1.4) * f(287.2) * T).astype(dtype)
+    dt, dx = f(dt), f(dx)
+    c = np.maximum(np.abs(u) * dt / dx, np.abs(u + a) * dt / dx)
+    return np.maximum(c, np.abs(u - a) * dt / dx)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_cfl_is_the_jet_demos_x_only_number(gpu, dtype):
+    """drn_diagnostics' CFL is the reference's x-only number and drn_cfl_axes the
+    three-axis extension.  The flow moves mostly along y and z, so the two
+    differ by far more than rounding; each equals its own numpy evaluation."""
+    w = configs.jet(core=8, blocks=2)
+    gg = solver.GpuGrid(w, dtype)
+    rng = np.random.default_rng(5)
+    dt = 1.3e-3
+    best_x, best_3 = 0.0, 0.0
+    for p, P in enumerate(w.patches):
+        n = P.cells
+        rho = 1.0 + 0.2 * rng.random(n)
+        vel = np.stack([0.05 * rng.standard_normal(n), 0.8 + 0.1 * rng.standard_normal(n),
+                        -0.6 + 0.1 * rng.standard_normal(n)])
+        T = (1.0 / 1.4 / 287.2) * (1.0 + 0.1 * rng.random(n))
+        U = np.empty((5, n))
+        U[0] = rho
+        U[1:4] = rho * vel
+        U[4] = rho * (717.5 * T + 0.5 * (vel * vel).sum(axis=0))
+        U = U.astype(dtype)
+        gg.ctx.upload(p, 0, U.copy())
+        dx, dy, dz = solver.patch_spacing(P, dtype)
+        best_x = max(best_x, float(cfl_reference_x(U, dt, dx, dtype).max()))
+        Ud = U.astype(np.float64)
+        ud = Ud[1:4] / Ud[0]
+        Td = (Ud[4] / Ud[0] - 0.5 * (ud * ud).sum(axis=0)) / 717.5
+        ad = np.sqrt(1.4 * 287.2 * Td)
+        d = np.array([dx, dy, dz])[:, None]
+        best_3 = max(best_3, float(((np.abs(ud) + ad) * dt / d).max()))
+    print("cfl", np.dtype(dtype).name, best_x, best_3, flush=True)
+    diag = gg.ctx.diagnostics(0, dt)
+    assert diag["cfl"] == pytest.approx(best_x, rel=2e-7 if dtype == np.float32 else 1e-15)
+    assert gg.ctx.cfl_axes(0, dt) == pytest.approx(best_3, rel=1e-14)
+    assert best_3 > 1.3 * best_x
+    gg.close()

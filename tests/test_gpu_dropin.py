@@ -201,7 +201,9 @@ def test_dropin_vtk_output(gpu, tmp_path, dtype):
 
 
 def cfl_numpy(w, states, dt, dtype):
-    """max over cells and axes of (|u_axis| + a) dt / d_axis, from the conserved state"""
+    """CFL_max of the jet demo's diagnostics loop (applications/drnumJetDemo/
+    main.h:196-205): max over cells of max(|u|, |u+a|, |u-a|) dt/dx, with the x
+    velocity and spacing only, evaluated in FP64"""
     from drnum_amd import solver
 
     best = 0.0
@@ -210,8 +212,9 @@ def cfl_numpy(w, states, dt, dtype):
         u = U[1:4] / U[0]
         T = (U[4] / U[0] - 0.5 * (u * u).sum(axis=0)) / 717.5
         a = np.sqrt(1.4 * 287.2 * T)
-        d = np.array(solver.patch_spacing(P, dtype), dtype=np.float64)
-        best = max(best, float(((np.abs(u) + a) * dt / d[:, None]).max()))
+        dx = float(solver.patch_spacing(P, dtype)[0])
+        c = np.maximum(np.abs(u[0]), np.maximum(np.abs(u[0] + a), np.abs(u[0] - a))) * dt / dx
+        best = max(best, float(c.max()))
     return best
 
 

@@ -198,6 +198,83 @@ def test_graph_equals_eager(gpu):
         assert np.array_equal(x, y)
 
 
+def test_graph_replays_from_any_field_map(gpu):
+    """A graph holds the physical buffers of the field map it was captured from.
+    Calls that leave the map elsewhere (a remainder step, an eager step, a second
+    call) must replay a graph of the map they start from: graph(3) twice, an
+    eager step then graph(3), and graph(1) then graph(4) all equal the same
+    number of eager steps bit for bit."""
+    w = configs.jet(core=8, blocks=2)
+    og = O.OracleGrid(w, np.float32)
+    og.compute_dependencies()
+    t = tables_of(og)
+
+    def fresh():
+        g = solver.GpuGrid(w, np.float32)
+        g.upload_initial()
+        g.set_tables(t)
+        return g
+
+    ref = fresh()
+    eager = []
+    for s in range(6):
+        ref.rk_step()
+        if s in (3, 4, 5):
+            eager.append(ref.state(0))
+    cases = {
+        "graph3+graph3": ([("g", 3), ("g", 3)], eager[2]),
+        "eager1+graph3": ([("e", 1), ("g", 3)], eager[0]),
+        "graph1+graph4": ([("g", 1), ("g", 4)], eager[1]),
+        "graph2+eager1+graph2": ([("g", 2), ("e", 1), ("g", 2)], eager[1]),
+    }
+    for name, (calls, want) in cases.items():
+        print("graph case", name, flush=True)
+        g = fresh()
+        for kind, n in calls:
+            if kind == "g":
+                g.ctx.rk_graph_run(w.dt, w.alpha, n)
+            else:
+                g.rk_step()
+        for x, y in zip(g.state(0), want):
+            assert np.array_equal(x, y), name
+        g.close()
+    ref.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_upload_after_steps_is_exchanged(gpu, graph):
+    """A state uploaded into a context that has already stepped starts its next
+    step with the leading exchange of rungekutta.cpp:30, as in a fresh context:
+    step, upload a new state, step equals upload the new state, step."""
+    w = configs.jet(core=8, blocks=2)
+    og = O.OracleGrid(w, np.float64)
+    og.compute_dependencies()
+    t = tables_of(og)
+    rng = np.random.default_rng(17)
+    new = [(w.initial_state(p, np.float64) * (1.0 + 0.01 * rng.random((5, P.cells)))).astype(np.float64)
+           for p, P in enumerate(w.patches)]
+    a = solver.GpuGrid(w, np.float64)
+    a.upload_initial()
+    a.set_tables(t)
+    b = solver.GpuGrid(w, np.float64)
+    b.set_tables(t)
+    b.upload_state(new)
+    if graph:
+        a.ctx.rk_graph_run(w.dt, w.alpha, 2)
+        a.upload_state(new)
+        a.ctx.rk_graph_run(w.dt, w.alpha, 2)
+        b.ctx.rk_graph_run(w.dt, w.alpha, 2)
+    else:
+        a.rk_step()
+        a.upload_state(new)
+        a.rk_step()
+        b.rk_step()
+    for x, y in zip(a.state(0), b.state(0)):
+        assert np.array_equal(x, y)
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_core_sums(gpu, dtype):
     w = configs.jet(core=8, blocks=2)

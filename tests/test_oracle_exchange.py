@@ -169,6 +169,53 @@ def test_weights_sum_to_one_and_reproduce_linear_fields(ratio, shift):
         assert abs(lin - coef @ xp[:, rec]) <= 1e-13
 
 
+def eliminated_expected(Pf, rec, Q, eps=1e-4):
+    """The brute-force trilinear weights after the reference's clean-up
+    (tranferToFixedArrays, cartesianpatch.cpp:1294-1296): weights below the
+    absolute threshold 10*m_Eps = 1e-4 dropped (WeightedSet::EliminateBelowEps,
+    weightedset.h:569-602, keep_sum false), then the same shift added to each
+    remaining weight so that they sum to 1 (adjustWeightSumShift, :605-619)."""
+    w = trilinear_expected(Pf, rec, Q)
+    kept = {c: wt for c, wt in w.items() if abs(wt) >= eps}
+    shift = (1.0 - sum(kept.values())) / len(kept)
+    return {c: wt + shift for c, wt in kept.items()}, {c: wt for c, wt in w.items() if abs(wt) < eps}
+
+
+def near_aligned():
+    """A patch at the coarse patch's resolution, offset by 5e-5 of a cell in x
+    and 3e-4 in y (exact powers of two elsewhere): every receiver centre sits
+    just off a donor centre, so its trilinear octet holds weights of about
+    5e-5 and 1.5e-8 (below the 1e-4 threshold) beside 3e-4 and ~0.9997 (above)."""
+    h = 1.0 / 16
+    Q = patch((16, 16, 16), [0] * 6, (0.0, 0.0, 0.0), (h,) * 3)
+    P = patch((12, 12, 12), [2] * 6, (2 * h + 5e-5 * h, 2 * h + 3e-4 * h, 2 * h), (h,) * 3)
+    return workload([Q, P])
+
+
+def test_small_donor_weights_are_eliminated_and_the_rest_shifted():
+    w = near_aligned()
+    g, tabs = oracle_tables(w)
+    Q, Pf = w.patches
+    rw = receiver_weights(tabs[1])
+    assert len(rw) == len(seek_cells(Pf))
+    dropped = kept_small = 0
+    for rec, lst in rw.items():
+        got = {}
+        for (_, c, wt) in lst:
+            if wt != 0.0:
+                got[c] = got.get(c, 0.0) + wt
+        want, small = eliminated_expected(Pf, rec, Q)
+        assert got.keys() == want.keys(), (rec, got, want)
+        for c in want:
+            assert abs(got[c] - want[c]) <= 1e-15
+        assert abs(sum(got.values()) - 1.0) <= 2 * np.finfo(np.float64).eps
+        dropped += sum(1 for wt in small.values() if wt > 0.0)
+        kept_small += sum(1 for wt in want.values() if wt < 5e-4)
+    # both sides of the threshold occur at every receiver
+    assert dropped == 2 * len(rw)
+    assert kept_small == len(rw)
+
+
 def test_exchange_reproduces_linear_fields():
     w = nested(2)
     g = O.OracleGrid(w, np.float64)
@@ -233,6 +280,7 @@ def _host_cases():
         "nest2": nested(2),
         "nest3": nested(3),
         "shift": nested(2, shift=(0.3, 0.1, 0.45)),
+        "eliminate": near_aligned(),
         "uneven": configs.uneven_batch(n_patches=12, size_range=(8, 14)),
     }
 
