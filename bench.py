@@ -12,13 +12,17 @@ is one RK step over all patches.
               all 64 patches), state resident in HBM, timed region = K steps.
   fp64_plain  FP64 plain per-patch form (one launch per patch per stage).
   fp32_per_patch the FP32 arithmetic launched per patch (the batching gain).
+  fp32_ieee   the FP32 batched form in the reference's arithmetic (IEEE division
+              and square root instead of the hardware reciprocal).
   roofline    stage kernel: algorithmic bytes per launch (10 or 15 values of
               4 B per stored cell: stage 1 reads U^n only) / average launch time
               from HIP events on the context stream over the timed region.
   valu_roofline the same kernel against its compute ceiling: VALU wave
               instructions per launch (committed rocprofv3 summary) / CUs /
               launch time, against the FP32 VALU issue rate measured by
-              drnum_amd/lib/valu_probe after the timed region.
+              drnum_amd/lib/valu_probe after the timed region.  Figures from a
+              committed summary carry the kernel-source sha256 they were taken
+              of; a summary of other sources is marked stale.
   pcie_inclusive the same steps with one upload of the whole state before them
               and one download after them (drn_patch_upload/download, pageable
               host buffers), as a caller that keeps the state on the host runs.
@@ -80,10 +84,12 @@ def stage_bytes_per_step(cells, esize, n_stages):
     return cells * vals * esize
 
 
-def run_form(w, dtype, steps, warmup, launch, label, tiling=None):
+def run_form(w, dtype, steps, warmup, launch, label, tiling=None, arith=None):
     from drnum_amd import hip
 
     hg, ctx = setup(w, dtype)
+    if arith is not None:
+        ctx.set_arith(arith)
     cells = w.total_cells
     esize = np.dtype(dtype).itemsize
     S = len(w.alpha)
@@ -272,6 +278,35 @@ def measured_traffic(tiling):
 
 
 VALU_TABLE = os.path.join(ROOT, "profiles", "stage_valu_jet4x64_fp32.md")
+KERNEL_SOURCES = [os.path.join(ROOT, "drnum_amd", "csrc", f) for f in ("drn_kernels.h", "drn_physics.h", "drnhip.hip")]
+
+
+def kernel_source_hash():
+    """sha256 (first 16 hex digits) of the device library's sources, the build
+    a committed rocprofv3 summary records as 'kernel source sha256'"""
+    import hashlib
+
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def profile_hash(path):
+    """the 'kernel source sha256: <hash>' line of a committed profile summary"""
+    try:
+        for ln in open(path).read().splitlines():
+            if "kernel source sha256:" in ln:
+                return ln.split("kernel source sha256:", 1)[1].strip().strip("`").split()[0]
+    except OSError:
+        pass
+    return None
+
+
+def profile_current(path):
+    """whether a committed profile summary was taken of the current kernel sources"""
+    return profile_hash(path) == kernel_source_hash()
 VALU_PROBE = os.path.join(ROOT, "drnum_amd", "lib", "valu_probe")
 
 
@@ -461,8 +496,18 @@ def main():
         cpu = cpu_baseline(w, np.float32, threads)
         log("cpu baseline", cpu)
 
+    # the reference's arithmetic: the same FP32 batched form with IEEE division
+    # and square root (DRN_ARITH_IEEE), as the reference's CUDA build computes
+    f32ieee = None
+    if not args.fp32_only:
+        f32ieee = run_form(w, np.float32, args.steps, args.warmup, hip.DRN_LAUNCH_BATCHED, "fp32-ieee-batched",
+                           args.tiling, arith=hip.DRN_ARITH_IEEE)
     ach = f32["stage_gbs"]
     default_workload = args.workload == "jet" and (args.core, args.blocks) == (64, 4)
+    # figures read from committed rocprofv3 summaries hold only for the kernel
+    # sources they were taken of: a summary of another build is reported stale
+    traffic_current = profile_current(TRAFFIC_TABLE)
+    valu_current = profile_current(VALU_TABLE)
     traffic = measured_traffic(f32["tiling"]) if default_workload else None
     # compute ceiling of the same kernel: VALU instructions per launch (committed
     # profile of this workload) over the live launch time, against the probe
@@ -473,6 +518,7 @@ def main():
         rate = insts / probe["cus"] / (f32["stage_ms"] * 1e6)
         valu = {"bound": "valu", "achieved": round(rate, 3), "peak": probe["peak"], "unit": "wave-instr/CU/ns",
                 "frac": round(rate / probe["peak"], 4), "instructions_per_launch": insts,
+                "stale": not valu_current,
                 "source": "SQ_INSTS_VALU from profiles/stage_valu_jet4x64_fp32.md (tiling %d); peak measured now by "
                           "drnum_amd/lib/valu_probe" % f32["tiling"]}
         log("valu", valu)
@@ -507,9 +553,12 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
+            "traffic": traffic if traffic_current else None,
             "traffic_source": ("rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/stage_traffic_jet4x64_fp32.md, tiling %d"
                                % f32["tiling"]) if traffic else None,
+            # a summary taken of other kernel sources: its figure, not used as traffic
+            "traffic_stale": None if (traffic is None or traffic_current) else traffic,
+            "kernel_source_sha256": kernel_source_hash(),
             "algorithmic_bytes_per_launch": round(f32["stage_bytes_per_launch"]),
         },
         "valu_roofline": valu,
@@ -519,6 +568,15 @@ def main():
             "stage_kernel_gbs": round(f64["stage_gbs"], 1),
             "tiling": f64["tiling"],
             "fp32_batched_over_fp64_plain": round(f32["mcups"] / f64["mcups"], 3),
+        },
+        "fp32_ieee": None if f32ieee is None else {
+            # the reference's arithmetic (IEEE division and square root) in the same batched form
+            "value": round(f32ieee["mcups"], 2),
+            "ms_per_step": round(f32ieee["ms_per_step"], 4),
+            "stage_kernel_ms": round(f32ieee["stage_ms"], 4),
+            "tiling": f32ieee["tiling"],
+            "fast_over_ieee": round(f32["mcups"] / f32ieee["mcups"], 3),
+            "ieee_over_fp64_plain": None if f64 is None else round(f32ieee["mcups"] / f64["mcups"], 3),
         },
         "graph": {
             "ms_per_step": round(f32["graph_ms_per_step"], 4),
@@ -545,7 +603,8 @@ def main():
         "stage_kernel_ms": round(f32["stage_ms"], 4),
         "exchange_ms": round(f32["exch_ms"], 4),
         "exchange_roofline": kernel_roofline(f32["exch_ms"], f32["exch_bytes"],
-                                             measured_exchange_traffic() if default_workload else None),
+                                             measured_exchange_traffic() if default_workload and traffic_current
+                                             else None),
         "exchange_fused_receivers": f32["fused"],
         "levelset_fill_roofline": kernel_roofline(f32["ls_ms"], f32["ls_bytes"]),
         "stream_copy_gbs": round(f32["copy_gbs"], 1),

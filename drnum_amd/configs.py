@@ -329,47 +329,60 @@ def levelset_spheres(core: int = 64, blocks=(3, 2, 2), n_spheres: int = 8, steps
 
 def uneven_batch(n_patches: int = 200, size_range=(16, 96), seed: int = 2024, refine_frac: float = 0.2,
                  steps: int = 100, mach: float = 0.5) -> Workload:
-    """Patches of uneven size (per-axis U[size_range]) placed in lattice rows with
-    2-4 cell overlaps; a fraction are replaced by ratio-2 refined patches nested
-    inside their coarse parent's core.  Uniform Mach 0.5 flow along x plus a
-    Gaussian density pulse (amplitude 0.1, width 8 coarse cells).  Farfield
-    functors on all faces (a patch face that is also covered by a neighbour is
-    overwritten by the exchange)."""
+    """Patches of uneven size, each with its own per-axis sizes drawn from
+    U[size_range], on a lattice of rows: along x the patches of a row follow
+    each other with a 2-4 cell overlap; along y and z each layer of rows starts
+    2-4 cells below the lowest top of the layer before it, so the rows of two
+    layers overlap by at least that and their patch boundaries do not line up
+    (a seek layer draws on one or two donor patches; seek cells no patch
+    covers keep their own update).  A fraction of the patches are ratio-2
+    refined blocks nested inside a coarse parent's core.  Uniform Mach 0.5 flow
+    along x plus a Gaussian density pulse (amplitude 0.1, width 8 coarse
+    cells).  Farfield functors on all faces (a patch face that is also covered
+    by a neighbour is overwritten by the exchange)."""
     rng = np.random.default_rng(seed)
     h = 1.0 / 256.0
     far = _par()
     for f in range(6):
         far[f, :5] = [P_AMB, T_AMB, mach, 0.0, 0.0]
-    # coarse lattice: rows of patches along x, stacked in y and z
-    nb = int(round((n_patches * (1.0 - refine_frac)) ** (1.0 / 3.0))) + 1
-    sizes = rng.integers(size_range[0], size_range[1] + 1, size=(nb, nb, nb, 3))
-    overl = rng.integers(2, 5, size=(nb, nb, nb, 3))
-    patches: List[Patch] = []
-    # cumulative placement per axis uses the size of the first patch of each slab
-    starts = []
-    for a in range(3):
-        s = [0]
-        for b in range(1, nb):
-            s.append(s[-1] + int(sizes[b - 1, 0, 0, a] if a == 0 else sizes[0, b - 1, 0, a] if a == 1
-                                 else sizes[0, 0, b - 1, a]) - int(overl[b, 0, 0, a]))
-        starts.append(s)
-    coarse = []
-    for bi in range(nb):
+    n_coarse = int(n_patches * (1.0 - refine_frac))
+    nb = 1
+    while nb ** 3 < n_coarse:
+        nb += 1
+    slots = [(bi, bj, bk) for bk in range(nb) for bj in range(nb) for bi in range(nb)][:n_coarse]
+    used = set(slots)
+    sizes = {b: [int(x) for x in rng.integers(size_range[0], size_range[1] + 1, size=3)] for b in slots}
+    overl = {b: [int(x) for x in rng.integers(2, 5, size=3)] for b in slots}
+    # z layers, then y rows inside them, then x along each row
+    g0 = {}
+    zs = 0
+    for bk in range(nb):
+        layer = [b for b in slots if b[2] == bk]
+        if not layer:
+            continue
+        ys = 0
         for bj in range(nb):
-            for bk in range(nb):
-                if len(coarse) >= int(n_patches * (1.0 - refine_frac)):
-                    break
-                b = (bi, bj, bk)
-                n = [int(sizes[0, 0, 0, 0]) if False else 0] * 3
-                n = [int(sizes[bi, 0, 0, 0]), int(sizes[0, bj, 0, 1]), int(sizes[0, 0, bk, 2])]
-                g0 = [starts[0][bi], starts[1][bj], starts[2][bk]]
-                seek = []
-                for a in range(3):
-                    seek += [2 if b[a] > 0 else 0, 2 if b[a] < nb - 1 else 0]
-                org = [g0[a] * h for a in range(3)]
-                L = [n[a] * h for a in range(3)]
-                coarse.append(Patch(n[0], n[1], n[2], seek, org, L, [BC_FARFIELD] * 6, far.copy(), g0))
-    patches.extend(coarse)
+            row = [b for b in layer if b[1] == bj]
+            if not row:
+                continue
+            xs = 0
+            for b in sorted(row):
+                g0[b] = [xs, ys, zs]
+                xs += sizes[b][0] - overl[b][0]
+            ys = min(g0[b][1] + sizes[b][1] for b in row) - max(overl[b][1] for b in row)
+        zs = min(g0[b][2] + sizes[b][2] for b in layer) - max(overl[b][2] for b in layer)
+    coarse = []
+    for b in slots:
+        n = sizes[b]
+        seek = []
+        for a in range(3):
+            lower = tuple(b[q] - (q == a) for q in range(3))
+            upper = tuple(b[q] + (q == a) for q in range(3))
+            seek += [2 if lower in used else 0, 2 if upper in used else 0]
+        org = [g0[b][a] * h for a in range(3)]
+        L = [n[a] * h for a in range(3)]
+        coarse.append(Patch(n[0], n[1], n[2], seek, org, L, [BC_FARFIELD] * 6, far.copy(), list(g0[b])))
+    patches: List[Patch] = list(coarse)
     n_fine = n_patches - len(coarse)
     for f in range(n_fine):
         parent = coarse[int(rng.integers(0, len(coarse)))]

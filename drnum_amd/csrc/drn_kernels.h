@@ -491,6 +491,281 @@ __global__ __launch_bounds__(256) void k_stage_cell_persist(const PatchDev* __re
 }
 
 // ---------------------------------------------------------------------------
+// i-marching z-shared cell kernel
+// ---------------------------------------------------------------------------
+
+// As k_stage_cell<..., ZS = true> (63 cells of the flattened (j,k) plane per
+// wavefront, z neighbours and z faces handed between lanes), but a work item
+// marches its 252 (j,k) cells over the planes [i0, i1) of the patch.  Each x
+// face i+1/2 is evaluated once, by the cell i: the next plane's residual
+// starts from it, and the MUSCL data of the face's left side (the limited
+// slope of cell i in the fast symmetric form, else the reconstructed left
+// state) is carried in registers: 4 face evaluations per cell instead of 5.
+// The per-lane work of the (j,k) cell (division by NK, the (j,k) part of the
+// fused-exchange boxes, the wall classification) is done once per item.
+//
+// The patch-face functors of a cell (cartesianiterator.h:190-308) are summed
+// first, in the reference's order xM, xP, yM, yP, zM, zP, while only the cell
+// and the carried x data are live; the residual is then that sum + Fx- + Fy-
+// + Fz- - Fz+ - Fy+ - Fx+.  The reference adds the functors after the faces:
+// for a cell on one patch face the two orders add the same terms in another
+// association only.  Requires NI, NJ, NK >= 3 (thinner patches run
+// k_stage_cell).
+template <typename T, int RECON, int LIM, bool FAST>
+__device__ __forceinline__ void stage_xm_item(const PatchDev* __restrict__ pds, const TileMarch* __restrict__ items,
+                                              const T* __restrict__ src, const T* base, T* dst, uint64_t vs, T factor,
+                                              int push) {
+  // the fast symmetric form carries the slope of cell i, the general form the
+  // left MUSCL state of face i+1/2
+  constexpr bool SYM = FAST && RECON == RECON_UPWIND2 && (LIM == LIM_VANALBADA || LIM == LIM_MINMOD);
+  const TileMarch t = items[blockIdx.x];
+  const PatchDev& P = pds[t.patch];
+  const int ni = P.ni, nj = P.nj, nk = P.nk, njk = P.njk;
+  const int lane = (int)(threadIdx.x & 63);
+  const int c2 = t.c0 + 63 * (int)(threadIdx.x >> 6) + lane;
+  const bool in_plane = c2 < njk;
+  const bool own = in_plane && lane < 63;
+  const int cc2 = in_plane ? c2 : njk - 1;
+  const int j = cc2 / nk;
+  const int k = cc2 - j * nk;
+  constexpr uint32_t W = sizeof(T);
+  const uint32_t sx = (uint32_t)njk * W, sy = (uint32_t)nk * W;
+  const char* sv[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) sv[v] = reinterpret_cast<const char*>(src + (uint64_t)v * vs);
+  const T dx = T(P.dx), dy = T(P.dy), dz = T(P.dz);
+  const T Ax = dy * dz, Ay = dx * dz, Az = dx * dy;
+  const T pf = factor / (dx * dy * dz);
+  const int i0 = t.i0, i1 = t.i1;
+  // fused exchange: the (j,k) part of the boxes per lane, the i part per plane
+  auto jk_in = [&](const CellBox& b) {
+    return (uint32_t)(j - b.j0) < (uint32_t)(b.j1 - b.j0) && (uint32_t)(k - b.k0) < (uint32_t)(b.k1 - b.k0);
+  };
+  const bool pjk0 = push && P.npush > 0 && jk_in(P.push[0]), pjk1 = push && P.npush > 1 && jk_in(P.push[1]);
+  const bool rjk0 = push && P.nrecv > 0 && jk_in(P.recv[0]), rjk1 = push && P.nrecv > 1 && jk_in(P.recv[1]);
+  const bool wyz = own && (j == 0 || j == nj - 1 || k == 0 || k == nk - 1);
+
+  // x data entering plane i: u0 = U(i), xs = slope of cell i (SYM) or the
+  // left state of face i+1/2, fxm = F(i-1/2) (zero at i = 0)
+  uint32_t c = (P.off + (uint32_t)i0 * (uint32_t)njk + (uint32_t)cc2) * W;
+  T u0[5], xs[5], fxm[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) u0[v] = ldv<T>(sv[v], c);
+  zero5(fxm);
+  {
+    // U(i0-2), U(i0-1), U(i0+1): the slope of cell i0 and the face i0-1/2
+    T um1[5], um2[5], up1[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) um1[v] = um2[v] = up1[v] = u0[v];
+    if (i0 >= 1) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) um1[v] = ldv<T>(sv[v], c - sx);
+    }
+    if (i0 >= 2) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) um2[v] = ldv<T>(sv[v], c - 2 * sx);
+    }
+    if (i0 + 1 < ni) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) up1[v] = ldv<T>(sv[v], c + sx);
+    }
+    if constexpr (SYM) {
+      const bool okm = i0 >= 2, ok0 = i0 >= 1 && i0 + 1 < ni;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) xs[v] = ok0 ? limited<T, LIM, true>(u0[v] - um1[v], up1[v] - u0[v]) : T(0);
+      if (i0 >= 1) {
+        T vl[5], vr[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          const T sm = okm ? limited<T, LIM, true>(um1[v] - um2[v], u0[v] - um1[v]) : T(0);
+          vl[v] = um1[v] + T(0.5) * sm;
+          vr[v] = u0[v] - T(0.5) * xs[v];
+        }
+        ausm_flux_set<T, FAST, 0>(vl, vr, Ax, fxm);
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) xs[v] = u0[v];
+      if constexpr (RECON == RECON_UPWIND2) {
+        if (i0 >= 1 && i0 + 1 < ni) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) xs[v] = u0[v] + T(0.5) * limited<T, LIM, FAST>(u0[v] - um1[v], up1[v] - u0[v]);
+        }
+      }
+      if (i0 >= 1) face_flux_r<T, RECON, LIM, FAST, 0>(um2, um1, u0, up1, i0 >= 2, i0 + 1 < ni, Ax, fxm);
+    }
+  }
+
+  for (int i = i0; i < i1; ++i, c += sx) {
+    T res[5];
+    zero5(res);
+    // patch-face functors: one call site, the faces in the reference's order
+    if (wyz || i == 0 || i == ni - 1) {
+#pragma unroll 1
+      for (int f = 0; f < 6; ++f) {
+        const int d = f >> 1;
+        const bool plus = (f & 1) != 0;
+        const int idx = d == 0 ? i : (d == 1 ? j : k);
+        const int n = d == 0 ? ni : (d == 1 ? nj : nk);
+        if (own && idx == (plus ? n - 1 : 0)) {
+          const T xc = T(0.5) * dx + T(i) * dx, yc = T(0.5) * dy + T(j) * dy, zc = T(0.5) * dz + T(k) * dz;
+          const T ta = d == 0 ? yc : xc, tb = d == 2 ? yc : zc;
+          const int la = d == 0 ? j : i, lb = d == 2 ? j : k;
+          const T A = d == 0 ? Ax : (d == 1 ? Ay : Az);
+          T g[5];
+          zero5(g);
+          wall_flux<T, FAST>(P.bc[f], P.par[f], u0, d, plus, ta, tb, la, lb, A, g);
+#pragma unroll
+          for (int v = 0; v < 5; ++v) res[v] = plus ? res[v] - g[v] : res[v] + g[v];
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] += fxm[v];
+    T fyp[5], fzp[5], f[5];
+    if (own) axis_faces_r<T, RECON, LIM, FAST, 1>(sv, c, sy, j, nj, u0, Ay, res, fyp);
+    {
+      // z faces by lane shuffles, as in stage_cell_item
+      T fzm[5];
+      zero5(fzm);
+      T um2[5], um1[5], up1[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        um1[v] = __shfl_up(u0[v], 1, 64);
+        um2[v] = __shfl_up(u0[v], 2, 64);
+        up1[v] = __shfl_down(u0[v], 1, 64);
+      }
+      if (lane == 0 && in_plane && k >= 1) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) um1[v] = ldv<T>(sv[v], c - W);
+      }
+      if (lane <= 1 && in_plane && k >= 2) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) um2[v] = ldv<T>(sv[v], c - 2 * W);
+      }
+      if (lane == 63 && in_plane && k + 1 < nk) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) up1[v] = ldv<T>(sv[v], c + W);
+      }
+      if (in_plane && k >= 1) face_flux_r<T, RECON, LIM, FAST, 2>(um2, um1, u0, up1, k >= 2, k + 1 < nk, Az, fzm);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        res[v] += fzm[v];
+        const T up = __shfl_down(fzm[v], 1, 64);
+        fzp[v] = k + 1 < nk ? up : T(0);
+      }
+    }
+    // the x face i+1/2 from U(i), U(i+1), U(i+2)
+    T up1[5];
+    zero5(f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) up1[v] = u0[v];
+    if (i + 1 < ni) {
+      T up2[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        up1[v] = ldv<T>(sv[v], c + sx);
+        up2[v] = up1[v];
+      }
+      if (i + 2 < ni) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) up2[v] = ldv<T>(sv[v], c + 2 * sx);
+      }
+      if (own) {
+        T vl[5], vr[5], xn[5];
+        if constexpr (SYM) {
+          const bool okp = i + 2 < ni;
+#pragma unroll
+          for (int v = 0; v < 5; ++v) {
+            xn[v] = okp ? limited<T, LIM, true>(up1[v] - u0[v], up2[v] - up1[v]) : T(0);
+            vl[v] = u0[v] + T(0.5) * xs[v];
+            vr[v] = up1[v] - T(0.5) * xn[v];
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) {
+            vl[v] = xs[v];
+            vr[v] = up1[v];
+            xn[v] = up1[v];
+          }
+          if constexpr (RECON == RECON_UPWIND2) {
+            if (i + 2 < ni) {
+#pragma unroll
+              for (int v = 0; v < 5; ++v) {
+                vr[v] = up1[v] + T(0.5) * limited<T, LIM, FAST>(up1[v] - up2[v], u0[v] - up1[v]);
+                xn[v] = up1[v] + T(0.5) * limited<T, LIM, FAST>(up1[v] - u0[v], up2[v] - up1[v]);
+              }
+            }
+          }
+        }
+        ausm_flux_set<T, FAST, 0>(vl, vr, Ax, f);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) xs[v] = xn[v];
+      }
+    }
+    if (own) {
+      // cartesianiterator.h:310-322
+      T nv[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        res[v] -= fzp[v];
+        res[v] -= fyp[v];
+        res[v] -= f[v];
+        const char* bv = reinterpret_cast<const char*>(base + (uint64_t)v * vs);
+        nv[v] = ldv<T>(bv, c) + pf * res[v];
+      }
+      bool store = true;
+      if (push) {
+        auto i_in = [&](const CellBox& b) { return (uint32_t)(i - b.i0) < (uint32_t)(b.i1 - b.i0); };
+        store = !(rjk0 && i_in(P.recv[0])) && !(rjk1 && i_in(P.recv[1]));
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          if ((s == 0 ? pjk0 : pjk1) && i_in(P.push[s])) {
+            const uint32_t tb =
+                ((uint32_t)P.push_base[s] + (uint32_t)i * P.push_si[s] + (uint32_t)j * P.push_sj[s] + (uint32_t)k) * W;
+#pragma unroll
+            for (int v = 0; v < 5; ++v)
+              *reinterpret_cast<T*>(reinterpret_cast<char*>(dst + (uint64_t)v * vs) + tb) = T(0) + nv[v];
+          }
+        }
+      }
+      if (store) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+          *reinterpret_cast<T*>(reinterpret_cast<char*>(dst + (uint64_t)v * vs) + c) = nv[v];
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      fxm[v] = f[v];
+      u0[v] = up1[v];
+    }
+  }
+}
+
+// The marching kernel as the compiler allocates it, and capped at 8 and at 7
+// waves per SIMD (at most 64 and 72 VGPRs; the 8-wave form spills a few
+// values per plane).  drn_autotune picks among them.
+template <typename T, int RECON, int LIM, bool FAST>
+__global__ __launch_bounds__(256) void k_stage_xm(const PatchDev* __restrict__ pds, const TileMarch* __restrict__ items,
+                                                  const T* __restrict__ src, const T* base, T* dst, uint64_t vs,
+                                                  T factor, int push) {
+  stage_xm_item<T, RECON, LIM, FAST>(pds, items, src, base, dst, vs, factor, push);
+}
+template <typename T, int RECON, int LIM, bool FAST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_stage_xm8(
+    const PatchDev* __restrict__ pds, const TileMarch* __restrict__ items, const T* __restrict__ src, const T* base,
+    T* dst, uint64_t vs, T factor, int push) {
+  stage_xm_item<T, RECON, LIM, FAST>(pds, items, src, base, dst, vs, factor, push);
+}
+template <typename T, int RECON, int LIM, bool FAST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 7))) void k_stage_xm7(
+    const PatchDev* __restrict__ pds, const TileMarch* __restrict__ items, const T* __restrict__ src, const T* base,
+    T* dst, uint64_t vs, T factor, int push) {
+  stage_xm_item<T, RECON, LIM, FAST>(pds, items, src, base, dst, vs, factor, push);
+}
+
+// ---------------------------------------------------------------------------
 // cell kernel for two planes per thread
 // ---------------------------------------------------------------------------
 

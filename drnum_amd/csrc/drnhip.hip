@@ -231,7 +231,10 @@ enum {
   K_CELLZ = 5,
   K_CELLYZ = 6,
   K_CELLX2 = 7,
-  K_CELLZP = 8  // K_CELLZ items, one persistent launch
+  K_CELLZP = 8,  // K_CELLZ items, one persistent launch
+  K_XM = 9,      // the z-shared cell kernel marching its (j,k) cells over an i-range
+  K_XM8 = 10,    // K_XM capped at 8 waves per SIMD (FP32 fast arithmetic; else K_XM)
+  K_XM7 = 11     // K_XM capped at 7 waves per SIMD (FP32 fast arithmetic; else K_XM)
 };
 struct TilingChoice {
   int irange;  // planes per work item, 0 = whole patch
@@ -248,7 +251,8 @@ static const TilingChoice kTilings[] = {
     {0, 1, K_CELL},    {0, 1, K_CELL_SLOPE}, {0, 1, K_WTILE}, {16, 1, K_WTILE}, {8, 1, K_WTILE},
     {4, 1, K_WTILE},   {8, 0, K_WTILE},      {0, 1, K_CELLZ},  {0, 0, K_CELLZ},  {0, 1, K_CELLYZ},
     {0, 0, K_CELLYZ},  {0, 1, K_CELLX2},     {0, 0, K_CELLX2}, {0, 1, K_CELLZP}, {0, 0, K_CELLZP},
-    {0, 2, K_CELLZ}};
+    {0, 2, K_CELLZ},   {0, 1, K_XM},         {32, 1, K_XM},    {16, 1, K_XM},    {8, 1, K_XM},
+    {0, 1, K_XM8},     {16, 1, K_XM8},       {0, 1, K_XM7},    {16, 1, K_XM7}};
 static const int kNumTilings = sizeof(kTilings) / sizeof(kTilings[0]);
 
 // the column kernel covers the fast MUSCL forms with a symmetric limited slope
@@ -331,6 +335,16 @@ static void launch_range(drn_ctx* ctx, int kind, const void* tiles, int ntiles, 
     } else if (ntiles > 0)
       hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, FAST, 256, true>), dim3(ntiles), dim3(256), 0, ctx->stream,
                          ctx->d_pd, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor, (int)ctx->push_now);
+  } else if (kind == K_XM || kind == K_XM8 || kind == K_XM7) {
+    if (ntiles > 0) {
+      auto kern = k_stage_xm<T, RECON, LIM, FAST>;
+      if constexpr (FAST && sizeof(T) == 4) {
+        if (kind == K_XM8) kern = k_stage_xm8<T, RECON, LIM, FAST>;
+        if (kind == K_XM7) kern = k_stage_xm7<T, RECON, LIM, FAST>;
+      }
+      hipLaunchKernelGGL(kern, dim3(ntiles), dim3(256), 0, ctx->stream, ctx->d_pd, (const TileMarch*)tiles, src, base,
+                         dst, ctx->vstride, factor, (int)ctx->push_now);
+    }
   } else if (kind == K_CELLZP) {
     if (ntiles > 0) {
       // resident workgroups per CU of this instantiation (a property of the
@@ -382,6 +396,8 @@ static int effective_kind(const drn_ctx* ctx, int choice) {
   const int k = kTilings[choice].kind;
   if (k == K_COLUMN && !column_capable(ctx)) return K_MARCH;
   if (k == K_CELL_SLOPE && !column_capable(ctx)) return K_CELL;
+  const bool fast32 = ctx->prec == DRN_F32 && (ctx->arith < 0 || ctx->arith == DRN_ARITH_FAST);
+  if ((k == K_XM8 || k == K_XM7) && !fast32) return K_XM;
   return k;
 }
 
@@ -451,9 +467,11 @@ static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int l
   }
 }
 
-// The cell kernels (choices of kind K_CELL, K_CELLZ, K_CELLZP) can fill the
+// The cell kernels (choices of kind K_CELL, K_CELLZ, K_CELLZP, K_XM*) can fill the
 // fused z-face receivers themselves; the other kinds leave them to the exchange.
-static bool push_kind(int kind) { return kind == K_CELL || kind == K_CELLZ || kind == K_CELLZP; }
+static bool push_kind(int kind) {
+  return kind == K_CELL || kind == K_CELLZ || kind == K_CELLZP || kind == K_XM || kind == K_XM8 || kind == K_XM7;
+}
 
 // whether a stage of this launch form fills the fused receivers (and the
 // exchange after it then runs on the lists without them)
@@ -498,7 +516,10 @@ static void wtile_items(const drn_patch_desc& d, int p, int L, std::vector<TileW
 
 // cells of the (j,k) plane per marching work item: the column kernel's four
 // wavefronts own 63 cells each
-static int chunk_of(const drn_ctx* ctx, int choice) { return effective_kind(ctx, choice) == K_COLUMN ? 252 : MARCH_BS; }
+static int chunk_of(const drn_ctx* ctx, int choice) {
+  const int k = effective_kind(ctx, choice);
+  return k == K_COLUMN || k == K_XM || k == K_XM8 || k == K_XM7 ? 252 : MARCH_BS;
+}
 
 // build the marching work-lists of the current tiling choices of both launch forms
 static int build_march(drn_ctx* ctx) {
@@ -1752,8 +1773,8 @@ int drn_autotune(drn_ctx* ctx, int launch, int* choice) {
   for (int c = 0; c < kNumTilings && rc == DRN_OK; ++c) {
     const int ek = effective_kind(ctx, c);
     // choices that fall back to the same cell kernel on the same list are one candidate
-    if (ek == K_CELLZ || ek == K_CELL) {
-      const std::pair<int, int> key(ek, kTilings[c].xcd);
+    if (ek == K_CELLZ || ek == K_CELL || ek == K_XM) {
+      const std::pair<int, int> key(ek, kTilings[c].xcd + 4 * kTilings[c].irange);
       bool seen = false;
       for (const auto& t : tried) seen = seen || t == key;
       if (seen) continue;

@@ -3,7 +3,7 @@ kernels (drn_set_exchange_fusion): the donor's stage thread writes the value
 the exchange would copy, the receiver's thread skips its store, and the
 exchange after the stage runs on the other receivers.  Every stored cell must
 come out bit-identical to the unfused stage + exchange, for each kernel that
-fuses (cell, z-shared, persistent) in both launch forms and both precisions, on
+fuses (cell, z-shared, persistent, i-marching) in both launch forms and both precisions, on
 a block grid (jet), a periodic box (KH), the viscous operator, the level-set
 fill and a nested grid, whose sides that are not aligned unit-weight copies
 stay with the exchange.  Tilings of the other kernel families never fuse."""
@@ -32,8 +32,9 @@ CASES = {
     "uneven": (lambda: configs.uneven_batch(n_patches=30, size_range=(12, 24)), 0),
 }
 # (tiling, launch form) of each fusing kernel: cell, z-shared, persistent,
-# z-shared on the push-chain list, per-patch z-shared
-FORMS = [(11, B), (27, B), (33, B), (35, B), (28, PP)]
+# z-shared on the push-chain list, per-patch z-shared, the i-marching z-shared
+# kernel (16-plane items; capped at 8 waves per SIMD; whole patches per patch)
+FORMS = [(11, B), (27, B), (33, B), (35, B), (28, PP), (38, B), (41, B), (36, PP)]
 
 
 def setup(w, dtype, fuse):

@@ -330,13 +330,13 @@ def test_arith_modes_fp32(gpu, arith):
 
 
 @pytest.mark.parametrize("arith", [hip.DRN_ARITH_IEEE, hip.DRN_ARITH_FAST])
-@pytest.mark.parametrize("tiling", [11, 27, 0, 12, 22, 29, 31, 33])
+@pytest.mark.parametrize("tiling", [11, 27, 0, 12, 22, 29, 31, 33, 38, 41])
 def test_sod_keeps_mirror_symmetry(gpu, tiling, arith):
     """Sod along x between slip walls in y and z.  In the oracle the y and z
     momenta stay exactly 0 and every x plane stays uniform; the GPU must keep
     both exactly, in both FP32 arithmetic modes and for each kernel family
     (cell, z-shared, march, column, wave tile, y/z-shared, two-plane,
-    persistent).  A slip wall whose pressure rounds differently from the
+    persistent, i-marching z-shared).  A slip wall whose pressure rounds differently from the
     interior faces' pressure term breaks this at the wall cells."""
     n = 16
     w = configs.sod(n=n)
@@ -371,7 +371,7 @@ def test_autotune_keeps_state(gpu):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("tiling", [11, 27, 35, 33, 0, 12, 22, 29, 31])
+@pytest.mark.parametrize("tiling", [11, 27, 35, 33, 0, 12, 22, 29, 31, 38])
 def test_upwind1_rk_steps(gpu, dtype, tiling):
     """First-order Upwind1 (upwind1.h:29-39) through every kernel family: the Sod
     tube on one patch and the smooth shear layer over 8 patches (exchange every
@@ -388,7 +388,7 @@ def test_upwind1_rk_steps(gpu, dtype, tiling):
         assert err <= TOL[dtype]
 
 
-FAMILY_TILINGS = [11, 27, 35, 33, 0, 12, 22, 29, 31]
+FAMILY_TILINGS = [11, 27, 35, 33, 0, 12, 22, 29, 31, 38, 41]
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
@@ -452,11 +452,11 @@ def z_wave(zc):
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("zc", [3, 57, 59, 60, 61, 62, 63, 123])
 def test_z_neighbours_across_wavefront_edges(gpu, dtype, zc):
-    """The z-shared kernels (cell, push-chain list, persistent) against the
+    """The z-shared kernels (cell, push-chain list, persistent, i-marching) against the
     oracle, 5 RK steps, for z lengths that put the wavefront edges at every
     position relative to the rows and the patch ends."""
     w = z_wave(zc)
-    for tiling in (27, 35, 33):
+    for tiling in (27, 35, 33, 38, 41):
         og, gg = make(w, dtype)
         gg.ctx.set_tiling(tiling)
         for s in range(5):
