@@ -14,6 +14,9 @@ is one RK step over all patches.
   fp32_per_patch the FP32 arithmetic launched per patch (the batching gain).
   fp32_ieee   the FP32 batched form in the reference's arithmetic (IEEE division
               and square root instead of the hardware reciprocal).
+  reference_faces the FP32 batched form on the same jet with the other domain
+              faces as the reference's JetFlux has them (far-field lateral
+              faces, a slip wall at the far end).
   roofline    stage kernel: algorithmic bytes per launch (10 or 15 values of
               4 B per stored cell: stage 1 reads U^n only) / average launch time
               from HIP events on the context stream over the timed region.
@@ -502,6 +505,13 @@ def main():
     if not args.fp32_only:
         f32ieee = run_form(w, np.float32, args.steps, args.warmup, hip.DRN_LAUNCH_BATCHED, "fp32-ieee-batched",
                            args.tiling, arith=hip.DRN_ARITH_IEEE)
+    # the same jet with the other domain faces as the reference's JetFlux has
+    # them: far-field functors on the lateral faces, a slip wall at the far end
+    f32ref = None
+    if args.workload == "jet" and not args.fp32_only:
+        wref = configs.jet(core=args.core, blocks=args.blocks, faces="reference")
+        f32ref = run_form(wref, np.float32, args.steps, args.warmup, hip.DRN_LAUNCH_BATCHED,
+                          "fp32-batched-reference-faces", args.tiling)
     ach = f32["stage_gbs"]
     default_workload = args.workload == "jet" and (args.core, args.blocks) == (64, 4)
     # figures read from committed rocprofv3 summaries hold only for the kernel
@@ -577,6 +587,13 @@ def main():
             "tiling": f32ieee["tiling"],
             "fast_over_ieee": round(f32["mcups"] / f32ieee["mcups"], 3),
             "ieee_over_fp64_plain": None if f64 is None else round(f32ieee["mcups"] / f64["mcups"], 3),
+        },
+        "reference_faces": None if f32ref is None else {
+            # far-field lateral faces and a slip wall at the far end (JetFlux, jetflux.h:140-198)
+            "value": round(f32ref["mcups"], 2),
+            "ms_per_step": round(f32ref["ms_per_step"], 4),
+            "stage_kernel_ms": round(f32ref["stage_ms"], 4),
+            "tiling": f32ref["tiling"],
         },
         "graph": {
             "ms_per_step": round(f32["graph_ms_per_step"], 4),

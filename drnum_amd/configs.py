@@ -162,7 +162,8 @@ T_AMB = P_AMB / (1.0 * gas.R)   # rho = 1, so a = sqrt(1.4 R T) = 1
 
 
 def jet(core: int = 64, blocks: int = 4, steps: int = 1000, mach: float = 0.9, radius_cells: float = 16.0,
-        noise: float = 0.01, seed: int = 1234, coflow: float = 0.1, p_jet: float = P_AMB) -> Workload:
+        noise: float = 0.01, seed: int = 1234, coflow: float = 0.1, p_jet: float = P_AMB,
+        faces: str = "transmissive") -> Workload:
     """Subsonic top-hat jet entering through the z=0 face of a (core*blocks)^3 box.
     Ambient rho=1, p=1/1.4 (a=1) at rest; inflow |w| = mach inside radius_cells
     of the face centre with seeded +/-noise velocity perturbation (fixed in
@@ -174,12 +175,27 @@ def jet(core: int = 64, blocks: int = 4, steps: int = 1000, mach: float = 0.9, r
     negative density within a few dozen steps of the impulsive start.  As in
     the jet demo (main.h:128, jetflux.h:176-178), a jet or co-flow faster than
     sound imposes p_jet inside the radius; a subsonic jet takes p from the
-    boundary cell."""
+    boundary cell.
+
+    faces="reference" gives the other domain faces the functors of the
+    reference's JetFlux (jetflux.h:140-198, with its jet axis x mapped to z
+    here): the far end (z+) a slip wall (m_WallFlux), the four lateral faces
+    the far-field functor (m_FarfieldFlux, whose far state the demo leaves at
+    the functor's default, the gas at rest; here the ambient state at rest)."""
     nc = core * blocks
     h = 1.0 / nc
-    dom_bc = [BC_TRANSMISSIVE] * 6
+    if faces == "transmissive":
+        dom_bc = [BC_TRANSMISSIVE] * 6
+    elif faces == "reference":
+        dom_bc = [BC_FARFIELD] * 4 + [BC_TRANSMISSIVE, BC_SLIP]
+    else:
+        raise ValueError("faces must be 'transmissive' or 'reference'")
     dom_bc[4] = BC_JET
-    patches = block_grid([nc] * 3, [blocks] * 3, [1.0] * 3, 2, [False] * 3, dom_bc)
+    far = _par()
+    for f in range(4):
+        far[f, :5] = [P_AMB, T_AMB, 0.0, 0.0, 0.0]
+    patches = block_grid([nc] * 3, [blocks] * 3, [1.0] * 3, 2, [False] * 3, dom_bc,
+                         domain_par=far if faces == "reference" else None)
     for P in patches:
         if P.bc[4] == BC_JET:
             par = P.bc_par[4]
@@ -203,7 +219,8 @@ def jet(core: int = 64, blocks: int = 4, steps: int = 1000, mach: float = 0.9, r
         return np.stack([P_AMB * one, T_AMB * one, zero, zero, zero])
 
     dt = 0.5 * h / (mach + 1.0)
-    return Workload("jet%dx%d" % (blocks, core), patches, RECON_UPWIND2, LIM_VANALBADA, [0.0] * 3,
+    name = "jet%dx%d" % (blocks, core) + ("" if faces == "transmissive" else "ref")
+    return Workload(name, patches, RECON_UPWIND2, LIM_VANALBADA, [0.0] * 3,
                     [0.25, 0.5, 1.0], dt, init, steps=steps)
 
 
@@ -328,7 +345,7 @@ def levelset_spheres(core: int = 64, blocks=(3, 2, 2), n_spheres: int = 8, steps
 # ----------------------------------------------------------------------------
 
 def uneven_batch(n_patches: int = 200, size_range=(16, 96), seed: int = 2024, refine_frac: float = 0.2,
-                 steps: int = 100, mach: float = 0.5) -> Workload:
+                 steps: int = 100, mach: float = 0.5, pulse: float = 0.1) -> Workload:
     """Patches of uneven size, each with its own per-axis sizes drawn from
     U[size_range], on a lattice of rows: along x the patches of a row follow
     each other with a 2-4 cell overlap; along y and z each layer of rows starts
@@ -337,9 +354,10 @@ def uneven_batch(n_patches: int = 200, size_range=(16, 96), seed: int = 2024, re
     (a seek layer draws on one or two donor patches; seek cells no patch
     covers keep their own update).  A fraction of the patches are ratio-2
     refined blocks nested inside a coarse parent's core.  Uniform Mach 0.5 flow
-    along x plus a Gaussian density pulse (amplitude 0.1, width 8 coarse
-    cells).  Farfield functors on all faces (a patch face that is also covered
-    by a neighbour is overwritten by the exchange)."""
+    along x plus a Gaussian density pulse (amplitude `pulse`, 0.1 by default,
+    width 8 coarse cells; pulse = 0 leaves the free stream alone).  Farfield
+    functors on all faces (a patch face that is also covered by a neighbour is
+    overwritten by the exchange)."""
     rng = np.random.default_rng(seed)
     h = 1.0 / 256.0
     far = _par()
@@ -402,7 +420,7 @@ def uneven_batch(n_patches: int = 200, size_range=(16, 96), seed: int = 2024, re
 
     def init(x, y, z):
         r2 = (x - centre[0]) ** 2 + (y - centre[1]) ** 2 + (z - centre[2]) ** 2
-        rho = 1.0 + 0.1 * np.exp(-r2 / (2 * width * width))
+        rho = 1.0 + pulse * np.exp(-r2 / (2 * width * width))
         p = np.full_like(x, P_AMB)
         T = p / (rho * gas.R)
         zero = np.zeros_like(x)

@@ -1,0 +1,171 @@
+"""The stated workloads at their stated sizes, on the GPU through the C ABI.
+
+  * Sod, one 64^3 patch, FP64, 100 RK steps (AUSM+ Upwind2 van Albada,
+    alpha = 1/4, 1/2, 1): against the FP64 oracle at 1e-11.
+  * Uneven batch, all 200 patches (per-patch sizes from U[16,96], ratio-2
+    nested patches): with the pulse amplitude set to 0 the uniform Mach-0.5
+    stream stays uniform to round-off across the 8-point donor stencils of the
+    ratio-2 interfaces; with the pulse, 2 steps against the oracle.
+  * Level set, 12 patches of 64^3 core cells and 8 spheres: the FP32 form on
+    FP64 geometry (donor and mirror tables built in FP64 by libdrnum.so,
+    rounded to FP32 on upload) against the FP64 oracle for 2 steps, and the
+    FP64 form against the same oracle.
+  * Kelvin-Helmholtz, 512^3 core as 8x8x8 patches (161 M stored cells),
+    periodic on every face: FP64 sums of mass and energy over the core cells
+    constant to 1e-13 (FP64 form) and to the rounding of the stored state
+    (FP32 form) after 20 steps; the FP32 density against FP64 reported.
+
+Tolerances as in test_gpu_parity: relative max-norm per conserved variable.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from drnum_amd import configs, hip, host
+from oracle import oracle as O
+
+from test_gpu_parity import TOL, make, rel_linf
+
+pytestmark = pytest.mark.gpu
+
+
+def _host_context(w, dtype, hg, levelset=False):
+    ctx = hip.Context(dtype)
+    ctx.set_operator(w.recon, w.limiter)
+    hg.to_device(ctx)
+    hg.upload_exchange(ctx)
+    if levelset:
+        hg.upload_levelset(ctx)
+    return ctx
+
+
+def test_sod_64_fp64_100_steps(gpu):
+    w = configs.sod(n=64, steps=100)
+    og, gg = make(w, np.float64, threads=1)
+    t0 = time.time()
+    for _ in range(w.steps):
+        og.rk_step()
+    t1 = time.time()
+    for _ in range(w.steps):
+        gg.rk_step()
+    err = rel_linf(gg.state(0), og)
+    print("sod64 FP64 100 steps: oracle %.1f s, GPU vs oracle %.3e" % (t1 - t0, err), flush=True)
+    gg.close()
+    assert err <= TOL[np.float64]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_uneven_200_free_stream(gpu, dtype):
+    w = configs.uneven_batch(pulse=0.0)
+    assert len(w.patches) == 200
+    hg = host.HostGrid.from_workload(w, fp64_geometry=True)
+    hg.compute_dependencies(w.periodic)
+    strides = {d[2] for p in range(len(w.patches)) for d in hg.tables(p)["donors"]}
+    assert max(strides) == 8  # trilinear stencils at the ratio-2 interfaces
+    ctx = _host_context(w, dtype, hg)
+    U0 = []
+    for p in range(len(w.patches)):
+        U = w.initial_state(p, dtype)
+        ctx.upload(p, 0, U)
+        U0.append(U[:, :1].astype(np.float64))  # the state is uniform: one cell per patch
+    # per-variable scale: the transverse momenta vanish, so the momenta are
+    # measured against the stream's x momentum
+    scale = np.abs(U0[0][:, 0])
+    scale[1:4] = scale[1]
+    for _ in range(5):
+        ctx.rk_step(w.dt, w.alpha)
+    worst = 0.0
+    for p in range(len(w.patches)):
+        G = ctx.download(p, 0).astype(np.float64)
+        worst = max(worst, float((np.abs(G - U0[p]) / scale[:, None]).max()))
+    ctx.close()
+    tol = 1e-13 if dtype == np.float64 else 1e-5
+    print("uneven200 free stream, 5 steps, %s: %.3e of the state (tolerance %.0e)" % (np.dtype(dtype).name, worst, tol),
+          flush=True)
+    assert worst <= tol
+
+
+def test_uneven_200_pulse_against_oracle(gpu):
+    w = configs.uneven_batch()
+    og, gg = make(w, np.float64, threads=16)
+    for _ in range(2):
+        og.rk_step()
+        gg.rk_step(hip.DRN_LAUNCH_BATCHED)
+    err = rel_linf(gg.state(0), og)
+    print("uneven200 pulse, 2 steps, FP64 GPU vs oracle %.3e (%d cells)" % (err, w.total_cells), flush=True)
+    gg.close()
+    assert err <= TOL[np.float64]
+
+
+def test_levelset_12_patches_8_spheres(gpu):
+    w = configs.levelset_spheres()
+    assert len(w.patches) == 12 and len(w.spheres) == 8
+    og = O.OracleGrid(w, np.float64, threads=16)
+    og.init_state()
+    og.compute_dependencies()
+    og.levelset(w.spheres, n_inner=3, n_outer=0, min_inner_rel=0.1)
+    hg = host.HostGrid.from_workload(w, fp64_geometry=True)
+    hg.compute_dependencies(w.periodic)
+    hg.levelset_spheres(w.spheres, 3, 0, 0.1)
+    n_ls = sum(len(hg.ls_cells(p, L)["cells"]) for p in range(len(w.patches)) for L in range(3))
+    ctxs = {}
+    for dtype in (np.float64, np.float32):
+        ctx = _host_context(w, dtype, hg, levelset=True)
+        for p in range(len(w.patches)):
+            ctx.upload(p, 0, w.initial_state(p, dtype))
+        ctxs[dtype] = ctx
+    for _ in range(2):
+        og.rk_step()
+        for ctx in ctxs.values():
+            ctx.rk_step(w.dt, w.alpha)
+    errs = {}
+    for dtype, ctx in ctxs.items():
+        errs[dtype] = rel_linf([ctx.download(p, 0) for p in range(len(w.patches))], og)
+        ctx.close()
+    print("spheres 12 patches, %d level-set cells, 2 steps vs FP64 oracle: FP64 %.3e, FP32 on FP64 geometry %.3e"
+          % (n_ls, errs[np.float64], errs[np.float32]), flush=True)
+    assert n_ls > 10000
+    assert errs[np.float64] <= TOL[np.float64]
+    assert errs[np.float32] <= TOL[np.float32]
+
+
+def test_kelvin_helmholtz_512_conservation(gpu):
+    w = configs.kelvin_helmholtz()
+    assert w.total_cells == 160989184
+    hg = host.HostGrid.from_workload(w, fp64_geometry=True)
+    hg.compute_dependencies(w.periodic, threads=16)
+    c64 = _host_context(w, np.float64, hg)
+    c32 = _host_context(w, np.float32, hg)
+    t0 = time.time()
+    for p in range(len(w.patches)):
+        U = w.initial_state(p, np.float64)
+        c64.upload(p, 0, U)
+        c32.upload(p, 0, U.astype(np.float32))
+    print("kh512: initial state of 512 patches uploaded in %.1f s" % (time.time() - t0), flush=True)
+    s64, s32 = c64.core_sums(0), c32.core_sums(0)
+    n_steps = 20
+    t0 = time.time()
+    for _ in range(n_steps):
+        c64.rk_step(w.dt, w.alpha)
+        c32.rk_step(w.dt, w.alpha)
+    c64.sync()
+    c32.sync()
+    print("kh512: %d steps of both forms in %.1f s" % (n_steps, time.time() - t0), flush=True)
+    e64, e32 = c64.core_sums(0), c32.core_sums(0)
+    d64 = np.abs(e64 - s64)[[0, 4]] / np.abs(s64)[[0, 4]]
+    d32 = np.abs(e32 - s32)[[0, 4]] / np.abs(s32)[[0, 4]]
+    rho_err = 0.0
+    rho_max = 0.0
+    for p in range(len(w.patches)):
+        a = c64.download(p, 0)[0]
+        b = c32.download(p, 0)[0].astype(np.float64)
+        rho_err = max(rho_err, float(np.abs(a - b).max()))
+        rho_max = max(rho_max, float(np.abs(a).max()))
+    c64.close()
+    c32.close()
+    print("kh512 %d steps: FP64 mass/energy drift %s, FP32 %s, FP32 density vs FP64 %.3e (relative)"
+          % (n_steps, d64, d32, rho_err / rho_max), flush=True)
+    assert np.all(d64 <= 1e-13)
+    assert np.all(d32 <= 1e-5)
+    assert rho_err / rho_max <= 1e-3
