@@ -34,6 +34,27 @@ class Patch:
     bc: List[int]              # face functor: xm, xp, ym, yp, zm, zp
     bc_par: np.ndarray         # (6, NPAR) float64
     gidx: List[int]            # global index of local cell (0,0,0) (for lattice grids)
+    # base vectors of the local i and j axes in the inertial frame, as a grid
+    # file gives them (patch.cpp:63-85); the default is axis aligned
+    base_i: List[float] = dataclasses.field(default_factory=lambda: [1.0, 0.0, 0.0])
+    base_j: List[float] = dataclasses.field(default_factory=lambda: [0.0, 1.0, 0.0])
+
+    @property
+    def aligned(self) -> bool:
+        return list(self.base_i) == [1.0, 0.0, 0.0] and list(self.base_j) == [0.0, 1.0, 0.0]
+
+    def rotation(self) -> np.ndarray:
+        """Local -> inertial rotation (columns: unit i, j, k axes), the matrix of
+        CoordTransform::setMatrixFromBaseIJ (coordtransform.cpp:37-58) in float64."""
+        bi = np.asarray(self.base_i, dtype=np.float64)
+        bj = np.asarray(self.base_j, dtype=np.float64)
+        bi = bi / np.linalg.norm(bi)
+        bj = bj / np.linalg.norm(bj)
+        bk = np.cross(bi, bj)
+        bj = bj - (bi @ bj) * bi
+        bj = bj / np.linalg.norm(bj)
+        bk = bk / np.linalg.norm(bk)
+        return np.stack([bi, bj, bk], axis=1)
 
     @property
     def cells(self) -> int:
@@ -73,17 +94,29 @@ class Workload:
         return sum(p.core_cells for p in self.patches)
 
     def cell_centres(self, p: int):
-        """Global cell-centre coordinates (float64) of patch p, flattened i,j,k with k fastest."""
+        """Inertial cell-centre coordinates (float64) of patch p, flattened i,j,k
+        with k fastest: origin + R (local centre) for a rotated patch."""
         P = self.patches[p]
-        xs = [P.origin[a] + (np.arange(n) + 0.5) * P.dx(a) for a, n in enumerate((P.ni, P.nj, P.nk))]
+        if P.aligned:
+            xs = [P.origin[a] + (np.arange(n) + 0.5) * P.dx(a) for a, n in enumerate((P.ni, P.nj, P.nk))]
+            X, Y, Z = np.meshgrid(xs[0], xs[1], xs[2], indexing="ij")
+            return X.ravel(), Y.ravel(), Z.ravel()
+        xs = [(np.arange(n) + 0.5) * P.dx(a) for a, n in enumerate((P.ni, P.nj, P.nk))]
         X, Y, Z = np.meshgrid(xs[0], xs[1], xs[2], indexing="ij")
-        return X.ravel(), Y.ravel(), Z.ravel()
+        loc = np.stack([X.ravel(), Y.ravel(), Z.ravel()])
+        g = P.rotation() @ loc + np.asarray(P.origin, dtype=np.float64)[:, None]
+        return g[0], g[1], g[2]
 
     def initial_state(self, p: int, dtype) -> np.ndarray:
         """Conserved variables (5, N) of patch p in `dtype`, converted with the
-        reference's primitiveToConservative (perfectgas.h:56-64)."""
+        reference's primitiveToConservative (perfectgas.h:56-64).  A rotated
+        patch holds its velocity in its own frame: R^T u."""
+        P = self.patches[p]
         x, y, z = self.cell_centres(p)
         prim = self.init(x, y, z)
+        if not P.aligned:
+            prim = np.array(prim, dtype=np.float64, copy=True)
+            prim[2:5] = P.rotation().T @ prim[2:5]
         return gas.prim_to_cons(prim[0], prim[1], prim[2], prim[3], prim[4], dtype)
 
 
@@ -429,3 +462,80 @@ def uneven_batch(n_patches: int = 200, size_range=(16, 96), seed: int = 2024, re
     dt = 0.5 * (h / 2.0) / (mach + 1.2)
     return Workload("uneven%d" % n_patches, patches, RECON_UPWIND2, LIM_VANALBADA, [0.0] * 3,
                     [0.25, 0.5, 1.0], dt, init, steps=steps)
+
+
+# ----------------------------------------------------------------------------
+# Rotated patches: a star of rotated patches around an unrotated centre patch
+# ----------------------------------------------------------------------------
+
+def _rot_z(phi: float) -> np.ndarray:
+    c, s = np.cos(phi), np.sin(phi)
+    return np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+
+
+def rotated_star(core: int = 16, seek: int = 2, steps: int = 10, flow=(0.3, 0.2, 0.1), pulse: float = 0.1,
+                 tilt_deg: float = 0.0, arms: str = "xyz") -> Workload:
+    """An unrotated centre patch C at the inertial origin and three rotated
+    patches on its +x, +y and +z sides, each overlapping C by `seek` seek
+    layers on both sides (patch files give each patch's base vectors,
+    patch.cpp:63-85):
+
+      C   cells [0, core+seek)^3, seek layers on its x, y and z max faces;
+      RX  x in [core-seek, 2 core), y, z in [0, core): turned 90 degrees about z
+          (local i = +y, local j = -x, local k = +z), seek layers on local j max;
+      RY  y in [core-seek, 2 core), x, z in [0, core): turned 90 degrees about x
+          (local i = +x, local j = +z, local k = -y), seek layers on local k max;
+      RZ  z in [core-seek, 2 core), x, y in [0, core): turned 180 degrees about z
+          (local i = -x, local j = -y, local k = +z), seek layers on local k min.
+
+    Every exchange relation joins a rotated patch with C, which is the
+    identity transform: for such pairs the reference's relative transform
+    (CoordTransformVV::setTransFromTo) is the geometric one.  With tilt_deg = 0
+    the rotated patches' cells coincide with C's, so every donor weight is
+    exactly 1 and the momentum is turned by an exact signed permutation.
+    tilt_deg turns RX further about the z axis through the middle of its
+    overlap with C (trilinear donor weights, a general turn matrix); the
+    tilted RX then also overlaps RY and RZ, two turned patches, for which
+    the reference's composition is not the geometric transform (unless both
+    turn about one axis).  `arms` selects which of RX, RY, RZ are present
+    (C's faces without an arm keep no seek layers).
+
+    Uniform flow `flow` (times the sound speed, inertial frame) plus a
+    Gaussian density pulse (amplitude `pulse`) in C; transmissive faces."""
+    h = 1.0 / 64.0
+    c, s = core, seek
+    tr = [BC_TRANSMISSIVE] * 6
+
+    def patch(n, seeks, origin, base_i, base_j):
+        return Patch(n[0], n[1], n[2], list(seeks), [float(o) for o in origin], [n[a] * h for a in range(3)],
+                     list(tr), _par(), [0, 0, 0], [float(b) for b in base_i], [float(b) for b in base_j])
+
+    sx, sy, sz = (s if a in arms else 0 for a in "xyz")
+    C = patch((c + sx, c + sy, c + sz), (0, sx, 0, sy, 0, sz), (0.0, 0.0, 0.0), (1, 0, 0), (0, 1, 0))
+    rx_o = np.array([2 * c * h, 0.0, 0.0])
+    rx_bi, rx_bj = np.array([0.0, 1.0, 0.0]), np.array([-1.0, 0.0, 0.0])
+    if tilt_deg != 0.0:
+        R = _rot_z(np.radians(tilt_deg))
+        pivot = np.array([c * h, 0.5 * c * h, 0.0])
+        rx_o = pivot + R @ (rx_o - pivot)
+        rx_bi, rx_bj = R @ rx_bi, R @ rx_bj
+    RX = patch((c, c + s, c), (0, 0, 0, s, 0, 0), rx_o, rx_bi, rx_bj)
+    RY = patch((c, c, c + s), (0, 0, 0, 0, 0, s), (0.0, 2 * c * h, 0.0), (1, 0, 0), (0, 0, 1))
+    RZ = patch((c, c, c + s), (0, 0, 0, 0, s, 0), (c * h, c * h, (c - s) * h), (-1, 0, 0), (0, -1, 0))
+    patches = [C] + [P for a, P in zip("xyz", (RX, RY, RZ)) if a in arms]
+    centre = np.array([0.5 * c * h] * 3)
+    width = 3 * h
+    a0 = 1.0
+
+    def init(x, y, z):
+        r2 = (x - centre[0]) ** 2 + (y - centre[1]) ** 2 + (z - centre[2]) ** 2
+        rho = 1.0 + pulse * np.exp(-r2 / (2 * width * width))
+        p = np.full_like(x, P_AMB)
+        T = p / (rho * gas.R)
+        return np.stack([p, T, flow[0] * a0 * np.ones_like(x), flow[1] * a0 * np.ones_like(x),
+                         flow[2] * a0 * np.ones_like(x)])
+
+    dt = 0.4 * h / (np.linalg.norm(flow) * a0 + 1.2 * a0)
+    return Workload("rotstar%d" % core + ("_tilt%g" % tilt_deg if tilt_deg else "") +
+                    ("" if arms == "xyz" else "_" + arms), patches, RECON_UPWIND2,
+                    LIM_VANALBADA, [0.0] * 3, [0.25, 0.5, 1.0], dt, init, steps=steps)

@@ -2188,14 +2188,18 @@ __global__ __launch_bounds__(256) void k_exchange_single(uint32_t n, int stride,
   for (int v = 0; v < 5; ++v) U[v * vs + g] = T(0) + acc[v];
 }
 
-// receivers served by several donor patches: donor sums added in donor order
-template <typename T>
+// receivers served by several donor patches, or by a turned one: donor sums
+// added in donor order.  TURN: a group with gturn[g] = t > 0 turns the momentum
+// of its donor sum into the receiver's frame with the row-major matrix t - 1 of
+// `turn` before adding it (patch.cpp:617-645: inter_vars, turned, added).
+template <typename T, bool TURN>
 __global__ __launch_bounds__(256) void k_exchange_multi(uint32_t n, const uint32_t* __restrict__ rec,
                                                         const uint32_t* __restrict__ gptr,
                                                         const uint32_t* __restrict__ gstride,
                                                         const uint32_t* __restrict__ gwi,
                                                         const uint32_t* __restrict__ idx, const T* __restrict__ w,
-                                                        T* U, uint64_t vs) {
+                                                        T* U, uint64_t vs, const uint32_t* __restrict__ gturn,
+                                                        const T* __restrict__ turn) {
 #pragma clang fp contract(off)
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
@@ -2210,6 +2214,16 @@ __global__ __launch_bounds__(256) void k_exchange_multi(uint32_t n, const uint32
       const T ww = w[w0 + s];
 #pragma unroll
       for (int v = 0; v < 5; ++v) acc[v] += U[v * vs + d] * ww;
+    }
+    if constexpr (TURN) {
+      const uint32_t t = gturn[g];
+      if (t > 0) {
+        const T* a = turn + 9 * (uint64_t)(t - 1);
+        const T m1 = acc[1], m2 = acc[2], m3 = acc[3];
+        acc[1] = a[0] * m1 + a[1] * m2 + a[2] * m3;
+        acc[2] = a[3] * m1 + a[4] * m2 + a[5] * m3;
+        acc[3] = a[6] * m1 + a[7] * m2 + a[8] * m3;
+      }
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) tot[v] += acc[v];

@@ -26,7 +26,18 @@ struct ExchangeTables {
   std::vector<int64_t> donor_info;  // 4 per donor: nrec, stride, rec_start, wi_start
   std::vector<int64_t> rec_unique, rec_concat, donor_index;
   std::vector<double> donor_weight;
+  // 9 per donor (row-major), the turn of the momentum from the donor's frame
+  // into the receiver's (donor_t axx..azz); empty when every donor is aligned
+  std::vector<double> turn;
 };
+
+// a turn matrix that changes no finite vector: exact 1 on the diagonal, zeros off it
+static bool is_identity_turn(const double* a) {
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      if (a[3 * r + c] != (r == c ? 1.0 : 0.0)) return false;
+  return true;
+}
 
 struct LsTables {
   std::vector<int64_t> cells, donors;
@@ -117,6 +128,8 @@ struct drn_ctx {
     uint32_t multi_n = 0;
     uint32_t *m_rec = nullptr, *m_gptr = nullptr, *m_gstride = nullptr, *m_gwi = nullptr, *m_idx = nullptr;
     void* m_w = nullptr;
+    uint32_t* m_gturn = nullptr;  // per group: 0 = aligned, t > 0 = turn matrix t - 1 of m_turn
+    void* m_turn = nullptr;       // 9 values per turn matrix, in the state precision
   };
   ExchangeLists xfull, xpart;
   bool have_exchange = false;
@@ -633,9 +646,14 @@ static void launch_exchange_t(drn_ctx* ctx, const drn_ctx::ExchangeLists& x, T* 
     hipLaunchKernelGGL(k_exchange_single<T>, dim3((c.n + 255) / 256), dim3(256), 0, ctx->stream, c.n, c.stride, c.rec,
                        c.idx, (const T*)c.w, U, ctx->vstride);
   }
-  if (x.multi_n > 0) {
-    hipLaunchKernelGGL(k_exchange_multi<T>, dim3((x.multi_n + 255) / 256), dim3(256), 0, ctx->stream, x.multi_n,
-                       x.m_rec, x.m_gptr, x.m_gstride, x.m_gwi, x.m_idx, (const T*)x.m_w, U, ctx->vstride);
+  if (x.multi_n > 0 && x.m_gturn) {
+    hipLaunchKernelGGL((k_exchange_multi<T, true>), dim3((x.multi_n + 255) / 256), dim3(256), 0, ctx->stream,
+                       x.multi_n, x.m_rec, x.m_gptr, x.m_gstride, x.m_gwi, x.m_idx, (const T*)x.m_w, U, ctx->vstride,
+                       x.m_gturn, (const T*)x.m_turn);
+  } else if (x.multi_n > 0) {
+    hipLaunchKernelGGL((k_exchange_multi<T, false>), dim3((x.multi_n + 255) / 256), dim3(256), 0, ctx->stream,
+                       x.multi_n, x.m_rec, x.m_gptr, x.m_gstride, x.m_gwi, x.m_idx, (const T*)x.m_w, U, ctx->vstride,
+                       nullptr, nullptr);
   }
 }
 
@@ -758,8 +776,10 @@ static void free_lists(drn_ctx::ExchangeLists& x) {
   dfree(x.m_gwi);
   dfree(x.m_idx);
   dfree(x.m_w);
-  x.m_rec = x.m_gptr = x.m_gstride = x.m_gwi = x.m_idx = nullptr;
-  x.m_w = nullptr;
+  dfree(x.m_gturn);
+  dfree(x.m_turn);
+  x.m_rec = x.m_gptr = x.m_gstride = x.m_gwi = x.m_idx = x.m_gturn = nullptr;
+  x.m_w = x.m_turn = nullptr;
   x.multi_n = 0;
 }
 
@@ -1145,6 +1165,19 @@ int drn_exchange_set(drn_ctx* ctx, int recv_id, int n_donors, const int32_t* don
   return DRN_OK;
 }
 
+int drn_exchange_set_turn(drn_ctx* ctx, int recv_id, int n_donors, const double* turn) {
+  if (!ctx) return DRN_E_ARG;
+  auto it = ctx->xt.find(recv_id);
+  if (it == ctx->xt.end()) return fail(ctx, DRN_E_STATE, "drn_exchange_set_turn: no tables set for this receiver patch");
+  if (n_donors != (int)it->second.donor_patch.size())
+    return fail(ctx, DRN_E_ARG, "drn_exchange_set_turn: donor count differs from the tables");
+  if (n_donors > 0 && !turn) return fail(ctx, DRN_E_ARG, "drn_exchange_set_turn: null matrix pointer");
+  for (int64_t a = 0; a < 9 * (int64_t)n_donors; ++a)
+    if (!std::isfinite(turn[a])) return fail(ctx, DRN_E_ARG, "drn_exchange_set_turn: non-finite matrix entry");
+  it->second.turn.assign(turn, turn + 9 * (size_t)n_donors);
+  return DRN_OK;
+}
+
 template <typename T>
 static int upload_vec(drn_ctx* ctx, T** dptr, const std::vector<T>& h) {
   HIPCHK(ctx, dalloc(dptr, h.size()));
@@ -1241,9 +1274,11 @@ int drn_exchange_commit(drn_ctx* ctx) {
   // per unique receiver: its contribution groups in donor order
   struct Group {
     int stride;
+    int turn = 0;  // 0 = aligned, t > 0 = matrix t - 1 of `turns`
     std::vector<uint32_t> idx;
     std::vector<double> w;
   };
+  std::vector<double> turns;  // 9 per turned (receiver patch, donor) relation
   std::map<int, std::vector<std::pair<uint32_t, std::vector<Group>>>> single_by_stride;
   std::vector<std::pair<uint32_t, std::vector<Group>>> multi;
   std::vector<bool> fused(ctx->vstride, false);  // receivers the stage kernel fills when fused
@@ -1257,9 +1292,15 @@ int drn_exchange_commit(drn_ctx* ctx) {
       const int dp = t.donor_patch[d];
       const int64_t nrec = t.donor_info[4 * d], stride = t.donor_info[4 * d + 1], r0 = t.donor_info[4 * d + 2],
                     w0 = t.donor_info[4 * d + 3];
+      int turn = 0;
+      if (!t.turn.empty() && !is_identity_turn(&t.turn[9 * d])) {
+        turns.insert(turns.end(), t.turn.begin() + 9 * d, t.turn.begin() + 9 * (d + 1));
+        turn = (int)(turns.size() / 9);
+      }
       for (int64_t ll = 0; ll < nrec; ++ll) {
         Group g;
         g.stride = (int)stride;
+        g.turn = turn;
         for (int64_t s = 0; s < stride; ++s) {
           g.idx.push_back((uint32_t)(ctx->offs[dp] + (uint64_t)t.donor_index[w0 + ll * stride + s]));
           g.w.push_back(t.donor_weight[w0 + ll * stride + s]);
@@ -1287,7 +1328,7 @@ int drn_exchange_commit(drn_ctx* ctx) {
               const auto it = groups.find((int64_t)i * njk + (int64_t)j * d.nk + k);
               if (it == groups.end() || it->second.size() != 1) return false;
               const Group& g = it->second[0];
-              if (g.stride != 1 || g.w[0] != 1.0) return false;
+              if (g.stride != 1 || g.w[0] != 1.0 || g.turn != 0) return false;
               const int dq = patch_of(ctx, g.idx[0]);
               const drn_patch_desc& e = ctx->descs[dq];
               const int64_t enjk = (int64_t)e.nj * e.nk, dl = (int64_t)(g.idx[0] - ctx->offs[dq]);
@@ -1324,7 +1365,11 @@ int drn_exchange_commit(drn_ctx* ctx) {
       auto it = groups.find(r);
       if (it == groups.end()) return fail(ctx, DRN_E_RANGE, "unique receiver without donor contribution");
       const uint32_t g = (uint32_t)(ctx->offs[p] + (uint64_t)r);
-      if (it->second.size() == 1)
+      // a receiver with a turned donor goes to the multi-donor list, whose
+      // kernel turns the momentum of each donor sum
+      bool turned = false;
+      for (const Group& gr : it->second) turned = turned || gr.turn != 0;
+      if (it->second.size() == 1 && !turned)
         single_by_stride[it->second[0].stride].push_back({g, std::move(it->second)});
       else
         multi.push_back({g, std::move(it->second)});
@@ -1371,12 +1416,13 @@ int drn_exchange_commit(drn_ctx* ctx) {
       x.single.push_back(c);  // freed with the lists, also after a failed upload
       if (rc) return fail(ctx, DRN_E_NOMEM, "exchange list upload failed");
     }
-    std::vector<uint32_t> rec, gptr{0}, gstride, gwi, idx;
+    std::vector<uint32_t> rec, gptr{0}, gstride, gwi, idx, gturn;
     std::vector<double> w;
     for (const auto& e : multi) {
       if (skip_fused && fused[e.first]) continue;
       rec.push_back(e.first);
       for (const auto& g : e.second) {
+        gturn.push_back((uint32_t)g.turn);
         gstride.push_back((uint32_t)g.stride);
         gwi.push_back((uint32_t)idx.size());
         idx.insert(idx.end(), g.idx.begin(), g.idx.end());
@@ -1388,6 +1434,8 @@ int drn_exchange_commit(drn_ctx* ctx) {
       x.multi_n = (uint32_t)rec.size();
       if (upload_vec(ctx, &x.m_rec, rec) || upload_vec(ctx, &x.m_gptr, gptr) || upload_vec(ctx, &x.m_gstride, gstride) ||
           upload_vec(ctx, &x.m_gwi, gwi) || upload_vec(ctx, &x.m_idx, idx) || upload_weights(ctx, &x.m_w, w))
+        return fail(ctx, DRN_E_NOMEM, "exchange list upload failed");
+      if (!turns.empty() && (upload_vec(ctx, &x.m_gturn, gturn) || upload_weights(ctx, &x.m_turn, turns)))
         return fail(ctx, DRN_E_NOMEM, "exchange list upload failed");
     }
     return DRN_OK;

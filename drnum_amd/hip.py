@@ -49,6 +49,7 @@ _SIGS = {
     "drn_stage": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int]),
     "drn_exchange_set": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
                                    C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "drn_exchange_set_turn": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "drn_exchange_commit": (C.c_int, [C.c_void_p]),
     "drn_exchange": (C.c_int, [C.c_void_p, C.c_int]),
     "drn_set_exchange_fusion": (C.c_int, [C.c_void_p, C.c_int]),
@@ -184,6 +185,9 @@ class Context:
         self._chk(self.L.drn_exchange_set(self.h, pid, len(donors), _p(dp), _p(info), ru.size, _p(ru), rc_.size,
                                           _p(rc_), di.size, _p(di), _p(dw)), "drn_exchange_set")
         del keep
+        if tables.get("turn") is not None and len(donors) > 0:
+            t = np.ascontiguousarray(tables["turn"], dtype=np.float64).reshape(len(donors), 9)
+            self._chk(self.L.drn_exchange_set_turn(self.h, pid, len(donors), _p(t)), "drn_exchange_set_turn")
 
     def commit_exchange(self):
         self._chk(self.L.drn_exchange_commit(self.h), "drn_exchange_commit")

@@ -33,6 +33,10 @@ _SIGS = {
     "drnh_grid_rec_concat": (C.c_int, [C.c_void_p, C.c_int, _I64P]),
     "drnh_grid_donor_index": (C.c_int, [C.c_void_p, C.c_int, _I64P]),
     "drnh_grid_donor_weight": (C.c_int, [C.c_void_p, C.c_int, _DP]),
+    "drnh_grid_donor_turn": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _DP]),
+    "drnh_grid_set_base": (C.c_int, [C.c_void_p, C.c_int, _DP, _DP]),
+    "drnh_grid_patch_base": (C.c_int, [C.c_void_p, C.c_int, _DP]),
+    "drnh_grid_patch_transform": (C.c_int, [C.c_void_p, C.c_int, _DP]),
     "drnh_grid_levelset_spheres": (C.c_int, [C.c_void_p, C.c_int, _DP, C.c_int, C.c_int, C.c_double]),
     "drnh_grid_levelset_values": (C.c_int, [C.c_void_p, C.c_int, _DP]),
     "drnh_grid_ls_num_cells": (C.c_int64, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
@@ -168,7 +172,30 @@ class HostGrid:
             pid = g._chk(g.L.drnh_grid_add_patch(g.h, P.ni, P.nj, P.nk, _ptr(seek, C.c_int32), _ptr(o, C.c_double),
                                                  _ptr(L, C.c_double)), "drnh_grid_add_patch")
             g.set_faces(pid, P.bc, P.bc_par)
+            if not getattr(P, "aligned", True):
+                g.set_base(pid, P.base_i, P.base_j)
         return g
+
+    def set_base(self, p, base_i, base_j):
+        bi = np.array(base_i, dtype=np.float64)
+        bj = np.array(base_j, dtype=np.float64)
+        self._chk(self.L.drnh_grid_set_base(self.h, p, _ptr(bi, C.c_double), _ptr(bj, C.c_double)), "set_base")
+
+    def patch_base(self, p):
+        out = np.zeros(6, np.float64)
+        self._chk(self.L.drnh_grid_patch_base(self.h, p, _ptr(out, C.c_double)), "patch_base")
+        return out[:3], out[3:]
+
+    def patch_transform(self, p):
+        """patch p's this->inertial transform as Patch::xyzoCell applies it: (A, b)"""
+        out = np.zeros(12, np.float64)
+        self._chk(self.L.drnh_grid_patch_transform(self.h, p, _ptr(out, C.c_double)), "patch_transform")
+        return out[:9].reshape(3, 3), out[9:]
+
+    def donor_turn(self, p, d):
+        out = np.zeros(9, np.float64)
+        self._chk(self.L.drnh_grid_donor_turn(self.h, p, d, _ptr(out, C.c_double)), "donor_turn")
+        return out.reshape(3, 3)
 
     def read(self, path, scale=1.0):
         self._chk(self.L.drnh_grid_read(self.h, path.encode(), scale), "drnh_grid_read")
@@ -231,8 +258,9 @@ class HostGrid:
         L.drnh_grid_rec_concat(h, p, _ptr(rc, C.c_int64))
         L.drnh_grid_donor_index(h, p, _ptr(di, C.c_int64))
         L.drnh_grid_donor_weight(h, p, _ptr(dw, C.c_double))
+        turn = np.array([self.donor_turn(p, d).ravel() for d in range(nd)]).reshape(nd, 9)
         return dict(receivers=recv[:nrec], donors=donors, rec_concat=rc[:nrc], donor_index=di[:nwi],
-                    donor_weight=dw[:nwi], no_donor=int(L.drnh_grid_num_no_donor(h, p)))
+                    donor_weight=dw[:nwi], no_donor=int(L.drnh_grid_num_no_donor(h, p)), turn=turn)
 
     def levelset_spheres(self, spheres, n_inner=3, n_outer=0, min_inner_rel=0.1):
         s = np.ascontiguousarray(spheres, dtype=np.float64).reshape(-1)

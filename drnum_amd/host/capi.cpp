@@ -257,6 +257,54 @@ int drnh_grid_donor_weight(const drnh_grid* g, int p, double* out) {
   });
 }
 
+int drnh_grid_donor_turn(const drnh_grid* g, int p, int d, double* out) {
+  if (bad_patch(g, p) || !out) return -1;
+  return with(g, [&](const auto& G) {
+    const auto& D = G.tables(p).donors;
+    if (d < 0 || d >= (int)D.size()) return -1;
+    memcpy(out, D[(size_t)d].turn, 9 * sizeof(double));
+    return 0;
+  });
+}
+
+int drnh_grid_set_base(drnh_grid* g, int p, const double* bi, const double* bj) {
+  if (bad_patch(g, p) || !bi || !bj) return -1;
+  const double cx = bi[1] * bj[2] - bi[2] * bj[1], cy = bi[2] * bj[0] - bi[0] * bj[2], cz = bi[0] * bj[1] - bi[1] * bj[0];
+  const double ni2 = bi[0] * bi[0] + bi[1] * bi[1] + bi[2] * bi[2], nj2 = bj[0] * bj[0] + bj[1] * bj[1] + bj[2] * bj[2];
+  if (!(ni2 > 0) || !(nj2 > 0) || !(cx * cx + cy * cy + cz * cz > 1e-12 * ni2 * nj2)) {
+    g->err = "patch base vectors are zero or parallel";
+    return -1;
+  }
+  return with(g, [&](auto& G) {
+    G.patch(p).setBase(bi, bj);
+    return 0;
+  });
+}
+
+int drnh_grid_patch_base(const drnh_grid* g, int p, double* out) {
+  if (bad_patch(g, p) || !out) return -1;
+  return with(g, [&](const auto& G) {
+    const auto& P = G.patch(p);
+    for (int a = 0; a < 3; ++a) {
+      out[a] = (double)P.baseI(a);
+      out[3 + a] = (double)P.baseJ(a);
+    }
+    return 0;
+  });
+}
+
+int drnh_grid_patch_transform(const drnh_grid* g, int p, double* out) {
+  if (bad_patch(g, p) || !out) return -1;
+  return with(g, [&](const auto& G) {
+    const auto& t = G.patch(p).this2inertial();
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c) out[3 * r + c] = (double)t.a[r][c];
+      out[9 + r] = (double)t.b[r];
+    }
+    return 0;
+  });
+}
+
 int drnh_grid_levelset_spheres(drnh_grid* g, int n_spheres, const double* spheres, int n_inner, int n_outer,
                                double min_inner_rel) {
   if (!g || n_spheres < 1 || !spheres || n_inner < 0 || n_outer < 0) return -1;
@@ -380,6 +428,20 @@ int drnh_grid_upload_exchange(drnh_grid* g, drn_ctx* ctx) {
       if (r) {
         g->err = std::string("drn_exchange_set: ") + drn_last_error(ctx);
         return r;
+      }
+      // the momentum's turn, where a donor is not aligned with this patch
+      std::vector<double> turn;
+      bool aligned = true;
+      for (const auto& D : T.donors) {
+        turn.insert(turn.end(), D.turn, D.turn + 9);
+        for (int e = 0; e < 9; ++e) aligned = aligned && D.turn[e] == (e % 4 == 0 ? 1.0 : 0.0);
+      }
+      if (!aligned) {
+        r = drn_exchange_set_turn(ctx, (int)p, (int)T.donors.size(), turn.data());
+        if (r) {
+          g->err = std::string("drn_exchange_set_turn: ") + drn_last_error(ctx);
+          return r;
+        }
       }
     }
     int r = drn_exchange_commit(ctx);

@@ -78,12 +78,122 @@ void WeightSet<G>::shiftToSum(G sum) {
 }
 
 // ---------------------------------------------------------------------------
+// Transform
+// ---------------------------------------------------------------------------
+
+template <typename G>
+Transform<G> Transform<G>::identity() {
+  Transform t;
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) t.a[r][c] = r == c ? G(1) : G(0);
+    t.b[r] = 0;
+  }
+  return t;
+}
+
+namespace {
+
+// MathVector::abs and normalise (mathvector_methods.h:122-127, 178-183)
+template <typename G>
+G abs3(const G* v) {
+  G l = 0;
+  for (int i = 0; i < 3; ++i) l += v[i] * v[i];
+  return std::sqrt(l);
+}
+template <typename G>
+void normalise3(G* v) {
+  const G l = abs3(v);
+  for (int i = 0; i < 3; ++i) v[i] /= l;
+}
+
+}  // namespace
+
+template <typename G>
+void Transform<G>::setBaseIJ(const G* base_i, const G* base_j) {
+  G bi[3] = {base_i[0], base_i[1], base_i[2]}, bj[3] = {base_j[0], base_j[1], base_j[2]}, bk[3];
+  normalise3(bi);
+  normalise3(bj);
+  // cross product (mathvector_methods.h:86-93)
+  bk[0] = bi[1] * bj[2] - bi[2] * bj[1];
+  bk[1] = bi[2] * bj[0] - bi[0] * bj[2];
+  bk[2] = bi[0] * bj[1] - bi[1] * bj[0];
+  const G bk_len = abs3(bk);
+  G scal_ij = 0;  // scalar product (mathvector_operators.h:208-216)
+  for (int i = 0; i < 3; ++i) scal_ij += bi[i] * bj[i];
+  for (int i = 0; i < 3; ++i) bj[i] = bj[i] - scal_ij * bi[i];
+  normalise3(bj);
+  // (1./len) * base_k: the double quotient becomes the `real` factor of operator*
+  const G inv_len = G(1. / bk_len);
+  for (int i = 0; i < 3; ++i) bk[i] = inv_len * bk[i];
+  for (int i = 0; i < 3; ++i) {
+    a[i][0] = bi[i];
+    a[i][1] = bj[i];
+    a[i][2] = bk[i];
+  }
+}
+
+template <typename G>
+Transform<G> Transform<G>::inverse() const {
+  // InvSmallSquareMatrix<T,3> (smallsquarematrix.h:470-489), the branch of a
+  // matrix without precision checks, then the shift -A^-1 b as M*v
+  // (smallsquarematrix.h:209-218)
+  const G(*M)[3] = a;
+  const G t4 = M[0][0] * M[1][1];
+  const G t6 = M[0][0] * M[1][2];
+  const G t8 = M[0][1] * M[1][0];
+  const G t10 = M[0][2] * M[1][0];
+  const G t12 = M[0][1] * M[2][0];
+  const G t14 = M[0][2] * M[2][0];
+  const G t17 = 1 / (t4 * M[2][2] - t6 * M[2][1] - t8 * M[2][2] + t10 * M[2][1] + t12 * M[1][2] - t14 * M[1][1]);
+  Transform t;
+  t.a[0][0] = (M[1][1] * M[2][2] - M[1][2] * M[2][1]) * t17;
+  t.a[0][1] = -(M[0][1] * M[2][2] - M[0][2] * M[2][1]) * t17;
+  t.a[0][2] = -(-M[0][1] * M[1][2] + M[0][2] * M[1][1]) * t17;
+  t.a[1][0] = -(M[1][0] * M[2][2] - M[1][2] * M[2][0]) * t17;
+  t.a[1][1] = (M[0][0] * M[2][2] - t14) * t17;
+  t.a[1][2] = -(t6 - t10) * t17;
+  t.a[2][0] = -(-M[1][0] * M[2][1] + M[1][1] * M[2][0]) * t17;
+  t.a[2][1] = -(M[0][0] * M[2][1] - t12) * t17;
+  t.a[2][2] = (t4 - t8) * t17;
+  const G v[3] = {-b[0], -b[1], -b[2]};
+  for (int i = 0; i < 3; ++i) {
+    t.b[i] = 0;
+    for (int j = 0; j < 3; ++j) t.b[i] += t.a[i][j] * v[j];
+  }
+  return t;
+}
+
+template <typename G>
+Transform<G> Transform<G>::concatenate(const Transform& c) const {
+  Transform t;
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) t.a[i][j] = a[i][0] * c.a[0][j] + a[i][1] * c.a[1][j] + a[i][2] * c.a[2][j];
+    t.b[i] = a[i][0] * c.b[0] + a[i][1] * c.b[1] + a[i][2] * c.b[2] + b[i];
+  }
+  return t;
+}
+
+template <typename G>
+void Transform<G>::transform(G x, G y, G z, G* out) const {
+  for (int i = 0; i < 3; ++i) out[i] = a[i][0] * x + a[i][1] * y + a[i][2] * z + b[i];
+}
+
+template <typename G>
+void setupTransformation(const G* origin, const G* base_i, const G* base_j, Transform<G>& i2t, Transform<G>& t2i) {
+  Transform<G> this2inertial = Transform<G>::identity();
+  for (int a = 0; a < 3; ++a) this2inertial.b[a] = origin[a];
+  this2inertial.setBaseIJ(base_i, base_j);
+  i2t = this2inertial.inverse();
+  t2i = i2t.inverse();
+}
+
+// ---------------------------------------------------------------------------
 // CartesianPatch
 // ---------------------------------------------------------------------------
 
 template <typename G>
 CartesianPatch<G>::CartesianPatch(int ni, int nj, int nk, const int* seek, const double* origin,
-                                  const double* length) {
+                                  const double* length, const double* base_i, const double* base_j) {
   n_[0] = ni;
   n_[1] = nj;
   n_[2] = nk;
@@ -98,7 +208,19 @@ CartesianPatch<G>::CartesianPatch(int ni, int nj, int nk, const int* seek, const
     // buildRegions (cartesianpatch.cpp:314-390) without additional protection layers
     dz_first_[a] = (size_t)seek_[2 * a];
     dz_after_[a] = (size_t)n_[a] - (size_t)seek_[2 * a + 1];
+    bi_[a] = base_i ? G(base_i[a]) : G(a == 0 ? 1 : 0);
+    bj_[a] = base_j ? G(base_j[a]) : G(a == 1 ? 1 : 0);
   }
+  setupTransformation(o_, bi_, bj_, i2t_, t2i_);
+}
+
+template <typename G>
+void CartesianPatch<G>::setBase(const double* base_i, const double* base_j) {
+  for (int a = 0; a < 3; ++a) {
+    bi_[a] = G(base_i[a]);
+    bj_[a] = G(base_j[a]);
+  }
+  setupTransformation(o_, bi_, bj_, i2t_, t2i_);
 }
 
 template <typename G>
@@ -270,19 +392,30 @@ int PatchGrid<G>::readGrid(const std::string& path, double scale, std::string* e
       if (err) *err = "malformed patch entry in " + path;
       return -1;
     }
-    if (!(bi[0] > 0 && bi[1] == 0 && bi[2] == 0 && bj[0] == 0 && bj[1] > 0 && bj[2] == 0)) {
-      if (err) *err = "only axis-aligned patches (base 1 0 0 / 0 1 0) are supported";
-      return -1;
+    {
+      // setMatrixFromBaseIJ divides by |base_i|, |base_j| and |base_i x base_j|
+      const double bid[3] = {(double)bi[0], (double)bi[1], (double)bi[2]};
+      const double bjd[3] = {(double)bj[0], (double)bj[1], (double)bj[2]};
+      const double cx = bid[1] * bjd[2] - bid[2] * bjd[1], cy = bid[2] * bjd[0] - bid[0] * bjd[2],
+                   cz = bid[0] * bjd[1] - bid[1] * bjd[0];
+      const double ni2 = bid[0] * bid[0] + bid[1] * bid[1] + bid[2] * bid[2];
+      const double nj2 = bjd[0] * bjd[0] + bjd[1] * bjd[1] + bjd[2] * bjd[2];
+      if (!(ni2 > 0) || !(nj2 > 0) || !(cx * cx + cy * cy + cz * cz > 1e-12 * ni2 * nj2)) {
+        if (err) *err = "patch base vectors are zero or parallel in " + path;
+        return -1;
+      }
     }
     ioscale *= G(scale);
-    double od[3], Ld[3];
+    double od[3], Ld[3], bid[3], bjd[3];
     for (int a = 0; a < 3; ++a) {
       o[a] *= ioscale;
       L[a] *= ioscale;
       od[a] = (double)o[a];
       Ld[a] = (double)L[a];
+      bid[a] = (double)bi[a];
+      bjd[a] = (double)bj[a];
     }
-    CartesianPatch<G> P(n[0], n[1], n[2], s, od, Ld);
+    CartesianPatch<G> P(n[0], n[1], n[2], s, od, Ld, bid, bjd);
     P.comment = comment;
     // the rest of the entry is the patch's code string (CodeString::buildFrom)
     P.solver_codes = CodeString(std::string(std::istreambuf_iterator<char>(is), std::istreambuf_iterator<char>()));
@@ -309,6 +442,10 @@ template <typename G>
 struct Contribution {
   size_t ll, l;
   WeightSet<G> ws;
+};
+
+struct Turn {
+  double a[9];
 };
 
 }  // namespace
@@ -339,6 +476,7 @@ int PatchGrid<G>::computeDependencies(const double* period, int threads) {
     if (recv.empty()) continue;
     std::vector<size_t> hits(recv.size(), 0);
     std::vector<std::pair<int, std::vector<Contribution<G>>>> neigh;
+    std::vector<Turn> turns;  // per entry of neigh
     // Patch::insertNeighbour (patch.cpp:264-317) for every candidate donor in ascending order
     for (int q = 0; q < np && !err; ++q) {
       for (int s = -1; s < (int)shifts.size(); ++s) {
@@ -347,24 +485,37 @@ int PatchGrid<G>::computeDependencies(const double* period, int threads) {
         G oq[3];
         for (int a = 0; a < 3; ++a)
           oq[a] = (s < 0 || shifts[s][a] == 0) ? Q.origin(a) : Q.origin(a) + G(shifts[s][a] * period[a]);
-        // translation receiver -> donor frame (coordtransform.cpp concatenate + transform)
-        G b[3];
-        for (int a = 0; a < 3; ++a) b[a] = P.origin(a) - oq[a];
+        // CoordTransformVV::setTransFromTo(receiver, donor) (coordtransformvv.cpp:65-71,
+        // patch.cpp:285-294): `rel` takes receiver cell centres into the donor's
+        // frame; `turn` (its stored inverse) turns the donor's momentum
+        Transform<G> q_i2t, q_t2i;
+        setupTransformation(oq, Q.baseI(), Q.baseJ(), q_i2t, q_t2i);
+        const Transform<G> rel = P.this2inertial().concatenate(q_i2t);
+        const Transform<G> turn = q_t2i.concatenate(P.inertial2this());
         {
-          double lo[3], hi[3];
+          // the receiver's cell-centre box in the donor frame, widened to the
+          // box of its eight transformed corners
+          double lo[3], hi[3], blo[3] = {1e300, 1e300, 1e300}, bhi[3] = {-1e300, -1e300, -1e300};
           P.cellCentreBox(lo, hi);
-          for (int a = 0; a < 3; ++a) {
-            lo[a] += (double)b[a];
-            hi[a] += (double)b[a];
+          for (int c = 0; c < 8; ++c) {
+            const double q[3] = {(c & 1) ? hi[0] : lo[0], (c & 2) ? hi[1] : lo[1], (c & 4) ? hi[2] : lo[2]};
+            for (int r = 0; r < 3; ++r) {
+              const double v = (double)rel.a[r][0] * q[0] + (double)rel.a[r][1] * q[1] + (double)rel.a[r][2] * q[2] +
+                               (double)rel.b[r];
+              blo[r] = std::min(blo[r], v);
+              bhi[r] = std::max(bhi[r], v);
+            }
           }
-          if (!Q.donorZoneMayContain(lo, hi)) continue;
+          if (!Q.donorZoneMayContain(blo, bhi)) continue;
         }
         std::vector<Contribution<G>> list;
         for (size_t ll = 0; ll < recv.size(); ++ll) {
           G x, y, z;
           P.xyzCell(recv[ll], x, y, z);
+          G xt[3];
+          rel.transform(x, y, z, xt);
           WeightSet<G> ws;
-          int r = Q.interpolCoeffs(x + b[0], y + b[1], z + b[2], ws);
+          int r = Q.interpolCoeffs(xt[0], xt[1], xt[2], ws);
           if (r < 0) {
             err = 1;
             break;
@@ -374,7 +525,13 @@ int PatchGrid<G>::computeDependencies(const double* period, int threads) {
             hits[ll]++;
           }
         }
-        if (!list.empty()) neigh.push_back({q, std::move(list)});
+        if (!list.empty()) {
+          Turn tn;
+          for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) tn.a[3 * r + c] = (double)turn.a[r][c];
+          neigh.push_back({q, std::move(list)});
+          turns.push_back(tn);
+        }
       }
     }
     if (err) continue;
@@ -408,7 +565,9 @@ int PatchGrid<G>::computeDependencies(const double* period, int threads) {
         }
         stride = std::max(stride, (size_t)e.ws.n);
       }
-      T.donors.push_back({nb.first, nb.second.size(), stride, nrc, nwi});
+      DonorTables::Donor D{nb.first, nb.second.size(), stride, nrc, nwi, {}};
+      std::memcpy(D.turn, turns[T.donors.size()].a, sizeof(D.turn));
+      T.donors.push_back(D);
       nrc += nb.second.size();
       nwi += nb.second.size() * stride;
     }
@@ -468,7 +627,10 @@ int PatchGrid<G>::levelSetSpheres(const std::vector<double>& spheres, int n_inne
     for (size_t l = 0; l < N; ++l) {
       G x, y, z;
       P.xyzCell(l, x, y, z);
-      const G xo = x + P.origin(0), yo = y + P.origin(1), zo = z + P.origin(2);
+      // Patch::xyzoCell (patch.cpp:91-98): m_TransformInertial2This.transformReverse
+      G xyzo[3];
+      P.this2inertial().transform(x, y, z, xyzo);
+      const G xo = xyzo[0], yo = xyzo[1], zo = xyzo[2];
       G dist = 0;
       for (size_t s = 0; s < ns; ++s) {
         const G dx = xo - G(spheres[4 * s]), dy = yo - G(spheres[4 * s + 1]), dz = zo - G(spheres[4 * s + 2]);
@@ -623,6 +785,12 @@ int PatchGrid<G>::levelSetSpheres(const std::vector<double>& spheres, int n_inne
 
 template struct WeightSet<float>;
 template struct WeightSet<double>;
+template struct Transform<float>;
+template struct Transform<double>;
+template void setupTransformation<float>(const float*, const float*, const float*, Transform<float>&,
+                                         Transform<float>&);
+template void setupTransformation<double>(const double*, const double*, const double*, Transform<double>&,
+                                          Transform<double>&);
 template class CartesianPatch<float>;
 template class CartesianPatch<double>;
 template class PatchGrid<float>;

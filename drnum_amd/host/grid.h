@@ -6,9 +6,10 @@
 // the device).  G = float reproduces the reference's default build, whose
 // `real` is float (drnum.h:43), and is used to check the builder against it.
 //
-// Scope: axis-aligned patches (base vectors (1,0,0), (0,1,0)); the relative
-// transform between two such patches is a translation, applied exactly as the
-// reference's CoordTransform does it (math/coordtransform.h:105-110).
+// Patches carry their base vectors (patch.cpp:63-85); the relative transform
+// between two patches and the turn of the momentum are composed exactly as the
+// reference's CoordTransform / CoordTransformVV compose them
+// (math/coordtransform.{h,cpp}, math/coordtransformvv.cpp).
 #pragma once
 
 #include <cstddef>
@@ -35,10 +36,29 @@ struct WeightSet {
   void shiftToSum(G sum);            // weightedset.h:605-619
 };
 
+// CoordTransform (math/coordtransform.h:26-94): x -> A x + b
+template <typename G>
+struct Transform {
+  G a[3][3], b[3];
+  static Transform identity();                             // coordtransform.cpp:23-28
+  void setBaseIJ(const G* base_i, const G* base_j);        // setMatrixFromBaseIJ, coordtransform.cpp:37-58
+  Transform inverse() const;                               // coordtransform.cpp:99-114
+  Transform concatenate(const Transform& c) const;         // coordtransform.h:136-165: this(c(x))
+  void transform(G x, G y, G z, G* out) const;             // coordtransform.h:105-111
+};
+
+// Patch::setupTransformation (patch.cpp:265-278): the inertial -> this transform
+// (m_TransformInertial2This) and its stored inverse (CoordTransformVV::computeInverse)
+template <typename G>
+void setupTransformation(const G* origin, const G* base_i, const G* base_j, Transform<G>& i2t, Transform<G>& t2i);
+
 template <typename G>
 class CartesianPatch {
  public:
-  CartesianPatch(int ni, int nj, int nk, const int* seek, const double* origin, const double* length);
+  // base_i / base_j: base vectors of the local i and j axes in the inertial
+  // frame; null means axis aligned (1,0,0) / (0,1,0)
+  CartesianPatch(int ni, int nj, int nk, const int* seek, const double* origin, const double* length,
+                 const double* base_i = nullptr, const double* base_j = nullptr);
 
   int ni() const { return n_[0]; }
   int nj() const { return n_[1]; }
@@ -56,6 +76,15 @@ class CartesianPatch {
   }
   G delta(int a) const { return d_[a]; }
   G origin(int a) const { return o_[a]; }
+  G baseI(int a) const { return bi_[a]; }
+  G baseJ(int a) const { return bj_[a]; }
+  const G* baseI() const { return bi_; }
+  const G* baseJ() const { return bj_; }
+  // m_TransformInertial2This and its stored inverse (this -> inertial, Patch::xyzoCell)
+  const Transform<G>& inertial2this() const { return i2t_; }
+  const Transform<G>& this2inertial() const { return t2i_; }
+  // new base vectors (the orientation a grid file gives, patch.cpp:63-85)
+  void setBase(const double* base_i, const double* base_j);
   G length(int a) const { return L_[a]; }
   int seek(int f) const { return seek_[f]; }
   // cell centre in patch coordinates (cartesianpatch.h:267-273)
@@ -85,6 +114,8 @@ class CartesianPatch {
   size_t njk_;
   int seek_[6];
   G o_[3], L_[3], d_[3], ccmin_[3];
+  G bi_[3], bj_[3];
+  Transform<G> i2t_, t2i_;
   size_t dz_first_[3], dz_after_[3];
 };
 
@@ -93,6 +124,7 @@ struct DonorTables {
   struct Donor {
     int patch;
     size_t nrec, stride, rec_start, wi_start;
+    double turn[9];  // donor_t axx..azz (row-major): the momentum's turn donor -> receiver
   };
   std::vector<size_t> receivers;   // unique receiver cells (m_ReceivingCellIndicesUnique)
   std::vector<Donor> donors;

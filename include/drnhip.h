@@ -144,6 +144,17 @@ int drn_exchange_set(drn_ctx* ctx, int recv_id, int n_donors, const int32_t* don
                      const int64_t* donor_info, int64_t n_unique, const int64_t* rec_unique,
                      int64_t n_rec_concat, const int64_t* rec_concat, int64_t n_wi,
                      const int64_t* donor_index, const double* donor_weight);
+/* The turn of vector variables of each donor of receiver patch recv_id, set
+ * after drn_exchange_set for that patch (which clears it): turn[9d..9d+8] is
+ * donor d's matrix axx, axy, axz, ayx, ..., azz of donor_t (donor_t.h:36-44,
+ * Patch::buildDonorTransferData, patch.cpp:178-186), rounded to the state
+ * precision.  The exchange turns the momentum (variables 1-3, the vector
+ * variable PatchGrid::defineVectorVar(1) declares, drnumJetDemo/main.h:99) of
+ * each donor's weighted sum with it before adding the sum to the receiver
+ * (Patch::accessDonorDataDirect, patch.cpp:617-645).  A matrix with exact 1 on
+ * the diagonal and zeros elsewhere (aligned patches) changes no finite value
+ * and is skipped.  Not calling it leaves every donor of the patch aligned. */
+int drn_exchange_set_turn(drn_ctx* ctx, int recv_id, int n_donors, const double* turn);
 /* Upload all tables set so far into the device gather lists.  Fails with
  * DRN_E_RANGE when a donor cell is also a receiver cell: the reference fills
  * receivers patch after patch, the device all at once, so such tables would

@@ -35,6 +35,16 @@ const char* drnh_last_error(const drnh_grid* g);
  * seek layers, seek layers per face, origin and lengths; returns the patch id. */
 int drnh_grid_add_patch(drnh_grid* g, int ni, int nj, int nk, const int32_t* seek6, const double* origin3,
                         const double* length3);
+/* The base vectors of patch p's local i and j axes in the inertial frame
+ * (Patch::readFromFile, patch.cpp:63-85, setupTransformation :265-278); a
+ * patch added with drnh_grid_add_patch is axis aligned, (1,0,0) / (0,1,0).
+ * Fails when a vector is zero or the two are parallel. */
+int drnh_grid_set_base(drnh_grid* g, int p, const double* base_i3, const double* base_j3);
+/* out6 = base_i, base_j of patch p as stored (geometry precision, widened) */
+int drnh_grid_patch_base(const drnh_grid* g, int p, double* out6);
+/* out12 = patch p's this -> inertial transform as Patch::xyzoCell applies it
+ * (patch.cpp:91-98): the row-major matrix, then the shift */
+int drnh_grid_patch_transform(const drnh_grid* g, int p, double* out12);
 /* PatchGrid::readGrid (patchgrid.cpp:479-561). */
 int drnh_grid_read(drnh_grid* g, const char* path, double scale);
 int drnh_grid_num_patches(const drnh_grid* g);
@@ -80,6 +90,10 @@ int64_t drnh_grid_num_concat(const drnh_grid* g, int p, int which);
 int drnh_grid_rec_concat(const drnh_grid* g, int p, int64_t* out);
 int drnh_grid_donor_index(const drnh_grid* g, int p, int64_t* out);
 int drnh_grid_donor_weight(const drnh_grid* g, int p, double* out);
+/* out9 = donor d's turn of the momentum into receiver p's frame, donor_t
+ * axx..azz row-major (patch.cpp:178-186); handed to the device with
+ * drn_exchange_set_turn by drnh_grid_upload_exchange */
+int drnh_grid_donor_turn(const drnh_grid* g, int p, int d, double* out9);
 
 /* Level set min_s(|x - c_s| - r_s) of spheres (xc, yc, zc, r) x n (spherelevelset.cpp,
  * combilevelsetor.cpp); layers as LevelSetObject(.., n_inner, n_outer, min_inner_rel, -1). */

@@ -81,7 +81,14 @@ typedef struct {
   size_t stride;     /* padded contributions per receiver                */
   size_t rec_start;  /* start in the concatenated receiver list          */
   size_t wi_start;   /* start in the concatenated index/weight lists     */
+  REAL a[3][3];      /* donor -> receiver turn of vector variables (donor_t.h:36-44) */
 } donor_t_;
+
+/* CoordTransform (math/coordtransform.h:26-94): x -> A x + b */
+typedef struct {
+  REAL a[3][3];
+  REAL b[3];
+} ctrans_t;
 
 typedef struct {
   size_t cell;
@@ -101,6 +108,8 @@ typedef struct {
   size_t N, njk;
   int seek[6];
   REAL o[3];
+  REAL base_i[3], base_j[3]; /* as read (patch.cpp:68-76) */
+  ctrans_t i2t, t2i;  /* m_TransformInertial2This: inertial -> this and its stored inverse */
   REAL L[3];
   REAL dx, dy, dz;
   REAL xccmin, yccmin, zccmin;
@@ -733,6 +742,114 @@ static void patch_compute(const GRID* g, opatch* P, REAL factor, REAL* res) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* coordinate transforms                                                      */
+/* ------------------------------------------------------------------------ */
+
+/* CoordTransform::CoordTransform (coordtransform.cpp:23-28) */
+static void ct_identity(ctrans_t* t) {
+  memset(t, 0, sizeof(*t));
+  t->a[0][0] = t->a[1][1] = t->a[2][2] = 1;
+}
+
+/* MathVector::abs / normalise (mathvector_methods.h:122-127, 178-183) */
+static REAL v3_abs(const REAL* v) {
+  REAL l = 0;
+  for (int i = 0; i < 3; ++i) l += v[i] * v[i];
+  return RSQRT(l);
+}
+static void v3_normalise(REAL* v) {
+  REAL l = v3_abs(v);
+  for (int i = 0; i < 3; ++i) v[i] /= l;
+}
+
+/* CoordTransform::setMatrixFromBaseIJ (coordtransform.cpp:37-58): the columns
+ * are the unit base vectors i, j (orthogonalised against i) and k = i x j */
+static void ct_set_base_ij(ctrans_t* t, const REAL* bi0, const REAL* bj0) {
+  REAL bi[3] = {bi0[0], bi0[1], bi0[2]}, bj[3] = {bj0[0], bj0[1], bj0[2]}, bk[3];
+  v3_normalise(bi);
+  v3_normalise(bj);
+  /* mathvector_methods.h:86-93 */
+  bk[0] = bi[1] * bj[2] - bi[2] * bj[1];
+  bk[1] = bi[2] * bj[0] - bi[0] * bj[2];
+  bk[2] = bi[0] * bj[1] - bi[1] * bj[0];
+  REAL bk_len = v3_abs(bk);
+  /* scalar product (mathvector_operators.h:208-216) */
+  REAL scal_ij = 0;
+  for (int i = 0; i < 3; ++i) scal_ij += bi[i] * bj[i];
+  for (int i = 0; i < 3; ++i) bj[i] = bj[i] - scal_ij * bi[i];
+  v3_normalise(bj);
+  /* (1./len) is a double expression converted to the `real` factor of
+   * operator*(const real&, MathVector) (mathvector_operators.h:184-190) */
+  REAL inv_len = (REAL)(1. / bk_len);
+  for (int i = 0; i < 3; ++i) bk[i] = inv_len * bk[i];
+  for (int i = 0; i < 3; ++i) {
+    t->a[i][0] = bi[i];
+    t->a[i][1] = bj[i];
+    t->a[i][2] = bk[i];
+  }
+}
+
+/* CoordTransform::inverse (coordtransform.cpp:99-114) with the closed-form 3x3
+ * inverse of InvSmallSquareMatrix<T,3> (smallsquarematrix.h:470-489, the
+ * unsafe branch: a copied matrix has prec_safe = false) and M*v
+ * (smallsquarematrix.h:209-218) for the shift */
+static ctrans_t ct_inverse(const ctrans_t* c) {
+  const REAL(*M)[3] = c->a;
+  REAL t4 = M[0][0] * M[1][1];
+  REAL t6 = M[0][0] * M[1][2];
+  REAL t8 = M[0][1] * M[1][0];
+  REAL t10 = M[0][2] * M[1][0];
+  REAL t12 = M[0][1] * M[2][0];
+  REAL t14 = M[0][2] * M[2][0];
+  REAL t17 = 1 / (t4 * M[2][2] - t6 * M[2][1] - t8 * M[2][2] + t10 * M[2][1] + t12 * M[1][2] - t14 * M[1][1]);
+  ctrans_t t;
+  t.a[0][0] = (M[1][1] * M[2][2] - M[1][2] * M[2][1]) * t17;
+  t.a[0][1] = -(M[0][1] * M[2][2] - M[0][2] * M[2][1]) * t17;
+  t.a[0][2] = -(-M[0][1] * M[1][2] + M[0][2] * M[1][1]) * t17;
+  t.a[1][0] = -(M[1][0] * M[2][2] - M[1][2] * M[2][0]) * t17;
+  t.a[1][1] = (M[0][0] * M[2][2] - t14) * t17;
+  t.a[1][2] = -(t6 - t10) * t17;
+  t.a[2][0] = -(-M[1][0] * M[2][1] + M[1][1] * M[2][0]) * t17;
+  t.a[2][1] = -(M[0][0] * M[2][1] - t12) * t17;
+  t.a[2][2] = (t4 - t8) * t17;
+  const REAL v[3] = {-c->b[0], -c->b[1], -c->b[2]};
+  for (int i = 0; i < 3; ++i) {
+    t.b[i] = 0;
+    for (int j = 0; j < 3; ++j) t.b[i] += t.a[i][j] * v[j];
+  }
+  return t;
+}
+
+/* CoordTransform::concatenate (coordtransform.h:136-165): ret = t * c, i.e.
+ * ret(x) = t(c(x)) */
+static ctrans_t ct_concatenate(const ctrans_t* t, const ctrans_t* c) {
+  ctrans_t r;
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) r.a[i][j] = t->a[i][0] * c->a[0][j] + t->a[i][1] * c->a[1][j] + t->a[i][2] * c->a[2][j];
+    r.b[i] = t->a[i][0] * c->b[0] + t->a[i][1] * c->b[1] + t->a[i][2] * c->b[2] + t->b[i];
+  }
+  return r;
+}
+
+/* CoordTransform::transform (coordtransform.h:105-111) */
+static void ct_transform(const ctrans_t* t, REAL x, REAL y, REAL z, REAL* out) {
+  for (int i = 0; i < 3; ++i) out[i] = t->a[i][0] * x + t->a[i][1] * y + t->a[i][2] * z + t->b[i];
+}
+
+/* Patch::setupTransformation (patch.cpp:265-278): this2inertial = (base matrix,
+ * origin); m_TransformInertial2This = its inverse, whose stored inverse
+ * (CoordTransformVV::computeInverse, coordtransformvv.cpp:105-108) is
+ * inverted once more */
+static void setup_transformation(const REAL* o, const REAL* bi, const REAL* bj, ctrans_t* i2t, ctrans_t* t2i) {
+  ctrans_t this2inertial;
+  ct_identity(&this2inertial);
+  for (int a = 0; a < 3; ++a) this2inertial.b[a] = o[a];
+  ct_set_base_ij(&this2inertial, bi, bj);
+  *i2t = ct_inverse(&this2inertial);
+  *t2i = ct_inverse(i2t);
+}
+
+/* ------------------------------------------------------------------------ */
 /* grid set-up                                                                */
 /* ------------------------------------------------------------------------ */
 
@@ -748,7 +865,10 @@ static void patch_setup(opatch* P, const o_patch_desc* d) {
   for (int a = 0; a < 3; ++a) {
     P->o[a] = (REAL)d->origin[a];
     P->L[a] = (REAL)d->length[a];
+    P->base_i[a] = (REAL)d->base_i[a];
+    P->base_j[a] = (REAL)d->base_j[a];
   }
+  setup_transformation(P->o, P->base_i, P->base_j, &P->i2t, &P->t2i);
   P->dx = P->L[0] / (REAL)(size_t)P->ni;
   P->dy = P->L[1] / (REAL)(size_t)P->nj;
   P->dz = P->L[2] / (REAL)(size_t)P->nk;
@@ -1009,23 +1129,37 @@ typedef struct {
   int patch;
   size_t n, cap;
   wsentry_t* v;
+  REAL a[3][3]; /* turn of vector variables, donor -> receiver */
 } neigh_t;
 
-/* patch.cpp:264-317 insertNeighbour + cartesianpatch.cpp:494-526 computeDependencies,
- * for axis-aligned patches: the relative transform is a translation by
- * fl(o_receiver - o_donor), applied as fl(x + b) (coordtransform.h, :105-110). */
+/* patch.cpp:264-317 insertNeighbour + cartesianpatch.cpp:494-526 computeDependencies.
+ * The relative transform is CoordTransformVV::setTransFromTo(receiver, donor)
+ * (coordtransformvv.cpp:65-71): m_transform = (receiver this->inertial)
+ * .concatenate(donor inertial->this), applied to the receiver's cell centres,
+ * and m_transform_inverse = (donor this->inertial).concatenate(receiver
+ * inertial->this), whose matrix turns the donor's vector variables
+ * (InterCoeffWS::setCoordTransform(extractReverse()), patch.cpp:293-294, 178-186).
+ * concatenate(c) applies c first (coordtransform.h:136-165), so the
+ * reference's composition equals the geometric one (donor inertial->this after
+ * receiver this->inertial) when the two commute, e.g. when one of the two
+ * patches is unrotated with its origin at the inertial origin, or when both
+ * are axis aligned (a translation by o_receiver - o_donor). */
 static int collect_neighbour(const opatch* P, const opatch* Q, const REAL* oq, size_t* hits, neigh_t* nb) {
-  REAL b[3];
-  for (int a = 0; a < 3; ++a) b[a] = P->o[a] - oq[a];
+  ctrans_t q_i2t, q_t2i;
+  setup_transformation(oq, Q->base_i, Q->base_j, &q_i2t, &q_t2i);
+  const ctrans_t rel = ct_concatenate(&P->t2i, &q_i2t);
+  const ctrans_t turn = ct_concatenate(&q_t2i, &P->i2t);
+  memcpy(nb->a, turn.a, sizeof(nb->a));
   for (size_t ll = 0; ll < P->n_recv; ++ll) {
     size_t l = P->recv[ll];
     size_t i = l / P->njk, rest = l - i * P->njk, j = rest / (size_t)P->nk, k = rest - j * (size_t)P->nk;
     REAL x = P->xccmin + (REAL)i * P->dx;
     REAL y = P->yccmin + (REAL)j * P->dy;
     REAL z = P->zccmin + (REAL)k * P->dz;
-    REAL xx = x + b[0], yy = y + b[1], zz = z + b[2];
+    REAL xt[3];
+    ct_transform(&rel, x, y, z, xt);
     wset_t ws;
-    int r = cc_interp(Q, xx, yy, zz, &ws);
+    int r = cc_interp(Q, xt[0], xt[1], xt[2], &ws);
     if (r < 0) return -1;
     if (r == 1) {
       if (nb->n == nb->cap) {
@@ -1158,6 +1292,7 @@ int F(grid_compute_dependencies)(GRID* g, const double* periodic) {
         P->donors[d].stride = stride;
         P->donors[d].rec_start = nrc;
         P->donors[d].wi_start = nwi;
+        memcpy(P->donors[d].a, nbs[d].a, sizeof(P->donors[d].a));
         nrc += nbs[d].n;
         nwi += nbs[d].n * stride;
       }
@@ -1198,6 +1333,19 @@ int F(grid_compute_dependencies)(GRID* g, const double* periodic) {
 }
 
 int F(grid_num_receivers)(const GRID* g, int p) { return (int)g->p[p].n_recv; }
+
+void F(grid_donor_matrix)(const GRID* g, int p, int d, double* a9) {
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) a9[3 * r + c] = (double)g->p[p].donors[d].a[r][c];
+}
+
+void F(grid_patch_transform)(const GRID* g, int p, double* a12) {
+  const ctrans_t* t = &g->p[p].t2i;
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) a12[3 * r + c] = (double)t->a[r][c];
+    a12[9 + r] = (double)t->b[r];
+  }
+}
 int F(grid_num_no_donor)(const GRID* g, int p) { return (int)g->p[p].n_no_donor; }
 const int64_t* F(grid_receivers)(const GRID* g, int p) { return g->p[p].recv64; }
 int F(grid_num_donors)(const GRID* g, int p) { return g->p[p].n_donors; }
@@ -1240,6 +1388,16 @@ void F(grid_exchange)(GRID* g, int field) {
           REAL w = P->donor_w[start + s];
           for (int v = 0; v < 5; ++v) inter[v] += fd[v * Q->N + dc] * w;
         }
+        /* the vector variable of the compressible state, momentum at index 1
+         * (PatchGrid::defineVectorVar(1), drnumJetDemo/main.h:99), turned into
+         * the receiver's frame (patch.cpp:627-641) */
+        const REAL(*A)[3] = D->a;
+        REAL u = A[0][0] * inter[1] + A[0][1] * inter[2] + A[0][2] * inter[3];
+        REAL vv = A[1][0] * inter[1] + A[1][1] * inter[2] + A[1][2] * inter[3];
+        REAL ww = A[2][0] * inter[1] + A[2][1] * inter[2] + A[2][2] * inter[3];
+        inter[1] = u;
+        inter[2] = vv;
+        inter[3] = ww;
         for (int v = 0; v < 5; ++v) f[v * P->N + rec] += inter[v];
       }
     }
@@ -1309,10 +1467,11 @@ int F(grid_levelset)(GRID* g, int n_spheres, const double* spheres, int n_inner,
     REAL* var = P->G;
     for (size_t l = 0; l < P->N; ++l) {
       size_t i = l / P->njk, rest = l - i * P->njk, j = rest / (size_t)P->nk, k = rest - j * (size_t)P->nk;
-      /* xyzoCell: local cell centre, then transformReverse = + origin */
-      REAL xc = P->xccmin + (REAL)i * P->dx + P->o[0];
-      REAL yc = P->yccmin + (REAL)j * P->dy + P->o[1];
-      REAL zc = P->zccmin + (REAL)k * P->dz + P->o[2];
+      /* Patch::xyzoCell (patch.cpp:91-98): m_TransformInertial2This.transformReverse */
+      REAL xo[3];
+      ct_transform(&P->t2i, P->xccmin + (REAL)i * P->dx, P->yccmin + (REAL)j * P->dy, P->zccmin + (REAL)k * P->dz,
+                   xo);
+      REAL xc = xo[0], yc = xo[1], zc = xo[2];
       REAL dist = 0;
       for (int s = 0; s < n_spheres; ++s) {
         REAL ddx = xc - (REAL)spheres[4 * s + 0];

@@ -51,6 +51,8 @@ typedef struct o_patch_desc {
   double  length[3];            /* Lx, Ly, Lz                                     */
   int32_t bc[6];                /* face functor: xm, xp, ym, yp, zm, zp           */
   double  bc_par[6][O_BC_NPAR];
+  double  base_i[3];            /* base vectors of the patch's i and j axes in the */
+  double  base_j[3];            /* inertial frame (patch.cpp:63-85); (1,0,0), (0,1,0) axis aligned */
 } o_patch_desc;
 
 typedef struct o32_grid o32_grid;
@@ -98,6 +100,12 @@ typedef struct o64_grid o64_grid;
   const REAL* PFX##grid_donor_weight(const GRID* g, int p);                                     \
   /* patch.cpp:573-653 for every patch */                                                       \
   void  PFX##grid_exchange(GRID* g, int field);                                                 \
+  /* donor d of patch p: the reference's donor_t matrix axx..azz (row-major), which turns    \
+   * the donor's momentum into the receiver's frame (patch.cpp:178-186, 627-641) */           \
+  void  PFX##grid_donor_matrix(const GRID* g, int p, int d, double* a9);                        \
+  /* patch p's this->inertial transform as Patch::xyzoCell applies it (patch.cpp:91-98):       \
+   * a12 = matrix (row-major) then shift */                                                     \
+  void  PFX##grid_patch_transform(const GRID* g, int p, double* a12);                           \
   /* level set: levelsetobject.cpp:44-388 with spheres (spherelevelset.cpp, combilevelsetor.cpp) */ \
   int   PFX##grid_levelset(GRID* g, int n_spheres, const double* spheres, int n_inner,          \
                            int n_outer, double min_inner_rel);                                  \

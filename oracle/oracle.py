@@ -26,6 +26,8 @@ class PatchDesc(C.Structure):
         ("length", C.c_double * 3),
         ("bc", C.c_int32 * 6),
         ("bc_par", (C.c_double * NPAR) * 6),
+        ("base_i", C.c_double * 3),
+        ("base_j", C.c_double * 3),
     ]
 
 
@@ -84,6 +86,8 @@ def lib():
                 "grid_rk_step": (None, [C.c_void_p, rt, C.c_int, rp]),
                 "grid_set_threads": (None, [C.c_void_p, C.c_int]),
                 "grid_core_sums": (None, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+                "grid_donor_matrix": (None, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+                "grid_patch_transform": (None, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, pfx + name)
@@ -114,9 +118,13 @@ def descs(patches):
             d.bc[a] = P.bc[a]
             for q in range(NPAR):
                 d.bc_par[a][q] = float(P.bc_par[a][q])
+        bi = getattr(P, "base_i", (1.0, 0.0, 0.0))
+        bj = getattr(P, "base_j", (0.0, 1.0, 0.0))
         for a in range(3):
             d.origin[a] = P.origin[a]
             d.length[a] = P.length[a]
+            d.base_i[a] = float(bi[a])
+            d.base_j[a] = float(bj[a])
     return arr
 
 
@@ -254,8 +262,22 @@ class OracleGrid:
         rc = arr(f("grid_rec_concat")(self.h, p), nrc, np.int64)
         di = arr(f("grid_donor_index")(self.h, p), nwi, np.int64)
         dw = arr(f("grid_donor_weight")(self.h, p), nwi, self.dtype)
+        turn = np.array([self.donor_matrix(p, d).ravel() for d in range(nd)]).reshape(nd, 9)
         return dict(receivers=recv, donors=donors, rec_concat=rc, donor_index=di, donor_weight=dw,
-                    no_donor=int(f("grid_num_no_donor")(self.h, p)))
+                    no_donor=int(f("grid_num_no_donor")(self.h, p)), turn=turn)
+
+    def donor_matrix(self, p, d):
+        """the turn of vector variables of donor d of receiver patch p (3x3)"""
+        out = (C.c_double * 9)()
+        self._f("grid_donor_matrix")(self.h, p, d, out)
+        return np.array(out[:]).reshape(3, 3)
+
+    def patch_transform(self, p):
+        """patch p's this->inertial transform as Patch::xyzoCell applies it: (A, b)"""
+        out = (C.c_double * 12)()
+        self._f("grid_patch_transform")(self.h, p, out)
+        a = np.array(out[:])
+        return a[:9].reshape(3, 3), a[9:]
 
     # ---- level set ----------------------------------------------------------
     def levelset(self, spheres, n_inner=3, n_outer=0, min_inner_rel=0.1):

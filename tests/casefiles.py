@@ -12,7 +12,9 @@ def grid_file_text(w):
     for n, P in enumerate(w.patches):
         txt = "1001 // index=%d name='p%d'\n{\n" % (n, n)
         txt += "  %r %r %r\n" % tuple(P.origin)
-        txt += "  1 0 0\n  0 1 0\n  1\n"
+        txt += "  %r %r %r\n" % tuple(float(b) for b in P.base_i)
+        txt += "  %r %r %r\n" % tuple(float(b) for b in P.base_j)
+        txt += "  1\n"
         txt += "  %d %d %d\n" % (P.ni, P.nj, P.nk)
         txt += "  " + " ".join(str(s) for s in P.seek) + "\n"
         txt += "  %r %r %r\n" % tuple(P.length)

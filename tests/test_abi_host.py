@@ -85,10 +85,19 @@ def test_grid_reader_errors_are_reported(tmp_path):
     g = host.HostGrid(True)
     with pytest.raises(RuntimeError, match="cannot open"):
         g.read(str(tmp_path / "nope"))
-    bad = tmp_path / "rot"
-    bad.write_text("1001 { 0 0 0  0 1 0  1 0 0  1  4 4 4  0 0 0 0 0 0  1 1 1 }\n0\n")
-    with pytest.raises(RuntimeError, match="axis-aligned"):
-        g.read(str(bad))
+    # a turned patch is read with its base vectors; parallel or zero ones are refused
+    rot = tmp_path / "rot"
+    rot.write_text("1001 { 0 0 0  0 1 0  -1 0 0  1  4 4 4  0 0 0 0 0 0  1 1 1 }\n0\n")
+    g.read(str(rot))
+    bi, bj = g.patch_base(0)
+    assert bi.tolist() == [0.0, 1.0, 0.0] and bj.tolist() == [-1.0, 0.0, 0.0]
+    A, b = g.patch_transform(0)
+    assert np.array_equal(A, [[0, -1, 0], [1, 0, 0], [0, 0, 1]]) and np.array_equal(b, [0, 0, 0])
+    for base in ("2 0 0  4 0 0", "0 0 0  0 1 0"):
+        bad = tmp_path / "degenerate"
+        bad.write_text("1001 { 0 0 0  %s  1  4 4 4  0 0 0 0 0 0  1 1 1 }\n0\n" % base)
+        with pytest.raises(RuntimeError, match="zero or parallel"):
+            host.HostGrid(True).read(str(bad))
     bad2 = tmp_path / "type"
     bad2.write_text("1002 { }\n0\n")
     with pytest.raises(RuntimeError, match="unsupported patch type"):
