@@ -531,6 +531,10 @@ class HIP_CartesianIterator : public PatchIterator<real> {
                              ru.data(), (int64_t)rc.size(), rc.data(), (int64_t)di.size(), di.data(),
                              t.donor_weight.data()),
             "drn_exchange_set");
+      // the momentum's turn of every donor (donor_t axx..azz, patch.cpp:178-186, 627-641)
+      std::vector<double> turn;
+      for (const auto& D : t.donors) turn.insert(turn.end(), D.turn, D.turn + 9);
+      check(drn_exchange_set_turn(ctx_, (int)p, (int)t.donors.size(), turn.data()), "drn_exchange_set_turn");
       any = true;
     }
     if (any) check(drn_exchange_commit(ctx_), "drn_exchange_commit");

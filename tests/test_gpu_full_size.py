@@ -57,7 +57,14 @@ def test_sod_64_fp64_100_steps(gpu):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_uneven_200_free_stream(gpu, dtype):
+    """Transmissive faces: the far-field functor does not keep a free stream
+    that equals its far state, since its energy flux takes DrNUM's cp = 1004.5
+    where the interior flux has cv + R = 1004.7 (compressiblefarfieldflux.h:101-105,
+    perfectgas.h:31-34; pinned in test_oracle_walls.py).  What is checked here is
+    the interfaces: the trilinear donor stencils of the ratio-2 nesting."""
     w = configs.uneven_batch(pulse=0.0)
+    for P in w.patches:
+        P.bc = [configs.BC_TRANSMISSIVE] * 6
     assert len(w.patches) == 200
     hg = host.HostGrid.from_workload(w, fp64_geometry=True)
     hg.compute_dependencies(w.periodic)

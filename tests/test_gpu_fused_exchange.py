@@ -28,8 +28,9 @@ CASES = {
     "kh_thin": (lambda: configs.kelvin_helmholtz(core=2, blocks=2), 8 * 2 * 2 * 2 * 2),
     # 12 patches of 40^3 core (3 x 2 x 2): the z sides between the two layers
     "spheres": (lambda: configs.levelset_spheres(core=40), 38400),
-    # overlapping and nested patches of 12-24 cells: no z side is an aligned unit-weight copy
-    "uneven": (lambda: configs.uneven_batch(n_patches=30, size_range=(12, 24)), 0),
+    # overlapping and nested patches of 12-24 cells (sizes drawn per patch): the z sides
+    # that are aligned unit-weight copies fuse, the others stay with the exchange
+    "uneven": (lambda: configs.uneven_batch(n_patches=30, size_range=(12, 24)), None),
 }
 # (tiling, launch form) of each fusing kernel: cell, z-shared, persistent,
 # z-shared on the push-chain list, per-patch z-shared, the i-marching z-shared
