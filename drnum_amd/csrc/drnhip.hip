@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -55,6 +56,7 @@ struct drn_ctx {
   int n_cu = 0;  // compute units of the device (persistent launch size)
   int prec = DRN_F64;
   hipStream_t stream = nullptr;
+  double* scratch = nullptr;  // device address of the file-scope `scratch` array on this device
   std::string err;
   int recon = DRN_RECON_UPWIND2, lim = DRN_LIM_VANALBADA;
   bool visc = false;  // CompressibleViscFlux added to the interior face fluxes
@@ -743,6 +745,33 @@ static int do_rk_step(drn_ctx* ctx, double dt, int n_alpha, const double* alpha,
 
 // The functions below were declared extern "C" by drnhip.h and keep C linkage.
 
+// Partial sums of drn_core_sums_noalloc: 5 doubles per plane chunk of the
+// stage work list, at most kScratchChunks chunks (40 MiB on every device the
+// library runs on, held until the process ends).  In host code `scratch` names
+// a host-side stand-in: kernels and copies take the address that
+// hipGetSymbolAddress gives for the current device, kept per device below.
+constexpr size_t kScratchChunks = (size_t)1 << 20;
+static __device__ __align__(256) double scratch[5 * kScratchChunks];
+static std::mutex g_scratch_mu;
+static std::map<int, double*> g_scratch_dev;
+
+// the device address of `scratch` on ctx->device (current), looked up once per device
+static int scratch_address(drn_ctx* ctx) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_dev.find(ctx->device);
+  if (it == g_scratch_dev.end()) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return fail(ctx, DRN_E_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    }
+    it = g_scratch_dev.emplace(ctx->device, (double*)p).first;
+  }
+  ctx->scratch = it->second;
+  return DRN_OK;
+}
+
 int drn_ctx_create(int device, int precision, drn_ctx** out) {
   if (!out) return DRN_E_ARG;
   *out = nullptr;
@@ -758,6 +787,11 @@ int drn_ctx_create(int device, int precision, drn_ctx** out) {
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return DRN_E_HIP;
+  }
+  if (int rc = scratch_address(ctx)) {
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return rc;
   }
   *out = ctx;
   return DRN_OK;
@@ -1649,6 +1683,38 @@ int drn_core_sums(drn_ctx* ctx, int field, double* out) {
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const void* U = ctx->buf[ctx->map[field]];
   return ctx->prec == DRN_F32 ? sums_t(ctx, (const float*)U, out) : sums_t(ctx, (const double*)U, out);
+}
+
+// drn_core_sums with the partials in `scratch` (through its device address
+// ctx->scratch) and the result left in device memory: the same kernels, so the
+// same sums bit for bit.  Every partial row read by k_sums_final is written by
+// k_sums_partial of the same call, so nothing needs clearing.
+template <typename T>
+static void sums_noalloc_t(drn_ctx* ctx, const T* U, double* d_out) {
+  const int nt = (int)ctx->tiles.size();
+  hipLaunchKernelGGL(k_sums_partial<T>, dim3(nt), dim3(256), 0, ctx->stream, ctx->d_pd, ctx->d_tiles, U, ctx->vstride,
+                     ctx->scratch);
+  hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(256), 0, ctx->stream, (uint32_t)nt, ctx->scratch, d_out);
+}
+
+int drn_core_sums_noalloc(drn_ctx* ctx, int field, double* d_out) {
+  if (!ctx || !d_out) return DRN_E_ARG;
+  int rc;
+  if ((rc = check_field(ctx, field))) return rc;
+  if (!ctx->scratch) return fail(ctx, DRN_E_STATE, "drn_core_sums_noalloc: no scratch address for this device");
+  if (ctx->tiles.empty()) return fail(ctx, DRN_E_STATE, "drn_core_sums_noalloc: no cells");
+  if (ctx->tiles.size() > kScratchChunks)
+    return fail(ctx, DRN_E_RANGE,
+                "drn_core_sums_noalloc: " + std::to_string(ctx->tiles.size()) + " plane chunks exceed the scratch of " +
+                    std::to_string(kScratchChunks) + " (use drn_core_sums)");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const void* U = ctx->buf[ctx->map[field]];
+  if (ctx->prec == DRN_F32)
+    sums_noalloc_t(ctx, (const float*)U, d_out);
+  else
+    sums_noalloc_t(ctx, (const double*)U, d_out);
+  HIPCHK(ctx, hipGetLastError());
+  return DRN_OK;
 }
 
 // out[0..4] as k_diag_partial's partials, reduced over all tiles

@@ -64,6 +64,7 @@ _SIGS = {
     "drn_rk_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_double), C.c_int]),
     "drn_rk_graph_run": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_double), C.c_int]),
     "drn_core_sums": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "drn_core_sums_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "drn_diagnostics": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double)]),
     "drn_cfl_axes": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double)]),
     "drn_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
@@ -251,6 +252,11 @@ class Context:
         out = (C.c_double * 5)()
         self._chk(self.L.drn_core_sums(self.h, field, out), "drn_core_sums")
         return np.array(out[:])
+
+    def core_sums_noalloc(self, d_out, field=0):
+        """drn_core_sums_noalloc: the sums into 5 doubles of device memory at d_out
+        (an address), asynchronous on the context stream"""
+        self._chk(self.L.drn_core_sums_noalloc(self.h, field, C.c_void_p(d_out)), "drn_core_sums_noalloc")
 
     def diagnostics(self, field=0, dt=0.0):
         out = (C.c_double * 4)()

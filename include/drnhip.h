@@ -218,6 +218,18 @@ int drn_rk_graph_run(drn_ctx* ctx, double dt, int n_alpha, const double* alpha, 
 /* ---- reductions (FP64 accumulators) -------------------------------------- */
 /* out[0..4]: sum over core cells of (rho, rhou, rhov, rhow, rhoE) * dV. Synchronous. */
 int drn_core_sums(drn_ctx* ctx, int field, double* out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): drn_core_sums with the result written to d_out, 5 doubles of
+ * DEVICE memory that the caller owns, asynchronously on the context stream
+ * (drn_sync before reading it).  The per-chunk partial sums go to a fixed
+ * device array of the library (40 MiB, held on every device the library runs
+ * on until the process ends) instead of context memory, so the call allocates
+ * nothing.  The sums equal drn_core_sums bit for bit.  At most 2^20 plane
+ * chunks of 256 cells (about 2.7e8 stored cells): DRN_E_RANGE above.  Every
+ * call on a device shares that array: calls that overlap on one device (two
+ * contexts on two host threads, or two streams) race. */
+int drn_core_sums_noalloc(drn_ctx* ctx, int field, double* d_out);
 /* The diagnostics loop of the jet demo (applications/drnumJetDemo/main.h:188-210)
  * over all cells: out[0] = CFL_max = max over cells of max(|u|, |u+a|, |u-a|)
  * dt/dx with the x velocity and spacing only, evaluated in the state precision
