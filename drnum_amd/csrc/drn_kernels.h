@@ -185,9 +185,17 @@ __device__ __forceinline__ void visc_add(const char* const* sv, const ViscGeom<T
   visc_face<T, FAST, DIR>(vl, vr, gla, gra, glb, grb, G.dn[DIR], A, flux);
 }
 
-template <typename T, int RECON, int LIM, bool FAST, int DIR, bool VISC = false>
+// Split faces (Patch::isSplitFace, patch_common.h:500-503; Upwind2 falls back
+// to first order there, upwind2.h:41): `ins` holds the inside bits of the cells
+// idx-2 .. idx+2 of the axis in bits 0..4 (a bit of a cell outside the patch is
+// never read).  A side's slope enters a face only when its outer cell i0 and
+// its cell i1 are on the same side of the inside boundary.
+__device__ __forceinline__ bool split_bits(uint32_t ins, int a, int b) { return ((ins >> a) ^ (ins >> b)) & 1u; }
+
+template <typename T, int RECON, int LIM, bool FAST, int DIR, bool VISC = false, bool FLG = false>
 __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, uint32_t st, int idx, int n,
-                                             const T* u0, T A, T* res, T* fp, const ViscGeom<T>* G = nullptr) {
+                                             const T* u0, T A, T* res, T* fp, const ViscGeom<T>* G = nullptr,
+                                             uint32_t ins = 0) {
   T um2[5], um1[5], up1[5], up2[5];
   const bool have = n > 2;
 #pragma unroll
@@ -219,7 +227,15 @@ __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, 
     // three slopes of the cells c-1, c, c+1 serve both faces of the axis.
     if (!have) return;
     T sm[5], s0[5], sp[5];
-    const bool okm = idx >= 2, ok0 = idx >= 1 && idx + 1 < n, okp = idx + 2 < n;
+    bool okm = idx >= 2, ok0 = idx >= 1 && idx + 1 < n, okp = idx + 2 < n;
+    // with split faces the cell's own slope may enter one of its faces only
+    bool ok0m = true, ok0p = true;
+    if constexpr (FLG) {
+      okm = okm && !split_bits(ins, 0, 1);
+      okp = okp && !split_bits(ins, 4, 3);
+      ok0m = !split_bits(ins, 3, 2);
+      ok0p = !split_bits(ins, 1, 2);
+    }
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
       sm[v] = okm ? limited<T, LIM, true>(um1[v] - um2[v], u0[v] - um1[v]) : T(0);
@@ -231,7 +247,7 @@ __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, 
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
         vl[v] = um1[v] + T(0.5) * sm[v];
-        vr[v] = u0[v] - T(0.5) * s0[v];
+        vr[v] = u0[v] - T(0.5) * (ok0m ? s0[v] : T(0));
       }
       zero5(f);
       ausm_flux_set<T, FAST, DIR>(vl, vr, A, f);
@@ -243,7 +259,7 @@ __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, 
       T vl[5], vr[5];
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        vl[v] = u0[v] + T(0.5) * s0[v];
+        vl[v] = u0[v] + T(0.5) * (ok0p ? s0[v] : T(0));
         vr[v] = up1[v] - T(0.5) * sp[v];
       }
       ausm_flux_set<T, FAST, DIR>(vl, vr, A, fp);  // fp was zeroed above
@@ -253,13 +269,15 @@ __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, 
     if (have && idx > 0) {
       T f[5];
       zero5(f);
-      face_flux_r<T, RECON, LIM, FAST, DIR>(um2, um1, u0, up1, idx >= 2, idx + 1 < n, A, f);
+      face_flux_r<T, RECON, LIM, FAST, DIR>(um2, um1, u0, up1, idx >= 2 && !(FLG && split_bits(ins, 0, 1)),
+                                            idx + 1 < n && !(FLG && split_bits(ins, 3, 2)), A, f);
       if constexpr (VISC) visc_add<T, FAST, DIR>(sv, *G, c - st, c, um1, u0, A, f);
 #pragma unroll
       for (int v = 0; v < 5; ++v) res[v] += f[v];
     }
     if (have && idx < n - 1) {
-      face_flux_r<T, RECON, LIM, FAST, DIR>(um1, u0, up1, up2, idx >= 1, idx + 2 < n, A, fp);
+      face_flux_r<T, RECON, LIM, FAST, DIR>(um1, u0, up1, up2, idx >= 1 && !(FLG && split_bits(ins, 1, 2)),
+                                            idx + 2 < n && !(FLG && split_bits(ins, 4, 3)), A, fp);
       if constexpr (VISC) visc_add<T, FAST, DIR>(sv, *G, c, c + st, u0, up1, A, fp);
     }
   }
@@ -273,10 +291,12 @@ __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, 
 //             evaluated once: 5 face evaluations per cell instead of 6.  The z
 //             neighbours themselves are the neighbouring lanes' cells, also
 //             taken by lane shuffles (15 global loads fewer per cell).
-template <typename T, int RECON, int LIM, bool FAST, int BS, bool ZS, bool VISC = false>
+// FLG: per-cell flags (DRN_CELL_INACTIVE = 1, DRN_CELL_INSIDE = 2) at flg[global
+// cell]: split faces reconstruct first order, inactive cells keep their state.
+template <typename T, int RECON, int LIM, bool FAST, int BS, bool ZS, bool VISC = false, bool FLG = false>
 __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds, const TileCell t,
                                                 const T* __restrict__ src, const T* base, T* dst, uint64_t vs,
-                                                T factor, int push) {
+                                                T factor, int push, const uint8_t* __restrict__ flg = nullptr) {
   const PatchDev& P = pds[t.patch];
   const int ni = P.ni, nj = P.nj, nk = P.nk, njk = P.njk;
   const int lane = (int)(threadIdx.x & 63);
@@ -325,9 +345,27 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
     G.dn[1] = T(1.0) / dy;
     G.dn[2] = T(1.0) / dz;
   }
+  // the cell's flags, and the inside bits of the x and y neighbours
+  uint32_t fo = 0, insx = 0, insy = 0;
+  const uint32_t gc = P.off + (uint32_t)i * (uint32_t)njk + (uint32_t)cc2;  // the cell's global index
+  if constexpr (FLG) {
+    fo = flg[gc];
+    auto ins_axis = [&](uint32_t stc, int idx, int n) {
+      uint32_t b = ((fo >> 1) & 1u) << 2;
+      if (idx >= 2) b |= ((uint32_t)(flg[gc - 2 * stc] >> 1) & 1u) << 0;
+      if (idx >= 1) b |= ((uint32_t)(flg[gc - stc] >> 1) & 1u) << 1;
+      if (idx + 1 < n) b |= ((uint32_t)(flg[gc + stc] >> 1) & 1u) << 3;
+      if (idx + 2 < n) b |= ((uint32_t)(flg[gc + 2 * stc] >> 1) & 1u) << 4;
+      return b;
+    };
+    if (own) {
+      insx = ins_axis((uint32_t)njk, i, ni);
+      insy = ins_axis((uint32_t)nk, j, nj);
+    }
+  }
   if (own) {
-    axis_faces_r<T, RECON, LIM, FAST, 0, VISC>(sv, c, sx, i, ni, u0, Ax, res, fxp, &G);
-    axis_faces_r<T, RECON, LIM, FAST, 1, VISC>(sv, c, sy, j, nj, u0, Ay, res, fyp, &G);
+    axis_faces_r<T, RECON, LIM, FAST, 0, VISC, FLG>(sv, c, sx, i, ni, u0, Ax, res, fxp, &G, insx);
+    axis_faces_r<T, RECON, LIM, FAST, 1, VISC, FLG>(sv, c, sy, j, nj, u0, Ay, res, fyp, &G, insy);
   }
   if constexpr (ZS) {
     // z face k-1/2 of every lane's cell (lane 63 included), then the plus face
@@ -357,8 +395,20 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
 #pragma unroll
       for (int v = 0; v < 5; ++v) up1[v] = ldv<T>(sv[v], c + W);
     }
+    bool zm_ok = k >= 2, zp_ok = k + 1 < nk;
+    if constexpr (FLG) {
+      // inside bits of k-2, k-1, k+1 from the neighbouring lanes (edge lanes load them)
+      const uint32_t io = (fo >> 1) & 1u;
+      uint32_t im1 = (uint32_t)__shfl_up((int)io, 1, 64), im2 = (uint32_t)__shfl_up((int)io, 2, 64);
+      uint32_t ip1 = (uint32_t)__shfl_down((int)io, 1, 64);
+      if (lane == 0 && in_plane && k >= 1) im1 = (flg[gc - 1] >> 1) & 1u;
+      if (lane <= 1 && in_plane && k >= 2) im2 = (flg[gc - 2] >> 1) & 1u;
+      if (lane == 63 && in_plane && k + 1 < nk) ip1 = (flg[gc + 1] >> 1) & 1u;
+      zm_ok = zm_ok && im2 == im1;
+      zp_ok = zp_ok && ip1 == io;
+    }
     if (in_plane && nk > 2 && k >= 1) {
-      face_flux_r<T, RECON, LIM, FAST, 2>(um2, um1, u0, up1, k >= 2, k + 1 < nk, Az, fzm);
+      face_flux_r<T, RECON, LIM, FAST, 2>(um2, um1, u0, up1, zm_ok, zp_ok, Az, fzm);
       if constexpr (VISC) visc_add<T, FAST, 2>(sv, G, c - W, c, um1, u0, Az, fzm);
     }
 #pragma unroll
@@ -369,6 +419,7 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
     }
     if (!own) return;
   } else {
+    static_assert(!FLG, "cell flags run the z-shared cell kernel");
     axis_faces_r<T, RECON, LIM, FAST, 2, VISC>(sv, c, W, k, nk, u0, Az, res, fzp, &G);
   }
 #pragma unroll
@@ -377,21 +428,24 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
     res[v] -= fyp[v];
     res[v] -= fxp[v];
   }
-  // patch-face functors (cartesianiterator.h:190-308)
-  const bool wx = ni > 2 && (i == 0 || i == ni - 1);
-  const bool wy = nj > 2 && (j == 0 || j == nj - 1);
-  const bool wz = nk > 2 && (k == 0 || k == nk - 1);
+  // patch-face functors (cartesianiterator.h:190-308); faces marked BC_DEAD
+  // (all boundary cells exchange receivers, in the RK step) are skipped
+  const bool lx0 = P.bc[0] != BC_DEAD, lx1 = P.bc[1] != BC_DEAD, ly0 = P.bc[2] != BC_DEAD,
+             ly1 = P.bc[3] != BC_DEAD, lz0 = P.bc[4] != BC_DEAD, lz1 = P.bc[5] != BC_DEAD;
+  const bool wx = ni > 2 && ((i == 0 && lx0) || (i == ni - 1 && lx1));
+  const bool wy = nj > 2 && ((j == 0 && ly0) || (j == nj - 1 && ly1));
+  const bool wz = nk > 2 && ((k == 0 && lz0) || (k == nk - 1 && lz1));
   if (wx || wy || wz) {
     const T* var = u0;
     const T xc = T(0.5) * dx + T(i) * dx, yc = T(0.5) * dy + T(j) * dy, zc = T(0.5) * dz + T(k) * dz;
     if (ni > 2) {
-      if (i == 0) {
+      if (i == 0 && lx0) {
         zero5(f);
         wall_flux<T, FAST>(P.bc[0], P.par[0], var, 0, false, yc, zc, j, k, Ax, f);
 #pragma unroll
         for (int v = 0; v < 5; ++v) res[v] += f[v];
       }
-      if (i == ni - 1) {
+      if (i == ni - 1 && lx1) {
         zero5(f);
         wall_flux<T, FAST>(P.bc[1], P.par[1], var, 0, true, yc, zc, j, k, Ax, f);
 #pragma unroll
@@ -399,13 +453,13 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
       }
     }
     if (nj > 2) {
-      if (j == 0) {
+      if (j == 0 && ly0) {
         zero5(f);
         wall_flux<T, FAST>(P.bc[2], P.par[2], var, 1, false, xc, zc, i, k, Ay, f);
 #pragma unroll
         for (int v = 0; v < 5; ++v) res[v] += f[v];
       }
-      if (j == nj - 1) {
+      if (j == nj - 1 && ly1) {
         zero5(f);
         wall_flux<T, FAST>(P.bc[3], P.par[3], var, 1, true, xc, zc, i, k, Ay, f);
 #pragma unroll
@@ -413,13 +467,13 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
       }
     }
     if (nk > 2) {
-      if (k == 0) {
+      if (k == 0 && lz0) {
         zero5(f);
         wall_flux<T, FAST>(P.bc[4], P.par[4], var, 2, false, xc, yc, i, j, Az, f);
 #pragma unroll
         for (int v = 0; v < 5; ++v) res[v] += f[v];
       }
-      if (k == nk - 1) {
+      if (k == nk - 1 && lz1) {
         zero5(f);
         wall_flux<T, FAST>(P.bc[5], P.par[5], var, 2, true, xc, yc, i, j, Az, f);
 #pragma unroll
@@ -434,6 +488,13 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
   for (int v = 0; v < 5; ++v) {
     const char* bv = reinterpret_cast<const char*>(base + (uint64_t)v * vs);
     nv[v] = ldv<T>(bv, c) + pf * res[v];
+  }
+  if constexpr (FLG) {
+    // an inactive cell keeps its state (Patch::isActive, cartesianiterator.h:315)
+    if (fo & 1u) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) nv[v] = u0[v];
+    }
   }
   bool store = true;
   if (push) {
@@ -464,6 +525,16 @@ __global__ __launch_bounds__(BS) void k_stage_cell(const PatchDev* __restrict__ 
                                                    const T* __restrict__ src, const T* base, T* dst, uint64_t vs,
                                                    T factor, int push) {
   stage_cell_item<T, RECON, LIM, FAST, BS, ZS, VISC>(pds, tiles[blockIdx.x], src, base, dst, vs, factor, push);
+}
+
+// the z-shared cell kernel with per-cell flags (inactive cells, split faces)
+template <typename T, int RECON, int LIM, bool FAST, bool VISC>
+__global__ __launch_bounds__(256) void k_stage_cell_flags(const PatchDev* __restrict__ pds,
+                                                          const TileCell* __restrict__ tiles, const T* __restrict__ src,
+                                                          const T* base, T* dst, uint64_t vs, T factor, int push,
+                                                          const uint8_t* __restrict__ flg) {
+  stage_cell_item<T, RECON, LIM, FAST, 256, true, VISC, true>(pds, tiles[blockIdx.x], src, base, dst, vs, factor,
+                                                              push, flg);
 }
 
 // The FP32 viscous form of k_stage_cell<..., ZS = true>.  Left to itself the
@@ -511,10 +582,14 @@ __global__ __launch_bounds__(256) void k_stage_cell_persist(const PatchDev* __re
 // for a cell on one patch face the two orders add the same terms in another
 // association only.  Requires NI, NJ, NK >= 3 (thinner patches run
 // k_stage_cell).
-template <typename T, int RECON, int LIM, bool FAST>
+// LC: the values carried from plane to plane besides U(i) -- the x slope (or
+// left state) xs and the x face flux fxm -- live in LDS, `carry[10][256]`
+// (variable-major, each lane its own column: no barrier, no bank conflict),
+// instead of 10 VGPRs; the kernel then fits 64 VGPRs (8 waves per SIMD).
+template <typename T, int RECON, int LIM, bool FAST, bool LC = false>
 __device__ __forceinline__ void stage_xm_item(const PatchDev* __restrict__ pds, const TileMarch* __restrict__ items,
                                               const T* __restrict__ src, const T* base, T* dst, uint64_t vs, T factor,
-                                              int push) {
+                                              int push, T* carry = nullptr) {
   // the fast symmetric form carries the slope of cell i, the general form the
   // left MUSCL state of face i+1/2
   constexpr bool SYM = FAST && RECON == RECON_UPWIND2 && (LIM == LIM_VANALBADA || LIM == LIM_MINMOD);
@@ -543,7 +618,10 @@ __device__ __forceinline__ void stage_xm_item(const PatchDev* __restrict__ pds, 
   };
   const bool pjk0 = push && P.npush > 0 && jk_in(P.push[0]), pjk1 = push && P.npush > 1 && jk_in(P.push[1]);
   const bool rjk0 = push && P.nrecv > 0 && jk_in(P.recv[0]), rjk1 = push && P.nrecv > 1 && jk_in(P.recv[1]);
-  const bool wyz = own && (j == 0 || j == nj - 1 || k == 0 || k == nk - 1);
+  // boundary cells of live faces (BC_DEAD faces are skipped, see stage_cell_item)
+  const bool wyz = own && ((j == 0 && P.bc[2] != BC_DEAD) || (j == nj - 1 && P.bc[3] != BC_DEAD) ||
+                           (k == 0 && P.bc[4] != BC_DEAD) || (k == nk - 1 && P.bc[5] != BC_DEAD));
+  const bool lx0 = P.bc[0] != BC_DEAD, lx1 = P.bc[1] != BC_DEAD;
 
   // x data entering plane i: u0 = U(i), xs = slope of cell i (SYM) or the
   // left state of face i+1/2, fxm = F(i-1/2) (zero at i = 0)
@@ -595,19 +673,27 @@ __device__ __forceinline__ void stage_xm_item(const PatchDev* __restrict__ pds, 
       if (i0 >= 1) face_flux_r<T, RECON, LIM, FAST, 0>(um2, um1, u0, up1, i0 >= 2, i0 + 1 < ni, Ax, fxm);
     }
   }
+  const int tid = (int)threadIdx.x;
+  if constexpr (LC) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      carry[v * 256 + tid] = xs[v];
+      carry[(5 + v) * 256 + tid] = fxm[v];
+    }
+  }
 
   for (int i = i0; i < i1; ++i, c += sx) {
     T res[5];
     zero5(res);
     // patch-face functors: one call site, the faces in the reference's order
-    if (wyz || i == 0 || i == ni - 1) {
+    if (wyz || (i == 0 && lx0) || (i == ni - 1 && lx1)) {
 #pragma unroll 1
       for (int f = 0; f < 6; ++f) {
         const int d = f >> 1;
         const bool plus = (f & 1) != 0;
         const int idx = d == 0 ? i : (d == 1 ? j : k);
         const int n = d == 0 ? ni : (d == 1 ? nj : nk);
-        if (own && idx == (plus ? n - 1 : 0)) {
+        if (P.bc[f] != BC_DEAD && own && idx == (plus ? n - 1 : 0)) {
           const T xc = T(0.5) * dx + T(i) * dx, yc = T(0.5) * dy + T(j) * dy, zc = T(0.5) * dz + T(k) * dz;
           const T ta = d == 0 ? yc : xc, tb = d == 2 ? yc : zc;
           const int la = d == 0 ? j : i, lb = d == 2 ? j : k;
@@ -620,8 +706,13 @@ __device__ __forceinline__ void stage_xm_item(const PatchDev* __restrict__ pds, 
         }
       }
     }
+    if constexpr (LC) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) res[v] += fxm[v];
+      for (int v = 0; v < 5; ++v) res[v] += carry[(5 + v) * 256 + tid];
+    } else {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) res[v] += fxm[v];
+    }
     T fyp[5], fzp[5], f[5];
     if (own) axis_faces_r<T, RECON, LIM, FAST, 1>(sv, c, sy, j, nj, u0, Ay, res, fyp);
     {
@@ -673,6 +764,10 @@ __device__ __forceinline__ void stage_xm_item(const PatchDev* __restrict__ pds, 
       }
       if (own) {
         T vl[5], vr[5], xn[5];
+        if constexpr (LC) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) xs[v] = carry[v * 256 + tid];
+        }
         if constexpr (SYM) {
           const bool okp = i + 2 < ni;
 #pragma unroll
@@ -701,6 +796,10 @@ __device__ __forceinline__ void stage_xm_item(const PatchDev* __restrict__ pds, 
         ausm_flux_set<T, FAST, 0>(vl, vr, Ax, f);
 #pragma unroll
         for (int v = 0; v < 5; ++v) xs[v] = xn[v];
+        if constexpr (LC) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) carry[v * 256 + tid] = xn[v];
+        }
       }
     }
     if (own) {
@@ -737,7 +836,10 @@ __device__ __forceinline__ void stage_xm_item(const PatchDev* __restrict__ pds, 
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      fxm[v] = f[v];
+      if constexpr (LC)
+        carry[(5 + v) * 256 + tid] = f[v];
+      else
+        fxm[v] = f[v];
       u0[v] = up1[v];
     }
   }
@@ -757,6 +859,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     const PatchDev* __restrict__ pds, const TileMarch* __restrict__ items, const T* __restrict__ src, const T* base,
     T* dst, uint64_t vs, T factor, int push) {
   stage_xm_item<T, RECON, LIM, FAST>(pds, items, src, base, dst, vs, factor, push);
+}
+// the same with xs and the x face flux carried in LDS
+template <typename T, int RECON, int LIM, bool FAST>
+__global__ __launch_bounds__(256) void k_stage_xml(const PatchDev* __restrict__ pds,
+                                                   const TileMarch* __restrict__ items, const T* __restrict__ src,
+                                                   const T* base, T* dst, uint64_t vs, T factor, int push) {
+  __shared__ T carry[10 * 256];
+  stage_xm_item<T, RECON, LIM, FAST, true>(pds, items, src, base, dst, vs, factor, push, carry);
 }
 template <typename T, int RECON, int LIM, bool FAST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 7))) void k_stage_xm7(

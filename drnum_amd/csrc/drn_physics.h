@@ -20,6 +20,11 @@ namespace drn {
 enum { RECON_UPWIND1 = 0, RECON_UPWIND2 = 1 };
 enum { LIM_VANALBADA = 0, LIM_MINMOD = 1, LIM_SECONDORDER = 2 };
 enum { BC_SLIP = 0, BC_FARFIELD = 1, BC_TRANSMISSIVE = 2, BC_JET = 3, BC_NONE = 4 };
+// Internal face code of the RK step's patch descriptors: a face whose layer of
+// boundary cells consists of exchange receivers only.  Their update is
+// overwritten by the exchange after the stage, so the stage kernels skip the
+// face functor there (it adds nothing that survives the step).
+constexpr int BC_DEAD = 5;
 
 template <typename T>
 struct Gas {

@@ -60,6 +60,11 @@ struct drn_ctx {
   std::string err;
   int recon = DRN_RECON_UPWIND2, lim = DRN_LIM_VANALBADA;
   bool visc = false;  // CompressibleViscFlux added to the interior face fluxes
+  // per-cell flags (DRN_CELL_*), one byte per stored cell in the state layout;
+  // null until a patch gets flags, flags_on while any cell has one
+  uint8_t* d_flags = nullptr;
+  bool flags_on = false;
+  std::vector<uint8_t> flags_any;  // per patch: whether it has a nonzero flag
 
   std::vector<drn_patch_desc> descs;
   std::vector<uint64_t> offs;
@@ -69,6 +74,12 @@ struct drn_ctx {
   int map[NBUF] = {0, 1, 2};  // logical field -> physical buffer
 
   PatchDev* d_pd = nullptr;
+  // the descriptors of the RK step: faces whose boundary cells are all exchange
+  // receivers carry BC_DEAD (their functor's contribution is overwritten by the
+  // exchange after the stage); null when no face is dead
+  PatchDev* d_pd_rk = nullptr;
+  int n_dead_faces = 0;
+  const PatchDev* pd_stage = nullptr;  // the descriptors of the stage being launched
   std::vector<TileCell> tiles;
   std::vector<int> tile_first;  // per patch, size np+1
   TileCell* d_tiles = nullptr;
@@ -249,7 +260,8 @@ enum {
   K_CELLZP = 8,  // K_CELLZ items, one persistent launch
   K_XM = 9,      // the z-shared cell kernel marching its (j,k) cells over an i-range
   K_XM8 = 10,    // K_XM capped at 8 waves per SIMD (FP32 fast arithmetic; else K_XM)
-  K_XM7 = 11     // K_XM capped at 7 waves per SIMD (FP32 fast arithmetic; else K_XM)
+  K_XM7 = 11,    // K_XM capped at 7 waves per SIMD (FP32 fast arithmetic; else K_XM)
+  K_XML = 12     // K_XM with the x slope and x face flux carried in LDS
 };
 struct TilingChoice {
   int irange;  // planes per work item, 0 = whole patch
@@ -267,7 +279,8 @@ static const TilingChoice kTilings[] = {
     {4, 1, K_WTILE},   {8, 0, K_WTILE},      {0, 1, K_CELLZ},  {0, 0, K_CELLZ},  {0, 1, K_CELLYZ},
     {0, 0, K_CELLYZ},  {0, 1, K_CELLX2},     {0, 0, K_CELLX2}, {0, 1, K_CELLZP}, {0, 0, K_CELLZP},
     {0, 2, K_CELLZ},   {0, 1, K_XM},         {32, 1, K_XM},    {16, 1, K_XM},    {8, 1, K_XM},
-    {0, 1, K_XM8},     {16, 1, K_XM8},       {0, 1, K_XM7},    {16, 1, K_XM7}};
+    {0, 1, K_XM8},     {16, 1, K_XM8},       {0, 1, K_XM7},    {16, 1, K_XM7},   {0, 1, K_XML},
+    {16, 1, K_XML},    {8, 1, K_XML}};
 static const int kNumTilings = sizeof(kTilings) / sizeof(kTilings[0]);
 
 // the column kernel covers the fast MUSCL forms with a symmetric limited slope
@@ -279,19 +292,19 @@ static void launch_march_t(drn_ctx* ctx, const TileMarch* tiles, int ntiles, con
   if (ntiles <= 0) return;
   if constexpr (FAST && RECON == RECON_UPWIND2 && (LIM == LIM_VANALBADA || LIM == LIM_MINMOD)) {
     if (column) {
-      hipLaunchKernelGGL((k_stage_column<T, LIM>), dim3(ntiles), dim3(256), 0, ctx->stream, ctx->d_pd, tiles, src,
+      hipLaunchKernelGGL((k_stage_column<T, LIM>), dim3(ntiles), dim3(256), 0, ctx->stream, ctx->pd_stage, tiles, src,
                          base, dst, ctx->vstride, factor);
       return;
     }
     // slope form: per-cell limited slopes shared by both faces of the cell
     const size_t lds = (size_t)(10 * ctx->lds_cells + 5 * (ctx->lds_sy + ctx->lds_sz + ctx->lds_faces)) * sizeof(T);
-    hipLaunchKernelGGL((k_stage_slope<T, LIM, MARCH_BS>), dim3(ntiles), dim3(MARCH_BS), lds, ctx->stream, ctx->d_pd,
+    hipLaunchKernelGGL((k_stage_slope<T, LIM, MARCH_BS>), dim3(ntiles), dim3(MARCH_BS), lds, ctx->stream, ctx->pd_stage,
                        tiles, src, base, dst, ctx->vstride, factor, ctx->lds_cells, ctx->lds_faces, ctx->lds_sy,
                        ctx->lds_sz);
   } else {
     const size_t lds = (size_t)5 * (ctx->lds_cells + ctx->lds_faces) * sizeof(T);
     hipLaunchKernelGGL((k_stage_march<T, RECON, LIM, FAST, MARCH_BS>), dim3(ntiles), dim3(MARCH_BS), lds, ctx->stream,
-                       ctx->d_pd, tiles, src, base, dst, ctx->vstride, factor, ctx->lds_cells, ctx->lds_faces);
+                       ctx->pd_stage, tiles, src, base, dst, ctx->vstride, factor, ctx->lds_cells, ctx->lds_faces);
   }
 }
 
@@ -301,13 +314,13 @@ static void launch_cell_t(drn_ctx* ctx, const TileCell* tiles, int ntiles, const
   if (ntiles <= 0) return;
   if constexpr (FAST && RECON == RECON_UPWIND2 && (LIM == LIM_VANALBADA || LIM == LIM_MINMOD)) {
     if (slope) {
-      hipLaunchKernelGGL((k_stage_cell_slope<T, LIM, 256>), dim3(ntiles), dim3(256), 0, ctx->stream, ctx->d_pd, tiles,
+      hipLaunchKernelGGL((k_stage_cell_slope<T, LIM, 256>), dim3(ntiles), dim3(256), 0, ctx->stream, ctx->pd_stage, tiles,
                          src, base, dst, ctx->vstride, factor);
       return;
     }
   }
   {
-    hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, FAST, 256>), dim3(ntiles), dim3(256), 0, ctx->stream, ctx->d_pd,
+    hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, FAST, 256>), dim3(ntiles), dim3(256), 0, ctx->stream, ctx->pd_stage,
                        tiles, src, base, dst, ctx->vstride, factor, (int)ctx->push_now);
   }
 }
@@ -315,7 +328,7 @@ static void launch_cell_t(drn_ctx* ctx, const TileCell* tiles, int ntiles, const
 template <typename T, int RECON, int LIM, bool FAST>
 static void launch_wtile_t(drn_ctx* ctx, const TileWave* items, int n, const T* src, const T* base, T* dst, T factor) {
   if (n <= 0) return;
-  hipLaunchKernelGGL((k_stage_wtile<T, RECON, LIM, FAST>), dim3((n + 3) / 4), dim3(256), 0, ctx->stream, ctx->d_pd,
+  hipLaunchKernelGGL((k_stage_wtile<T, RECON, LIM, FAST>), dim3((n + 3) / 4), dim3(256), 0, ctx->stream, ctx->pd_stage,
                      items, n, src, base, dst, ctx->vstride, factor);
 }
 
@@ -332,32 +345,44 @@ static void launch_range(drn_ctx* ctx, int kind, const void* tiles, int ntiles, 
     launch_wtile_t<T, RECON, LIM, FAST>(ctx, (const TileWave*)tiles, ntiles, src, base, dst, factor);
   else if (kind == K_CELLX2) {
     if (ntiles > 0)
-      hipLaunchKernelGGL((k_stage_cell_x2<T, RECON, LIM, FAST>), dim3(ntiles), dim3(256), 0, ctx->stream, ctx->d_pd,
+      hipLaunchKernelGGL((k_stage_cell_x2<T, RECON, LIM, FAST>), dim3(ntiles), dim3(256), 0, ctx->stream, ctx->pd_stage,
                          (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
   } else if (kind == K_CELLYZ) {
     if (ntiles > 0)
-      hipLaunchKernelGGL((k_stage_cell_yz<T, RECON, LIM, FAST>), dim3(ntiles), dim3(512), 0, ctx->stream, ctx->d_pd,
+      hipLaunchKernelGGL((k_stage_cell_yz<T, RECON, LIM, FAST>), dim3(ntiles), dim3(512), 0, ctx->stream, ctx->pd_stage,
                          (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
+  } else if (kind == K_CELLZ && ctx->flags_on) {
+    if (ntiles > 0) {
+      if (ctx->visc)
+        hipLaunchKernelGGL((k_stage_cell_flags<T, RECON, LIM, FAST, true>), dim3(ntiles), dim3(256), 0, ctx->stream,
+                           ctx->pd_stage, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor,
+                           (int)ctx->push_now, (const uint8_t*)ctx->d_flags);
+      else
+        hipLaunchKernelGGL((k_stage_cell_flags<T, RECON, LIM, FAST, false>), dim3(ntiles), dim3(256), 0, ctx->stream,
+                           ctx->pd_stage, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor,
+                           (int)ctx->push_now, (const uint8_t*)ctx->d_flags);
+    }
   } else if (kind == K_CELLZ) {
     if (ntiles > 0 && ctx->visc) {
       if constexpr (sizeof(T) == 4)
         hipLaunchKernelGGL((k_stage_cell_visc_f32<RECON, LIM, FAST>), dim3(ntiles), dim3(256), 0, ctx->stream,
-                           ctx->d_pd, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor, (int)ctx->push_now);
+                           ctx->pd_stage, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor, (int)ctx->push_now);
       else
         hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, FAST, 256, true, true>), dim3(ntiles), dim3(256), 0,
-                           ctx->stream, ctx->d_pd, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor,
+                           ctx->stream, ctx->pd_stage, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor,
                            (int)ctx->push_now);
     } else if (ntiles > 0)
       hipLaunchKernelGGL((k_stage_cell<T, RECON, LIM, FAST, 256, true>), dim3(ntiles), dim3(256), 0, ctx->stream,
-                         ctx->d_pd, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor, (int)ctx->push_now);
-  } else if (kind == K_XM || kind == K_XM8 || kind == K_XM7) {
+                         ctx->pd_stage, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor, (int)ctx->push_now);
+  } else if (kind == K_XM || kind == K_XM8 || kind == K_XM7 || kind == K_XML) {
     if (ntiles > 0) {
       auto kern = k_stage_xm<T, RECON, LIM, FAST>;
+      if (kind == K_XML) kern = k_stage_xml<T, RECON, LIM, FAST>;
       if constexpr (FAST && sizeof(T) == 4) {
         if (kind == K_XM8) kern = k_stage_xm8<T, RECON, LIM, FAST>;
         if (kind == K_XM7) kern = k_stage_xm7<T, RECON, LIM, FAST>;
       }
-      hipLaunchKernelGGL(kern, dim3(ntiles), dim3(256), 0, ctx->stream, ctx->d_pd, (const TileMarch*)tiles, src, base,
+      hipLaunchKernelGGL(kern, dim3(ntiles), dim3(256), 0, ctx->stream, ctx->pd_stage, (const TileMarch*)tiles, src, base,
                          dst, ctx->vstride, factor, (int)ctx->push_now);
     }
   } else if (kind == K_CELLZP) {
@@ -376,7 +401,7 @@ static void launch_range(drn_ctx* ctx, int kind, const void* tiles, int ntiles, 
       // a multiple of 8 keeps every workgroup on the XCD of its first item
       int g = std::max(8, (ctx->n_cu * per_cu) / 8 * 8);
       if (g > ntiles) g = ntiles;
-      hipLaunchKernelGGL((k_stage_cell_persist<T, RECON, LIM, FAST>), dim3(g), dim3(256), 0, ctx->stream, ctx->d_pd,
+      hipLaunchKernelGGL((k_stage_cell_persist<T, RECON, LIM, FAST>), dim3(g), dim3(256), 0, ctx->stream, ctx->pd_stage,
                          (const TileCell*)tiles, ntiles, src, base, dst, ctx->vstride, factor, (int)ctx->push_now);
     }
   }
@@ -407,7 +432,7 @@ static bool column_capable(const drn_ctx* ctx) {
 // with the viscous flux every choice runs the z-shared cell kernel on its list
 // order (the other kernels carry the Euler flux only)
 static int effective_kind(const drn_ctx* ctx, int choice) {
-  if (ctx->visc) return K_CELLZ;
+  if (ctx->visc || ctx->flags_on) return K_CELLZ;
   const int k = kTilings[choice].kind;
   if (k == K_COLUMN && !column_capable(ctx)) return K_MARCH;
   if (k == K_CELL_SLOPE && !column_capable(ctx)) return K_CELL;
@@ -485,7 +510,8 @@ static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int l
 // The cell kernels (choices of kind K_CELL, K_CELLZ, K_CELLZP, K_XM*) can fill the
 // fused z-face receivers themselves; the other kinds leave them to the exchange.
 static bool push_kind(int kind) {
-  return kind == K_CELL || kind == K_CELLZ || kind == K_CELLZP || kind == K_XM || kind == K_XM8 || kind == K_XM7;
+  return kind == K_CELL || kind == K_CELLZ || kind == K_CELLZP || kind == K_XM || kind == K_XM8 || kind == K_XM7 ||
+         kind == K_XML;
 }
 
 // whether a stage of this launch form fills the fused receivers (and the
@@ -495,11 +521,15 @@ static bool push_active(const drn_ctx* ctx, int launch) {
   return push_kind(effective_kind(ctx, launch == DRN_LAUNCH_BATCHED ? ctx->tiling : ctx->tiling_pp));
 }
 
-static int do_stage(drn_ctx* ctx, int psrc, int pbase, int pdst, double factor, int launch, bool push = false) {
+// rk: a stage of the RK step (or the autotuner's stand-in for one), whose
+// exchange follows at once, so that dead faces may skip their functor
+static int do_stage(drn_ctx* ctx, int psrc, int pbase, int pdst, double factor, int launch, bool push = false,
+                    bool rk = false) {
   wrote(ctx, pdst);
   hipEvent_t stop;
   timer_begin(ctx, 0, &stop);
   ctx->push_now = push;
+  ctx->pd_stage = rk && ctx->fuse && ctx->d_pd_rk ? ctx->d_pd_rk : ctx->d_pd;
   if (ctx->prec == DRN_F32)
     stage_t<float>(ctx, psrc, pbase, pdst, (float)factor, launch);
   else
@@ -533,7 +563,7 @@ static void wtile_items(const drn_patch_desc& d, int p, int L, std::vector<TileW
 // wavefronts own 63 cells each
 static int chunk_of(const drn_ctx* ctx, int choice) {
   const int k = effective_kind(ctx, choice);
-  return k == K_COLUMN || k == K_XM || k == K_XM8 || k == K_XM7 ? 252 : MARCH_BS;
+  return k == K_COLUMN || k == K_XM || k == K_XM8 || k == K_XM7 || k == K_XML ? 252 : MARCH_BS;
 }
 
 // build the marching work-lists of the current tiling choices of both launch forms
@@ -726,7 +756,7 @@ static int do_rk_step(drn_ctx* ctx, double dt, int n_alpha, const double* alpha,
       if ((rc = do_exchange(ctx, cur))) return rc;
     }
     const int dst = (s % 2 == 0) ? Y : Z;
-    if ((rc = do_stage(ctx, cur, X, dst, alpha[s] * dt, launch, push))) return rc;
+    if ((rc = do_stage(ctx, cur, X, dst, alpha[s] * dt, launch, push, true))) return rc;
     if ((rc = do_exchange(ctx, dst, push))) return rc;
     if (post) {
       if ((rc = do_ls_fill(ctx, dst, 0.5))) return rc;
@@ -844,6 +874,8 @@ void drn_ctx_destroy(drn_ctx* ctx) {
   free_ls(ctx);
   for (int b = 0; b < NBUF; ++b) dfree(ctx->buf[b]);
   dfree(ctx->d_pd);
+  dfree(ctx->d_pd_rk);
+  dfree(ctx->d_flags);
   dfree(ctx->d_tiles);
   dfree(ctx->d_tiles_xcd);
   dfree(ctx->d_tiles_xcd_pp);
@@ -1148,6 +1180,43 @@ int drn_swap_fields(drn_ctx* ctx, int a, int b) {
   int rc;
   if ((rc = check_field(ctx, a)) || (rc = check_field(ctx, b))) return rc;
   std::swap(ctx->map[a], ctx->map[b]);
+  return DRN_OK;
+}
+
+int drn_set_cell_flags(drn_ctx* ctx, int patch, const uint8_t* flags) {
+  if (!ctx) return DRN_E_ARG;
+  if (!ctx->laid_out) return fail(ctx, DRN_E_STATE, "drn_layout not called");
+  const int np = (int)ctx->descs.size();
+  if (patch < 0 || patch >= np) return fail(ctx, DRN_E_ARG, "patch id out of range");
+  const size_t n = drn_patch_cells(ctx, patch);
+  bool any = false;
+  if (flags)
+    for (size_t l = 0; l < n; ++l) {
+      if (flags[l] & ~(uint8_t)(DRN_CELL_INACTIVE | DRN_CELL_INSIDE))
+        return fail(ctx, DRN_E_ARG, "unknown cell flag bit");
+      any = any || flags[l] != 0;
+    }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (!ctx->d_flags) {
+    if (!any) return DRN_OK;
+    HIPCHK(ctx, dalloc(&ctx->d_flags, ctx->vstride));
+    HIPCHK(ctx, hipMemset(ctx->d_flags, 0, ctx->vstride));
+    ctx->flags_any.assign(np, 0);
+  }
+  if (any)
+    HIPCHK(ctx, hipMemcpy(ctx->d_flags + ctx->offs[patch], flags, n, hipMemcpyHostToDevice));
+  else
+    HIPCHK(ctx, hipMemset(ctx->d_flags + ctx->offs[patch], 0, n));
+  ctx->flags_any[patch] = any ? 1 : 0;
+  bool on = false;
+  for (uint8_t a : ctx->flags_any) on = on || a;
+  if (on != ctx->flags_on) {
+    ctx->flags_on = on;
+    drop_graph(ctx);
+    int rc = build_march(ctx);
+    if (rc) return rc;
+  }
   return DRN_OK;
 }
 
@@ -1479,6 +1548,47 @@ int drn_exchange_commit(drn_ctx* ctx) {
   ctx->have_exchange = !ctx->xfull.single.empty() || ctx->xfull.multi_n > 0;
   ctx->n_fused = n_fused;
   HIPCHK(ctx, hipMemcpy(ctx->d_pd, ctx->pd_host.data(), ctx->pd_host.size() * sizeof(PatchDev), hipMemcpyHostToDevice));
+  // Dead faces: a face with seek layers whose every boundary cell (the layer the
+  // face functor acts on, cartesianiterator.h:190-308) is a receiver of the
+  // exchange.  In the RK step the exchange after the stage overwrites those
+  // cells, so the functor's flux there never reaches the state.
+  {
+    std::vector<PatchDev> rk = ctx->pd_host;
+    int dead = 0;
+    for (const auto& kv : ctx->xt) {
+      const int p = kv.first;
+      const drn_patch_desc& d = ctx->descs[p];
+      std::vector<bool> is_rec((size_t)d.ni * d.nj * d.nk, false);
+      for (int64_t r : kv.second.rec_unique) is_rec[(size_t)r] = true;
+      const int n3[3] = {d.ni, d.nj, d.nk};
+      for (int f = 0; f < 6; ++f) {
+        const int a = f >> 1;
+        if (d.seek[f] < 1 || n3[a] <= 2) continue;
+        const int fixed = (f & 1) ? n3[a] - 1 : 0;
+        const int b = (a + 1) % 3, c = (a + 2) % 3;
+        bool all = true;
+        for (int u = 0; u < n3[b] && all; ++u)
+          for (int v = 0; v < n3[c] && all; ++v) {
+            int ijk[3];
+            ijk[a] = fixed;
+            ijk[b] = u;
+            ijk[c] = v;
+            all = is_rec[((size_t)ijk[0] * d.nj + ijk[1]) * d.nk + ijk[2]];
+          }
+        if (all) {
+          rk[p].bc[f] = BC_DEAD;
+          ++dead;
+        }
+      }
+    }
+    dfree(ctx->d_pd_rk);
+    ctx->d_pd_rk = nullptr;
+    ctx->n_dead_faces = dead;
+    if (dead > 0) {
+      HIPCHK(ctx, dalloc(&ctx->d_pd_rk, rk.size()));
+      HIPCHK(ctx, hipMemcpy(ctx->d_pd_rk, rk.data(), rk.size() * sizeof(PatchDev), hipMemcpyHostToDevice));
+    }
+  }
   return build_chain_list(ctx, push_pairs);
 }
 
@@ -1899,7 +2009,7 @@ int drn_autotune(drn_ctx* ctx, int launch, int* choice) {
     // receivers and leave the exchange less to do
     const bool push = push_active(ctx, launch);
     auto trial = [&]() {
-      int r = do_stage(ctx, ctx->map[0], ctx->map[1], ctx->map[2], 0.0, launch, push);
+      int r = do_stage(ctx, ctx->map[0], ctx->map[1], ctx->map[2], 0.0, launch, push, true);
       return r ? r : do_exchange(ctx, ctx->map[2], push);
     };
     rc = trial();  // warm-up

@@ -18,6 +18,7 @@ DRN_F32, DRN_F64 = 0, 1
 DRN_LAUNCH_BATCHED, DRN_LAUNCH_PER_PATCH = 0, 1
 DRN_ARITH_IEEE, DRN_ARITH_FAST = 0, 1
 DRN_LS_SLIP, DRN_LS_SWALL = 0, 1
+DRN_CELL_INACTIVE, DRN_CELL_INSIDE = 1, 2
 NPAR = 16
 
 
@@ -50,6 +51,7 @@ _SIGS = {
     "drn_exchange_set": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
                                    C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     "drn_exchange_set_turn": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "drn_set_cell_flags": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "drn_exchange_commit": (C.c_int, [C.c_void_p]),
     "drn_exchange": (C.c_int, [C.c_void_p, C.c_int]),
     "drn_set_exchange_fusion": (C.c_int, [C.c_void_p, C.c_int]),
@@ -189,6 +191,14 @@ class Context:
         if tables.get("turn") is not None and len(donors) > 0:
             t = np.ascontiguousarray(tables["turn"], dtype=np.float64).reshape(len(donors), 9)
             self._chk(self.L.drn_exchange_set_turn(self.h, pid, len(donors), _p(t)), "drn_exchange_set_turn")
+
+    def set_cell_flags(self, pid, flags):
+        """per-cell flags of patch pid (DRN_CELL_INACTIVE = 1, DRN_CELL_INSIDE = 2), None clears"""
+        if flags is None:
+            self._chk(self.L.drn_set_cell_flags(self.h, pid, None), "drn_set_cell_flags")
+            return
+        f = np.ascontiguousarray(flags, dtype=np.uint8)
+        self._chk(self.L.drn_set_cell_flags(self.h, pid, _p(f)), "drn_set_cell_flags")
 
     def commit_exchange(self):
         self._chk(self.L.drn_exchange_commit(self.h), "drn_exchange_commit")

@@ -122,6 +122,23 @@ int drn_copy_field(drn_ctx* ctx, int src, int dst);
  * CartesianIterator::compute (field 0 = field 1 + factor/dV * Res(field 0)). */
 int drn_swap_fields(drn_ctx* ctx, int a, int b);
 
+/* Per-cell flags of one patch: `flags` holds one byte per cell in the patch's
+ * cell order, NULL (or all zero) clears them.
+ *   DRN_CELL_INACTIVE  Patch::deactivate (patch_common.h:505): the stage leaves
+ *                      the cell's state as it is (CartesianIterator::compute,
+ *                      cartesianiterator.h:315, updates active cells only);
+ *   DRN_CELL_INSIDE    m_IsInsideCell (patch_common.h:56): a face between an
+ *                      inside and an outside cell is a split face
+ *                      (Patch::isSplitFace, patch_common.h:500-503), and the
+ *                      Upwind2 reconstruction of a cell whose face toward its
+ *                      outer neighbour i0 is a split face stays first order
+ *                      (upwind2.h:41).
+ * While any patch has flags every tiling choice runs the z-shared cell kernel
+ * (27 or 28 by list order) with the flag tests.  After drn_layout; drops a
+ * captured graph. */
+enum { DRN_CELL_INACTIVE = 1, DRN_CELL_INSIDE = 2 };
+int drn_set_cell_flags(drn_ctx* ctx, int patch, const uint8_t* flags);
+
 /* ---- one RK stage: flux + residual + update ------------------------------ */
 /* GPU_CartesianIterator::compute (gpu_cartesianiterator.h:219-304) and
  * CartesianIterator::compute (cartesianiterator.h:87-325): for every patch
@@ -291,6 +308,15 @@ int drn_post_variables(drn_ctx* ctx, int field, int patch, int mask, double ref_
  *       z-face pushes (drn_set_exchange_fusion) plane by plane together, then
  *       XCD segmented: a donor's pushed rows and the receiver's own rows meet
  *       in one L2.  Without fused pushes it is 27; one launch per patch runs 27.
+ * 36-39 the z-shared cell kernel marching its 252 (j,k) cells over an i-range
+ *       (whole patch / 32 / 16 / 8 planes per item, XCD segmented): each x
+ *       face is evaluated once and its left state carried to the next plane,
+ *       so a cell evaluates 4 faces (the march needs 3 or more cells per axis;
+ *       thinner patches run 11);
+ * 40, 41 as 36 / 38 capped at 8 waves per SIMD (FP32 fast arithmetic, else 36 / 38);
+ * 42, 43 as 36 / 38 capped at 7 waves per SIMD (FP32 fast arithmetic, else 36 / 38);
+ * 44-46 as 36 / 38 / 39 with the carried x slope and x face flux held in LDS
+ *       instead of registers.
  * Default for both forms: 11; drn_autotune picks per device. */
 int drn_num_tilings(void);
 int drn_set_tiling(drn_ctx* ctx, int launch, int choice);

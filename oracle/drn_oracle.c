@@ -25,10 +25,12 @@
  *   cartesianpatch_common.h (getXGrad/getYGrad/getZGrad),
  *   applications/drnumCylinder/main.h (EaFlux: Euler flux, then viscous flux).
  *
- * Scope restrictions: patches are axis aligned
- * (base vectors (1,0,0),(0,1,0)); there are no split faces (the immersed
- * boundary path used here is the level set, which never sets
- * m_IsInsideCell, patch.cpp:711); all cells are active (patch.cpp:713).
+ * Patches may be turned (base vectors, patch.cpp:63-85).  Per-cell flags
+ * (grid_set_cell_flags) mark inactive cells, which the update skips
+ * (cartesianiterator.h:315), and inside cells, between which and outside
+ * cells a face is a split face where MUSCL falls back to first order
+ * (upwind2.h:41); without flags every cell is active and none is inside, as
+ * patch.cpp:703-715 allocates them.
  */
 #include "drn_oracle.h"
 
@@ -118,6 +120,7 @@ typedef struct {
   double bc_par[6][O_BC_NPAR];
   REAL far_var[6][5];
   REAL* data; /* 3 fields x 5 vars x N */
+  unsigned char* flags; /* per cell: O_CELL_INACTIVE, O_CELL_INSIDE; NULL = all active, none inside */
 
   /* dependencies (patch.cpp:128-218 direct transfer lists) */
   size_t* recv;
@@ -210,7 +213,15 @@ REAL F(limiter)(int lim, REAL d1, REAL d2) {
   return 1;
 }
 
-/* upwind2.h:29-52 and upwind1.h:29-39; no split faces on this path */
+/* Patch::isSplitFace (patch_common.h:500-503): the face between two cells one
+ * of which is inside (m_IsInsideCell) and the other not */
+static int split_face(const opatch* P, size_t c0, size_t c1) {
+  if (!P->flags) return 0;
+  return ((P->flags[c0] & O_CELL_INSIDE) != 0) != ((P->flags[c1] & O_CELL_INSIDE) != 0);
+}
+
+/* upwind2.h:29-52 and upwind1.h:29-39: second order only where i0 and i2 lie in
+ * the patch and the face i0-i1 is not a split face */
 static void project(const opatch* P, const REAL* f, int recon, int lim, size_t i1, size_t j1, size_t k1,
                     size_t i2, size_t j2, size_t k2, REAL* var) {
   const size_t N = P->N;
@@ -218,7 +229,7 @@ static void project(const opatch* P, const REAL* f, int recon, int lim, size_t i
   for (int v = 0; v < 5; ++v) var[v] = f[v * N + c1];
   if (recon == O_RECON_UPWIND1) return;
   size_t i0 = 2 * i1 - i2, j0 = 2 * j1 - j2, k0 = 2 * k1 - k2;
-  if (in_range(P, i0, j0, k0) && in_range(P, i2, j2, k2)) {
+  if (in_range(P, i0, j0, k0) && in_range(P, i2, j2, k2) && !split_face(P, IDX(P, i0, j0, k0), c1)) {
     const size_t c0 = IDX(P, i0, j0, k0), c2 = IDX(P, i2, j2, k2);
     for (int v = 0; v < 5; ++v) {
       REAL d01 = f[v * N + c1] - f[v * N + c0];
@@ -738,7 +749,23 @@ static void patch_compute(const GRID* g, opatch* P, REAL factor, REAL* res) {
   REAL patch_factor = factor / dV;
   REAL* f0 = fieldp(P, 0);
   const REAL* f1 = fieldp(P, 1);
-  for (size_t l = 0; l < 5 * P->N; ++l) f0[l] = f1[l] + patch_factor * res[l];
+  if (!P->flags) {
+    for (size_t l = 0; l < 5 * P->N; ++l) f0[l] = f1[l] + patch_factor * res[l];
+    return;
+  }
+  /* inactive cells keep field 0 (Patch::isActive, cartesianiterator.h:315) */
+  for (size_t c = 0; c < P->N; ++c)
+    if (!(P->flags[c] & O_CELL_INACTIVE))
+      for (int v = 0; v < 5; ++v) f0[v * P->N + c] = f1[v * P->N + c] + patch_factor * res[v * P->N + c];
+}
+
+void F(grid_set_cell_flags)(GRID* g, int p, const unsigned char* flags) {
+  opatch* P = &g->p[p];
+  free(P->flags);
+  P->flags = NULL;
+  if (!flags) return;
+  P->flags = (unsigned char*)malloc(P->N);
+  memcpy(P->flags, flags, P->N);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -940,6 +967,7 @@ void F(grid_destroy)(GRID* g) {
   if (!g) return;
   for (int p = 0; p < g->np; ++p) {
     free(g->p[p].data);
+    free(g->p[p].flags);
     free_deps(&g->p[p]);
     free_ls(&g->p[p]);
   }

@@ -44,6 +44,11 @@ enum { O_BC_SLIP = 0, O_BC_FARFIELD = 1, O_BC_TRANSMISSIVE = 2, O_BC_JET = 3, O_
  *            [11] p_jet, [12] supersonic flag (nonzero: p = p_jet inside the radius)  */
 #define O_BC_NPAR 16
 
+/* per-cell flags: Patch::deactivate (patch_common.h:505, the update skips the
+ * cell, cartesianiterator.h:315) and m_IsInsideCell (split faces,
+ * patch_common.h:500-503, upwind2.h:41) */
+enum { O_CELL_INACTIVE = 1, O_CELL_INSIDE = 2 };
+
 typedef struct o_patch_desc {
   int32_t ni, nj, nk;           /* cells per axis, seek layers included          */
   int32_t seek[6];              /* seek layers: imin, imax, jmin, jmax, kmin, kmax */
@@ -100,6 +105,8 @@ typedef struct o64_grid o64_grid;
   const REAL* PFX##grid_donor_weight(const GRID* g, int p);                                     \
   /* patch.cpp:573-653 for every patch */                                                       \
   void  PFX##grid_exchange(GRID* g, int field);                                                 \
+  /* per-cell O_CELL_* flags of patch p (N bytes; NULL clears) */                              \
+  void  PFX##grid_set_cell_flags(GRID* g, int p, const unsigned char* flags);                   \
   /* donor d of patch p: the reference's donor_t matrix axx..azz (row-major), which turns    \
    * the donor's momentum into the receiver's frame (patch.cpp:178-186, 627-641) */           \
   void  PFX##grid_donor_matrix(const GRID* g, int p, int d, double* a9);                        \

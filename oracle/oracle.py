@@ -73,6 +73,7 @@ def lib():
                 "grid_donor_index": (C.POINTER(C.c_int64), [C.c_void_p, C.c_int]),
                 "grid_donor_weight": (rp, [C.c_void_p, C.c_int]),
                 "grid_exchange": (None, [C.c_void_p, C.c_int]),
+                "grid_set_cell_flags": (None, [C.c_void_p, C.c_int, C.c_void_p]),
                 "grid_levelset": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_int, C.c_int,
                                             C.c_double]),
                 "grid_levelset_g": (rp, [C.c_void_p, C.c_int]),
@@ -228,6 +229,15 @@ class OracleGrid:
 
     def exchange(self, field=0):
         self._f("grid_exchange")(self.h, field)
+
+    def set_cell_flags(self, p, flags):
+        """per-cell flags of patch p (1 inactive, 2 inside); None clears"""
+        if flags is None:
+            self._f("grid_set_cell_flags")(self.h, p, None)
+            return
+        f = np.ascontiguousarray(flags, dtype=np.uint8)
+        assert f.size == self.cells(p)
+        self._f("grid_set_cell_flags")(self.h, p, f.ctypes.data_as(C.c_void_p))
 
     def rk_step(self, dt=None, alpha=None):
         alpha = self.w.alpha if alpha is None else alpha
