@@ -29,6 +29,12 @@
 //              Without codes=, every patch is added to the iterator.
 //   vtk=B:     at the end, B_<steps, 6 digits>.vtm and B_<steps>/ with one .vts
 //              per patch (PatchGrid::writeToVtk with CompressibleVariables)
+//   jetflux=P,u_jet,u_far,p_jet,p_far,T_jet,T_far,radius,supersonic:
+//              the operator is the jet demo's JetFlux (drnum_amd/host/jetflux.h,
+//              set up as JetFlux::setup, jetflux.h:86-97) around the Euler flux
+//              (or the viscous one with `visc`); it chooses every patch's face
+//              functors (the jet on patch P's x-minus face), in place of the
+//              case file's codes
 // Prints one line per phase and exits non-zero on any error.
 #include <algorithm>
 #include <cmath>
@@ -40,6 +46,7 @@
 #include <vector>
 
 #include "../host/iterators.h"
+#include "../host/jetflux.h"
 
 using namespace drnum;
 
@@ -49,7 +56,7 @@ typedef AusmPlus<5, Upwind2<5, VanAlbada>, PerfectGas> euler_t;
 typedef EulerViscFlux<euler_t, CompressibleViscFlux<5, PerfectGas>> visc_t;
 
 struct Options {
-  std::string grid_file, case_file, out_file, mode, restart, dnd, codes, vtk;
+  std::string grid_file, case_file, out_file, mode, restart, dnd, codes, vtk, jetflux;
   int steps = 0;
   double dt = 0;
   bool levelset = false;
@@ -58,7 +65,7 @@ struct Options {
 };
 
 template <typename real, class OP>
-static int run(const Options& o) {
+static int run(const Options& o, const OP& op = OP()) {
   const std::string &grid_file = o.grid_file, &case_file = o.case_file, &out_file = o.out_file, &mode = o.mode;
   const std::string &restart = o.restart, &dnd = o.dnd, &codes = o.codes, &vtk = o.vtk;
   const int steps = o.steps;
@@ -122,7 +129,7 @@ static int run(const Options& o) {
     printf("restart: t = %g from %s\n", t0, restart.c_str());
   }
 
-  HIP_CartesianIterator<5, OP, real> iterator(OP(), 0);
+  HIP_CartesianIterator<5, OP, real> iterator(op, 0);
   if (codes.empty()) {
     for (auto& P : patches) iterator.addPatch(&P);
   } else {
@@ -247,8 +254,25 @@ int main(int argc, char** argv) {
     if (!strncmp(argv[a], "dnd=", 4)) o.dnd = argv[a] + 4;
     if (!strncmp(argv[a], "codes=", 6)) o.codes = argv[a] + 6;
     if (!strncmp(argv[a], "vtk=", 4)) o.vtk = argv[a] + 4;
+    if (!strncmp(argv[a], "jetflux=", 8)) o.jetflux = argv[a] + 8;
   }
   try {
+    if (!o.jetflux.empty()) {
+      double v[9];
+      if (sscanf(o.jetflux.c_str(), "%lf,%lf,%lf,%lf,%lf,%lf,%lf,%lf,%lf", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5],
+                 &v[6], &v[7], &v[8]) != 9) {
+        fprintf(stderr, "jetflux= needs 9 comma-separated values\n");
+        return 2;
+      }
+      if (visc) {
+        JetFlux<visc_t> op;
+        op.setup(v[1], v[2], v[3], v[4], v[5], v[6], v[7], (int)v[0], v[8] != 0);
+        return f64 ? run<double>(o, op) : run<float>(o, op);
+      }
+      JetFlux<euler_t> op;
+      op.setup(v[1], v[2], v[3], v[4], v[5], v[6], v[7], (int)v[0], v[8] != 0);
+      return f64 ? run<double>(o, op) : run<float>(o, op);
+    }
     if (f64 && visc) return run<double, visc_t>(o);
     if (f64) return run<double, euler_t>(o);
     if (visc) return run<float, visc_t>(o);

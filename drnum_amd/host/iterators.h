@@ -38,6 +38,8 @@
 #include <list>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/drnhip.h"
@@ -83,10 +85,14 @@ struct limiter_code<SecondOrder> {
   static constexpr int value = DRN_LIM_SECONDORDER;
 };
 
-template <class OP>
+template <class OP, class Enable = void>
 struct op_traits {
   static_assert(sizeof(OP) == 0, "this flux operator has no HIP implementation");
 };
+// An application's own flux class (the reference's JetFlux, EaFlux, ...) names
+// the interior flux it composes as `interior_flux`, one of the types below.
+template <class OP>
+struct op_traits<OP, std::void_t<typename OP::interior_flux>> : op_traits<typename OP::interior_flux> {};
 template <class L>
 struct op_traits<AusmPlus<5, Upwind2<5, L>, PerfectGas>> {
   static constexpr int recon = DRN_RECON_UPWIND2;
@@ -105,6 +111,21 @@ struct op_traits<EulerViscFlux<TEuler, CompressibleViscFlux<5, PerfectGas>>> {
   static constexpr int limiter = op_traits<TEuler>::limiter;
   static constexpr int viscous = 1;
 };
+
+// The face functors of a flux class, per patch.  The reference's OP decides in
+// its wall methods which functor a patch face gets (JetFlux::xWallM: the jet on
+// the jet patch, a slip wall elsewhere, jetflux.h:158-192).  An OP that has
+//   bool faces(const CartesianPatch<double>& geometry, size_t patch_index,
+//              int32_t bc[6], double par[6][DRN_BC_NPAR]) const;
+// sets the DRN_BC_* functor and parameters of every face of a patch there
+// (returning true); the grid's face codes are used otherwise.  The far-field
+// state is given in primitive variables p, T, u, v, w.
+template <class OP, class = void>
+struct has_faces : std::false_type {};
+template <class OP>
+struct has_faces<OP, std::void_t<decltype(std::declval<const OP&>().faces(
+                         std::declval<const CartesianPatch<double>&>(), size_t(0), (int32_t*)nullptr,
+                         (double(*)[DRN_BC_NPAR]) nullptr))>> : std::true_type {};
 
 class DeviceError : public std::runtime_error {
  public:
@@ -492,10 +513,26 @@ class HIP_CartesianIterator : public PatchIterator<real> {
       for (int f = 0; f < 6; ++f) {
         d.bc[f] = P.face[f];
         for (int q = 0; q < DRN_BC_NPAR; ++q) d.bc_par[f][q] = P.face_par[f][q];
-        if (P.face[f] == FACE_FARFIELD) {
+      }
+      if constexpr (has_faces<OP>::value) {
+        int32_t bc[6];
+        double par[6][DRN_BC_NPAR];
+        for (int f = 0; f < 6; ++f) {
+          bc[f] = d.bc[f];
+          for (int q = 0; q < DRN_BC_NPAR; ++q) par[f][q] = d.bc_par[f][q];
+        }
+        if (op_.faces(P, hp.getIndex(), bc, par)) {
+          for (int f = 0; f < 6; ++f) {
+            d.bc[f] = bc[f];
+            for (int q = 0; q < DRN_BC_NPAR; ++q) d.bc_par[f][q] = par[f][q];
+          }
+        }
+      }
+      for (int f = 0; f < 6; ++f) {
+        if (d.bc[f] == FACE_FARFIELD) {
           // PerfectGas::primitiveToConservative in `real` (perfectgas.h:56-64)
-          const real pr = real(P.face_par[f][0]), T = real(P.face_par[f][1]), u = real(P.face_par[f][2]),
-                     v = real(P.face_par[f][3]), w = real(P.face_par[f][4]);
+          const real pr = real(d.bc_par[f][0]), T = real(d.bc_par[f][1]), u = real(d.bc_par[f][2]),
+                     v = real(d.bc_par[f][3]), w = real(d.bc_par[f][4]);
           const real rho = pr / (real(287.2) * T);
           d.bc_par[f][0] = (double)rho;
           d.bc_par[f][1] = (double)(real)(rho * u);
