@@ -301,7 +301,7 @@ def profile_hash(path):
     try:
         for ln in open(path).read().splitlines():
             if "kernel source sha256:" in ln:
-                return ln.split("kernel source sha256:", 1)[1].strip().strip("`").split()[0]
+                return ln.split("kernel source sha256:", 1)[1].split()[0].strip("`")
     except OSError:
         pass
     return None
