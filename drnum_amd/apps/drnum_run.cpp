@@ -9,6 +9,7 @@
 // usage: drnum_run <grid file> <case file> <out file> <steps> <dt> <mode> [f64] [levelset] [swall] [visc]
 //                  [restart=<file.dnd>] [dnd=<base name>] [codes=<code string>] [vtk=<base name>]
 //                  [cfl=<target>]  (dt rescaled to the target CFL number after each step)
+//                  [jetflux=<parameters>] [flags=<file>]
 //   case file (binary, little endian), per patch in grid order:
 //     int32 bc[6], float64 par[6][16], then 5*N initial values in `real`
 //     (variable-major, the layout of field 0 in Patch::m_Data);
@@ -35,6 +36,12 @@
 //              (or the viscous one with `visc`); it chooses every patch's face
 //              functors (the jet on patch P's x-minus face), in place of the
 //              case file's codes
+//   flags=F:   per-cell flags, one byte per cell of every patch in grid order
+//              (bit 0 inactive, bit 1 inside): inactive cells are deactivated
+//              (Patch::deactivate) and the inside cells given to the patch as
+//              split faces (Patch::setSplitFaces: every face between an inside
+//              and an outside cell, from both sides), from which the patch
+//              fills its inside
 // Prints one line per phase and exits non-zero on any error.
 #include <algorithm>
 #include <cmath>
@@ -56,7 +63,7 @@ typedef AusmPlus<5, Upwind2<5, VanAlbada>, PerfectGas> euler_t;
 typedef EulerViscFlux<euler_t, CompressibleViscFlux<5, PerfectGas>> visc_t;
 
 struct Options {
-  std::string grid_file, case_file, out_file, mode, restart, dnd, codes, vtk, jetflux;
+  std::string grid_file, case_file, out_file, mode, restart, dnd, codes, vtk, jetflux, flags;
   int steps = 0;
   double dt = 0;
   bool levelset = false;
@@ -127,6 +134,48 @@ static int run(const Options& o, const OP& op = OP()) {
   if (!restart.empty()) {
     t0 = (double)readData<real>(pp, 0, restart);
     printf("restart: t = %g from %s\n", t0, restart.c_str());
+  }
+
+  if (!o.flags.empty()) {
+    std::ifstream fin(o.flags, std::ios::binary);
+    size_t n_inactive = 0, n_faces = 0, n_inside = 0;
+    for (size_t p = 0; p < grid.numPatches(); ++p) {
+      const CartesianPatch<double>& P = grid.patch(p);
+      std::vector<uint8_t> f(P.cells());
+      fin.read(reinterpret_cast<char*>(f.data()), (std::streamsize)f.size());
+      std::vector<splitface_t> faces;
+      for (size_t l = 0; l < P.cells(); ++l) {
+        if (f[l] & DRN_CELL_INACTIVE) {
+          patches[p].deactivate(l);
+          ++n_inactive;
+        }
+        size_t i, j, k;
+        P.ijk(l, i, j, k);
+        const size_t up[3][3] = {{i + 1, j, k}, {i, j + 1, k}, {i, j, k + 1}};
+        for (const auto& q : up) {
+          if (!P.inRange(q[0], q[1], q[2])) continue;
+          const size_t m = P.index(q[0], q[1], q[2]);
+          if (((f[l] ^ f[m]) & DRN_CELL_INSIDE) == 0) continue;
+          splitface_t a, b;
+          a.idx = l;
+          a.idx_neigh = m;
+          a.inside = (f[l] & DRN_CELL_INSIDE) != 0;
+          b.idx = m;
+          b.idx_neigh = l;
+          b.inside = (f[m] & DRN_CELL_INSIDE) != 0;
+          faces.push_back(a);
+          faces.push_back(b);
+        }
+      }
+      patches[p].setSplitFaces(faces);
+      n_faces += faces.size() / 2;
+      for (size_t l = 0; l < P.cells(); ++l) n_inside += patches[p].isInsideCell(l) ? 1 : 0;
+    }
+    if (!fin) {
+      fprintf(stderr, "flags file %s is short\n", o.flags.c_str());
+      return 2;
+    }
+    printf("flags: %zu inactive cells, %zu split faces, %zu inside cells\n", n_inactive, n_faces, n_inside);
   }
 
   HIP_CartesianIterator<5, OP, real> iterator(op, 0);
@@ -232,7 +281,7 @@ int main(int argc, char** argv) {
   if (argc < 7) {
     fprintf(stderr,
             "usage: %s <grid> <case> <out> <steps> <dt> <rk|hip|graph> [f64] [levelset] [swall] [visc] [restart=F] [dnd=B] "
-            "[codes=S] [vtk=B] [cfl=C]\n",
+            "[codes=S] [vtk=B] [cfl=C] [jetflux=...] [flags=F]\n",
             argv[0]);
     return 2;
   }
@@ -255,6 +304,7 @@ int main(int argc, char** argv) {
     if (!strncmp(argv[a], "codes=", 6)) o.codes = argv[a] + 6;
     if (!strncmp(argv[a], "vtk=", 4)) o.vtk = argv[a] + 4;
     if (!strncmp(argv[a], "jetflux=", 8)) o.jetflux = argv[a] + 8;
+    if (!strncmp(argv[a], "flags=", 6)) o.flags = argv[a] + 6;
   }
   try {
     if (!o.jetflux.empty()) {

@@ -165,10 +165,74 @@ class HostPatch {
     }
   }
 
+  // Per-cell flags, the patch's m_Active and m_IsInsideCell (patch_common.h:31,56):
+  // an inactive cell keeps its state through the update (cartesianiterator.h:315),
+  // and a face between an inside and an outside cell is a split face, where the
+  // MUSCL reconstruction keeps first order (upwind2.h:41).  The iterator hands
+  // them to the device with the state (updateDevice).
+  void deactivate(size_t l) { flag(l) |= DRN_CELL_INACTIVE; }  // patch_common.h:505
+  void activate(size_t l) {                                       // patch_common.h:506
+    if (!flags_.empty()) flags_[l] &= (uint8_t)~DRN_CELL_INACTIVE;
+  }
+  bool isActive(size_t l) const { return flags_.empty() || !(flags_[l] & DRN_CELL_INACTIVE); }  // :507
+  void deactivate(size_t i, size_t j, size_t k) { deactivate(geometry().index(i, j, k)); }  // cartesianpatch_common.h:332
+  bool isInsideCell(size_t l) const { return !flags_.empty() && (flags_[l] & DRN_CELL_INSIDE); }  // patch_common.h:485
+  bool isSplitFace(size_t a, size_t b) const { return isInsideCell(a) != isInsideCell(b); }     // patch_common.h:500
+  // Patch::setSplitFaces (patch.h:895-931): the cells of the split faces are
+  // split cells, those of faces marked `inside` are inside cells, and the
+  // inside is then filled through the face neighbours (CartesianPatch::
+  // getNeighbours, cartesianpatch.cpp:1841-1866) up to the split cells.  C is
+  // a container of elements with `idx` and `inside` (splitface_t).
+  template <class C>
+  void setSplitFaces(const C& split_faces) {
+    const CartesianPatch<double>& G = geometry();
+    std::vector<char> split(variableSize(), 0);
+    std::vector<size_t> front, next;
+    for (const auto& f : split_faces) {
+      split[f.idx] = 1;
+      if (f.inside) {
+        flag(f.idx) |= DRN_CELL_INSIDE;
+        front.push_back(f.idx);
+      }
+    }
+    while (!front.empty()) {
+      next.clear();
+      for (size_t l : front) {
+        size_t i, j, k;
+        G.ijk(l, i, j, k);
+        const size_t nb[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+        for (const auto& q : nb) {
+          if (!G.inRange(q[0], q[1], q[2])) continue;  // size_t wrap-around of -1 is out of range
+          const size_t m = G.index(q[0], q[1], q[2]);
+          if (!split[m] && !isInsideCell(m)) {
+            flag(m) |= DRN_CELL_INSIDE;
+            next.push_back(m);
+          }
+        }
+      }
+      front.swap(next);
+    }
+  }
+  // the flags in the layout drn_set_cell_flags takes; null while every cell is
+  // active and outside
+  const uint8_t* cellFlags() const { return flags_.empty() ? nullptr : flags_.data(); }
+
  private:
+  uint8_t& flag(size_t l) {
+    if (flags_.empty()) flags_.assign(variableSize(), 0);
+    return flags_[l];
+  }
   const PatchGrid<double>* grid_;
   size_t index_, nf_, nv_;
   std::vector<real> data_;
+  std::vector<uint8_t> flags_;
+};
+
+// splitface_t (splitface_t.h:24-38): the fields setSplitFaces reads are idx and inside
+struct splitface_t {
+  size_t idx = 0, idx_neigh = 0;
+  double nx = 0, ny = 0, nz = 0, wnx = 0, wny = 0, wnz = 0, dist = 0, h1 = 0, h2 = 0;
+  bool inside = false;
 };
 
 // PatchGrid::writeData / readData (patchgrid.cpp:563-610): field i_field of
@@ -356,11 +420,16 @@ class HIP_CartesianIterator : public PatchIterator<real> {
   }
 
   // Patch -> device copy of all fields (GPU_Patch::copyToDevice, gpu_patch.h:130-134)
+  // and the per-cell flags (m_Active, m_IsInsideCell) of every patch
   void updateDevice() override {
     setup();
-    for (size_t p = 0; p < this->patches_.size(); ++p)
+    for (size_t p = 0; p < this->patches_.size(); ++p) {
       for (size_t f = 0; f < this->patches_[p]->numFields() && f < 3; ++f)
         check(drn_patch_upload(ctx_, (int)p, (int)f, this->patches_[p]->getField(f)), "drn_patch_upload");
+      const uint8_t* fl = this->patches_[p]->cellFlags();
+      if (fl || flags_sent_) check(drn_set_cell_flags(ctx_, (int)p, fl), "drn_set_cell_flags");
+      if (fl) flags_sent_ = true;
+    }
   }
   // device -> Patch copy of all fields (GPU_Patch::copyFromDevice, gpu_patch.h:136-139)
   void updateHost() override {
@@ -581,6 +650,7 @@ class HIP_CartesianIterator : public PatchIterator<real> {
   OP op_;
   drn_ctx* ctx_ = nullptr;
   bool laid_out_ = false;
+  bool flags_sent_ = false;  // some patch has had flags on the device
   int launch_ = DRN_LAUNCH_BATCHED;
   const GenericOperation* ls_op_ = nullptr;
 };

@@ -8,7 +8,9 @@ captured graph.
   * every mode gives the oracle's result (FP64 1e-11, FP32 2e-5 relative max
     norm) on the jet and on spheres with the level-set post operation;
   * the three modes give bit-identical states (the same kernels on the same
-    inputs; only the number of copies and exchanges differs).
+    inputs; only the number of copies and exchanges differs);
+  * per-cell flags set on the host patches (deactivate, setSplitFaces) reach
+    the device and give the oracle's result with the same flags.
 """
 import os
 import subprocess
@@ -240,3 +242,39 @@ def test_dropin_cfl_time_step(gpu, tmp_path, dtype):
     assert outs["rk"][0] == outs["hip"][0]
     for a, b in zip(outs["rk"][1], outs["hip"][1]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_dropin_cell_flags(gpu, tmp_path, dtype):
+    """HostPatch::deactivate and HostPatch::setSplitFaces (patch_common.h:505,
+    patch.h:895-931) through the drop-in: drnum_run deactivates the inactive
+    cells of a flags file and gives the patches the faces between inside and
+    outside cells, from which they fill their inside; every mode then equals
+    the oracle with the same flags, and the filled inside equals the file's."""
+    from test_gpu_cellflags import jet_flags
+
+    w = configs.jet(core=8, blocks=2)
+    flags = jet_flags(w)
+    path = tmp_path / "flags.bin"
+    path.write_bytes(b"".join(f.tobytes() for f in flags))
+    og = O.OracleGrid(w, dtype, threads=8)
+    og.init_state()
+    og.compute_dependencies()
+    for p, f in enumerate(flags):
+        og.set_cell_flags(p, f)
+    for _ in range(5):
+        og.rk_step()
+    out = []
+    res = {m: run_app(tmp_path, w, dtype, 5, m, extra=("flags=%s" % path,), tag="fl", stdout=out)
+           for m in ("rk", "hip", "graph")}
+    want = "flags: %d inactive cells" % sum(int((f & 1).sum()) for f in flags)
+    inside = "%d inside cells" % sum(int(((f >> 1) & 1).sum()) for f in flags)
+    for text in out:
+        assert want in text and inside in text, text
+    for m, st in res.items():
+        err = rel_linf(st, og)
+        print("drop-in flags", m, np.dtype(dtype).name, err, flush=True)
+        assert err <= TOL[dtype]
+    for m in ("hip", "graph"):
+        for a, b in zip(res["rk"], res[m]):
+            assert np.array_equal(a, b), m
