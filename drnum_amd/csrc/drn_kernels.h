@@ -2299,44 +2299,80 @@ __global__ __launch_bounds__(256) void k_exchange_single(uint32_t n, int stride,
 }
 
 // receivers served by several donor patches, or by a turned one: donor sums
-// added in donor order.  TURN: a group with gturn[g] = t > 0 turns the momentum
-// of its donor sum into the receiver's frame with the row-major matrix t - 1 of
-// `turn` before adding it (patch.cpp:617-645: inter_vars, turned, added).
+// added in donor order.  The receivers are sorted by their number of (donor
+// cell, weight) entries, and each wavefront's 64 receivers keep their entries
+// interleaved: entry q of lane L at tbase[r / 64] + 64 q + L, so that a lane
+// loops over its own entries only and every entry load is one coalesced
+// access.  An entry is the donor cell index with bit 31 set where it closes
+// the sum of one donor patch.  TURN: at such an entry, et = t > 0 turns the
+// momentum of that donor sum into the receiver's frame with the row-major
+// matrix t - 1 of `turn` before it is added (patch.cpp:617-645: inter_vars,
+// turned, added).
 template <typename T, bool TURN>
 __global__ __launch_bounds__(256) void k_exchange_multi(uint32_t n, const uint32_t* __restrict__ rec,
-                                                        const uint32_t* __restrict__ gptr,
-                                                        const uint32_t* __restrict__ gstride,
-                                                        const uint32_t* __restrict__ gwi,
-                                                        const uint32_t* __restrict__ idx, const T* __restrict__ w,
-                                                        T* U, uint64_t vs, const uint32_t* __restrict__ gturn,
+                                                        const uint32_t* __restrict__ cnt,
+                                                        const uint32_t* __restrict__ tbase,
+                                                        const uint32_t* __restrict__ ent, const T* __restrict__ w,
+                                                        T* U, uint64_t vs, const uint32_t* __restrict__ et,
                                                         const T* __restrict__ turn) {
 #pragma clang fp contract(off)
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
-  T tot[5];
+  const uint32_t e0 = tbase[r >> 6] + (r & 63u), ne = cnt[r];
+  T tot[5], acc[5];
   zero5(tot);
-  for (uint32_t g = gptr[r]; g < gptr[r + 1]; ++g) {
-    T acc[5];
-    zero5(acc);
-    const uint32_t st = gstride[g], w0 = gwi[g];
-    for (uint32_t s = 0; s < st; ++s) {
-      const uint64_t d = idx[w0 + s];
-      const T ww = w[w0 + s];
+  zero5(acc);
+  // one entry in order: accumulate, and at the end of a donor patch's entries
+  // (turn and) add the donor sum
+  auto add = [&](uint32_t e, uint32_t x, T ww, const T* u) {
 #pragma unroll
-      for (int v = 0; v < 5; ++v) acc[v] += U[v * vs + d] * ww;
-    }
-    if constexpr (TURN) {
-      const uint32_t t = gturn[g];
-      if (t > 0) {
-        const T* a = turn + 9 * (uint64_t)(t - 1);
-        const T m1 = acc[1], m2 = acc[2], m3 = acc[3];
-        acc[1] = a[0] * m1 + a[1] * m2 + a[2] * m3;
-        acc[2] = a[3] * m1 + a[4] * m2 + a[5] * m3;
-        acc[3] = a[6] * m1 + a[7] * m2 + a[8] * m3;
+    for (int v = 0; v < 5; ++v) acc[v] += u[v] * ww;
+    if (x >> 31) {
+      if constexpr (TURN) {
+        const uint32_t t = et[e];
+        if (t > 0) {
+          const T* a = turn + 9 * (uint64_t)(t - 1);
+          const T m1 = acc[1], m2 = acc[2], m3 = acc[3];
+          acc[1] = a[0] * m1 + a[1] * m2 + a[2] * m3;
+          acc[2] = a[3] * m1 + a[4] * m2 + a[5] * m3;
+          acc[3] = a[6] * m1 + a[7] * m2 + a[8] * m3;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        tot[v] += acc[v];
+        acc[v] = T(0);
       }
     }
+  };
+  uint32_t q = 0;
+  // four entries at a time: their index, weight and donor loads are issued
+  // together, the sums are still formed entry after entry
+  for (; q + 4 <= ne; q += 4) {
+    uint32_t x[4];
+    T ww[4], u[4][5];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) tot[v] += acc[v];
+    for (int m = 0; m < 4; ++m) {
+      x[m] = ent[e0 + 64u * (q + m)];
+      ww[m] = w[e0 + 64u * (q + m)];
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const uint64_t d = x[m] & 0x7fffffffu;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) u[m][v] = U[v * vs + d];
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) add(e0 + 64u * (q + m), x[m], ww[m], u[m]);
+  }
+  for (; q < ne; ++q) {
+    const uint32_t e = e0 + 64u * q;
+    const uint32_t x = ent[e];
+    const uint64_t d = x & 0x7fffffffu;
+    T u[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) u[v] = U[v * vs + d];
+    add(e, x, w[e], u);
   }
   const uint64_t g = rec[r];
 #pragma unroll

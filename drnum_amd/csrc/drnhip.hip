@@ -138,11 +138,13 @@ struct drn_ctx {
       void* w;
     };
     std::vector<SingleClass> single;
+    // several donor patches per receiver (k_exchange_multi): receivers sorted by
+    // entry count, entries interleaved per wavefront of 64 receivers
     uint32_t multi_n = 0;
-    uint32_t *m_rec = nullptr, *m_gptr = nullptr, *m_gstride = nullptr, *m_gwi = nullptr, *m_idx = nullptr;
+    uint32_t *m_rec = nullptr, *m_cnt = nullptr, *m_tbase = nullptr, *m_ent = nullptr;
     void* m_w = nullptr;
-    uint32_t* m_gturn = nullptr;  // per group: 0 = aligned, t > 0 = turn matrix t - 1 of m_turn
-    void* m_turn = nullptr;       // 9 values per turn matrix, in the state precision
+    uint32_t* m_et = nullptr;  // per entry closing a donor sum: 0 = aligned, t > 0 = turn matrix t - 1 of m_turn
+    void* m_turn = nullptr;    // 9 values per turn matrix, in the state precision
   };
   ExchangeLists xfull, xpart;
   bool have_exchange = false;
@@ -678,14 +680,14 @@ static void launch_exchange_t(drn_ctx* ctx, const drn_ctx::ExchangeLists& x, T* 
     hipLaunchKernelGGL(k_exchange_single<T>, dim3((c.n + 255) / 256), dim3(256), 0, ctx->stream, c.n, c.stride, c.rec,
                        c.idx, (const T*)c.w, U, ctx->vstride);
   }
-  if (x.multi_n > 0 && x.m_gturn) {
+  if (x.multi_n > 0 && x.m_et) {
     hipLaunchKernelGGL((k_exchange_multi<T, true>), dim3((x.multi_n + 255) / 256), dim3(256), 0, ctx->stream,
-                       x.multi_n, x.m_rec, x.m_gptr, x.m_gstride, x.m_gwi, x.m_idx, (const T*)x.m_w, U, ctx->vstride,
-                       x.m_gturn, (const T*)x.m_turn);
+                       x.multi_n, x.m_rec, x.m_cnt, x.m_tbase, x.m_ent, (const T*)x.m_w, U, ctx->vstride, x.m_et,
+                       (const T*)x.m_turn);
   } else if (x.multi_n > 0) {
     hipLaunchKernelGGL((k_exchange_multi<T, false>), dim3((x.multi_n + 255) / 256), dim3(256), 0, ctx->stream,
-                       x.multi_n, x.m_rec, x.m_gptr, x.m_gstride, x.m_gwi, x.m_idx, (const T*)x.m_w, U, ctx->vstride,
-                       nullptr, nullptr);
+                       x.multi_n, x.m_rec, x.m_cnt, x.m_tbase, x.m_ent, (const T*)x.m_w, U, ctx->vstride, nullptr,
+                       nullptr);
   }
 }
 
@@ -835,14 +837,13 @@ static void free_lists(drn_ctx::ExchangeLists& x) {
   }
   x.single.clear();
   dfree(x.m_rec);
-  dfree(x.m_gptr);
-  dfree(x.m_gstride);
-  dfree(x.m_gwi);
-  dfree(x.m_idx);
+  dfree(x.m_cnt);
+  dfree(x.m_tbase);
+  dfree(x.m_ent);
   dfree(x.m_w);
-  dfree(x.m_gturn);
+  dfree(x.m_et);
   dfree(x.m_turn);
-  x.m_rec = x.m_gptr = x.m_gstride = x.m_gwi = x.m_idx = x.m_gturn = nullptr;
+  x.m_rec = x.m_cnt = x.m_tbase = x.m_ent = x.m_et = nullptr;
   x.m_w = x.m_turn = nullptr;
   x.multi_n = 0;
 }
@@ -1519,26 +1520,50 @@ int drn_exchange_commit(drn_ctx* ctx) {
       x.single.push_back(c);  // freed with the lists, also after a failed upload
       if (rc) return fail(ctx, DRN_E_NOMEM, "exchange list upload failed");
     }
-    std::vector<uint32_t> rec, gptr{0}, gstride, gwi, idx, gturn;
-    std::vector<double> w;
+    // the multi-donor receivers, sorted by their number of entries (stable),
+    // so that the 64 lanes of a wavefront loop over about as many entries each;
+    // wavefront b's entries start at tbase[b], entry q of lane L at + 64 q + L
+    std::vector<std::pair<size_t, const std::pair<uint32_t, std::vector<Group>>*>> order;  // (entries, receiver)
     for (const auto& e : multi) {
       if (skip_fused && fused[e.first]) continue;
-      rec.push_back(e.first);
-      for (const auto& g : e.second) {
-        gturn.push_back((uint32_t)g.turn);
-        gstride.push_back((uint32_t)g.stride);
-        gwi.push_back((uint32_t)idx.size());
-        idx.insert(idx.end(), g.idx.begin(), g.idx.end());
-        w.insert(w.end(), g.w.begin(), g.w.end());
+      size_t n = 0;
+      for (const auto& g : e.second) n += g.idx.size();
+      order.push_back({n, &e});
+    }
+    std::stable_sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+    std::vector<uint32_t> rec, cnt, tbase, ent, et;
+    std::vector<double> w;
+    const bool turned = !turns.empty();
+    for (size_t b = 0; b < order.size(); b += 64) {
+      const size_t nb = std::min<size_t>(64, order.size() - b);
+      const size_t emax = order[b].first;  // the first of a descending run
+      if (ent.size() + 64 * emax > 0xffffffffu) return fail(ctx, DRN_E_RANGE, "exchange lists exceed 2^32 entries");
+      tbase.push_back((uint32_t)ent.size());
+      ent.resize(ent.size() + 64 * emax, 0u);
+      w.resize(ent.size(), 0.0);
+      if (turned) et.resize(ent.size(), 0u);
+      for (size_t L = 0; L < nb; ++L) {
+        const auto& e = *order[b + L].second;
+        rec.push_back(e.first);
+        size_t q = 0;
+        for (const auto& g : e.second) {
+          for (size_t s2 = 0; s2 < g.idx.size(); ++s2, ++q) {
+            const size_t at = tbase.back() + 64 * q + L;
+            if (g.idx[s2] & 0x80000000u) return fail(ctx, DRN_E_RANGE, "donor cell index beyond 2^31");
+            ent[at] = g.idx[s2] | (s2 + 1 == g.idx.size() ? 0x80000000u : 0u);
+            w[at] = g.w[s2];
+            if (turned && s2 + 1 == g.idx.size()) et[at] = (uint32_t)g.turn;
+          }
+        }
+        cnt.push_back((uint32_t)q);
       }
-      gptr.push_back((uint32_t)gstride.size());
     }
     if (!rec.empty()) {
       x.multi_n = (uint32_t)rec.size();
-      if (upload_vec(ctx, &x.m_rec, rec) || upload_vec(ctx, &x.m_gptr, gptr) || upload_vec(ctx, &x.m_gstride, gstride) ||
-          upload_vec(ctx, &x.m_gwi, gwi) || upload_vec(ctx, &x.m_idx, idx) || upload_weights(ctx, &x.m_w, w))
+      if (upload_vec(ctx, &x.m_rec, rec) || upload_vec(ctx, &x.m_cnt, cnt) || upload_vec(ctx, &x.m_tbase, tbase) ||
+          upload_vec(ctx, &x.m_ent, ent) || upload_weights(ctx, &x.m_w, w))
         return fail(ctx, DRN_E_NOMEM, "exchange list upload failed");
-      if (!turns.empty() && (upload_vec(ctx, &x.m_gturn, gturn) || upload_weights(ctx, &x.m_turn, turns)))
+      if (turned && (upload_vec(ctx, &x.m_et, et) || upload_weights(ctx, &x.m_turn, turns)))
         return fail(ctx, DRN_E_NOMEM, "exchange list upload failed");
     }
     return DRN_OK;
