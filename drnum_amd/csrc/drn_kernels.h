@@ -2319,6 +2319,13 @@ __global__ __launch_bounds__(256) void k_exchange_multi(uint32_t n, const uint32
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const uint32_t e0 = tbase[r >> 6] + (r & 63u), ne = cnt[r];
+  // uniform bases of the five variable arrays and 32-bit byte offsets (the
+  // layout keeps a variable array below 2^32 bytes): one address register per
+  // donor cell instead of five 64-bit ones
+  constexpr uint32_t W = sizeof(T);
+  const char* ub[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) ub[v] = reinterpret_cast<const char*>(U + (uint64_t)v * vs);
   T tot[5], acc[5];
   zero5(tot);
   zero5(acc);
@@ -2358,9 +2365,9 @@ __global__ __launch_bounds__(256) void k_exchange_multi(uint32_t n, const uint32
     }
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      const uint64_t d = x[m] & 0x7fffffffu;
+      const uint32_t b = (x[m] & 0x7fffffffu) * W;
 #pragma unroll
-      for (int v = 0; v < 5; ++v) u[m][v] = U[v * vs + d];
+      for (int v = 0; v < 5; ++v) u[m][v] = ldv<T>(ub[v], b);
     }
 #pragma unroll
     for (int m = 0; m < 4; ++m) add(e0 + 64u * (q + m), x[m], ww[m], u[m]);
@@ -2368,15 +2375,15 @@ __global__ __launch_bounds__(256) void k_exchange_multi(uint32_t n, const uint32
   for (; q < ne; ++q) {
     const uint32_t e = e0 + 64u * q;
     const uint32_t x = ent[e];
-    const uint64_t d = x & 0x7fffffffu;
+    const uint32_t b = (x & 0x7fffffffu) * W;
     T u[5];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) u[v] = U[v * vs + d];
+    for (int v = 0; v < 5; ++v) u[v] = ldv<T>(ub[v], b);
     add(e, x, w[e], u);
   }
-  const uint64_t g = rec[r];
+  const uint32_t g = rec[r] * W;
 #pragma unroll
-  for (int v = 0; v < 5; ++v) U[v * vs + g] = tot[v];
+  for (int v = 0; v < 5; ++v) *reinterpret_cast<T*>(const_cast<char*>(ub[v]) + g) = tot[v];
 }
 
 // ---------------------------------------------------------------------------
