@@ -246,3 +246,20 @@ def test_dnd_rejects_a_file_of_another_grid(tmp_path):
         host.dnd_read(path, [f.shape for f in fields] + [(5, 4)], np.float32)  # file shorter than the grid
     with pytest.raises(RuntimeError):
         host.dnd_read(str(tmp_path / "missing.dnd"), [(5, 1)], np.float32)
+
+
+def test_dnd_rejects_a_corrupted_file(tmp_path):
+    """A precision word other than 4 or 8, a file cut inside its header and a
+    file cut inside its last value are refused, not read as a state."""
+    fields = _fields(np.float32)
+    shapes = [f.shape for f in fields]
+    path = tmp_path / "good.dnd"
+    host.dnd_write(str(path), fields, 1.0)
+    raw = path.read_bytes()
+    bad = tmp_path / "bad.dnd"
+    for data in (np.array([7], dtype=">i4").tobytes() + raw[4:], raw[:6], raw[:len(raw) - 3]):
+        bad.write_bytes(data)
+        with pytest.raises(RuntimeError):
+            host.dnd_read(str(bad), shapes, np.float32)
+    got, t, prec = host.dnd_read(str(path), shapes, np.float32)
+    assert t == 1.0 and prec == 4

@@ -93,3 +93,24 @@ def test_cfl_is_the_jet_demos_x_only_number(gpu, dtype):
     assert gg.ctx.cfl_axes(0, dt) == pytest.approx(best_3, rel=1e-14)
     assert best_3 > 1.3 * best_x
     gg.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_nonfinite_values_are_counted(gpu, dtype):
+    """drn_diagnostics counts every NaN and infinity of the state over all stored
+    cells (seek layers included), so that a run can refuse a diverged result;
+    a clean state counts none."""
+    w = configs.jet(core=8, blocks=2)
+    gg = solver.GpuGrid(w, dtype)
+    gg.upload_initial()
+    assert gg.ctx.diagnostics(0, w.dt)["nonfinite"] == 0
+    states = gg.state(0)
+    n_last = w.patches[7].cells - 1
+    bad = [(0, 0, 5, np.nan), (3, 4, 17, np.inf), (7, 2, n_last, -np.inf), (5, 1, 0, np.nan), (5, 3, 0, np.nan)]
+    for p, v, cell, x in bad:
+        states[p][v, cell] = x
+    gg.upload_state(states, 0)
+    d = gg.ctx.diagnostics(0, w.dt)
+    print("non-finite values counted", d["nonfinite"], np.dtype(dtype).name, flush=True)
+    assert d["nonfinite"] == len(bad)
+    gg.close()
