@@ -192,10 +192,12 @@ __device__ __forceinline__ void visc_add(const char* const* sv, const ViscGeom<T
 // its cell i1 are on the same side of the inside boundary.
 __device__ __forceinline__ bool split_bits(uint32_t ins, int a, int b) { return ((ins >> a) ^ (ins >> b)) & 1u; }
 
+// minus = false: the minus face is left out (another wavefront evaluates it as
+// its plus face); its slope is still formed where the plus face needs it
 template <typename T, int RECON, int LIM, bool FAST, int DIR, bool VISC = false, bool FLG = false>
 __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, uint32_t st, int idx, int n,
                                              const T* u0, T A, T* res, T* fp, const ViscGeom<T>* G = nullptr,
-                                             uint32_t ins = 0) {
+                                             uint32_t ins = 0, bool minus = true) {
   T um2[5], um1[5], up1[5], up2[5];
   const bool have = n > 2;
 #pragma unroll
@@ -242,7 +244,7 @@ __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, 
       s0[v] = ok0 ? limited<T, LIM, true>(u0[v] - um1[v], up1[v] - u0[v]) : T(0);
       sp[v] = okp ? limited<T, LIM, true>(up1[v] - u0[v], up2[v] - up1[v]) : T(0);
     }
-    if (idx > 0) {
+    if (idx > 0 && minus) {
       T vl[5], vr[5], f[5];
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
@@ -266,7 +268,7 @@ __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, 
       if constexpr (VISC) visc_add<T, FAST, DIR>(sv, *G, c, c + st, u0, up1, A, fp);
     }
   } else {
-    if (have && idx > 0) {
+    if (have && idx > 0 && minus) {
       T f[5];
       zero5(f);
       face_flux_r<T, RECON, LIM, FAST, DIR>(um2, um1, u0, up1, idx >= 2 && !(FLG && split_bits(ins, 0, 1)),
@@ -293,26 +295,38 @@ __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, 
 //             taken by lane shuffles (15 global loads fewer per cell).
 // FLG: per-cell flags (DRN_CELL_INACTIVE = 1, DRN_CELL_INSIDE = 2) at flg[global
 // cell]: split faces reconstruct first order, inactive cells keep their state.
-template <typename T, int RECON, int LIM, bool FAST, int BS, bool ZS, bool VISC = false, bool FLG = false>
+// XP (z-shared form only): planes per work item.  The four wavefronts of an
+// item cover 4/XP x 63 cells of each of XP consecutive planes i0 .. i0+XP-1 at
+// the same (j,k) cells; a wavefront above the first plane takes its x face
+// i-1/2 from the wavefront below, which evaluated it as its plus face and left
+// it in xbuf (LDS, 4 x 5 x 64 values), so a cell evaluates 5.02 - (XP-1)/XP
+// faces.  The face is the same function of the same four cells either way.
+template <typename T, int RECON, int LIM, bool FAST, int BS, bool ZS, bool VISC = false, bool FLG = false,
+          int XP = 1>
 __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds, const TileCell t,
                                                 const T* __restrict__ src, const T* base, T* dst, uint64_t vs,
-                                                T factor, int push, const uint8_t* __restrict__ flg = nullptr) {
+                                                T factor, int push, const uint8_t* __restrict__ flg = nullptr,
+                                                T* xbuf = nullptr) {
+  static_assert(XP == 1 || (ZS && !VISC && !FLG && (XP == 2 || XP == 4)), "planes per item: 1, 2 or 4");
   const PatchDev& P = pds[t.patch];
   const int ni = P.ni, nj = P.nj, nk = P.nk, njk = P.njk;
   const int lane = (int)(threadIdx.x & 63);
-  const int c2 = ZS ? t.c2 + 63 * (int)(threadIdx.x >> 6) + lane : t.c2 + (int)threadIdx.x;
+  constexpr int WPP = 4 / XP;  // wavefronts per plane
+  const int wv = (int)(threadIdx.x >> 6), pw = wv / WPP;
+  const int c2 = ZS ? t.c2 + 63 * (wv % WPP) + lane : t.c2 + (int)threadIdx.x;
   if constexpr (!ZS) {
     if (c2 >= njk) return;
   }
-  const bool in_plane = c2 < njk;               // a cell of the patch plane
-  const bool own = in_plane && (!ZS || lane < 63);  // this lane updates its cell
-  const int i = t.i;
-  const int cc2 = in_plane ? c2 : njk - 1;
+  const int i = t.i + (XP > 1 ? pw : 0);
+  const bool in_plane = c2 < njk && (XP == 1 || i < ni);  // a cell of the patch
+  const bool own = in_plane && (!ZS || lane < 63);        // this lane updates its cell
+  const int ii = XP > 1 && i >= ni ? ni - 1 : i;          // a plane of the patch for the addresses
+  const int cc2 = c2 < njk ? c2 : njk - 1;
   const int j = cc2 / nk;
   const int k = cc2 - j * nk;
   // byte offsets within one variable array (the layout keeps them below 2^32)
   constexpr uint32_t W = sizeof(T);
-  const uint32_t c = (P.off + (uint32_t)i * (uint32_t)njk + (uint32_t)cc2) * W;
+  const uint32_t c = (P.off + (uint32_t)ii * (uint32_t)njk + (uint32_t)cc2) * W;
   const uint32_t sx = (uint32_t)njk * W, sy = (uint32_t)nk * W;
   const char* sv[5];
 #pragma unroll
@@ -347,7 +361,7 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
   }
   // the cell's flags, and the inside bits of the x and y neighbours
   uint32_t fo = 0, insx = 0, insy = 0;
-  const uint32_t gc = P.off + (uint32_t)i * (uint32_t)njk + (uint32_t)cc2;  // the cell's global index
+  const uint32_t gc = P.off + (uint32_t)ii * (uint32_t)njk + (uint32_t)cc2;  // the cell's global index
   if constexpr (FLG) {
     fo = flg[gc];
     auto ins_axis = [&](uint32_t stc, int idx, int n) {
@@ -363,7 +377,19 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
       insy = ins_axis((uint32_t)nk, j, nj);
     }
   }
-  if (own) {
+  if constexpr (XP > 1) {
+    if (own) axis_faces_r<T, RECON, LIM, FAST, 0>(sv, c, sx, i, ni, u0, Ax, res, fxp, &G, 0, pw == 0);
+    if (own) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) xbuf[(wv * 5 + v) * 64 + lane] = fxp[v];
+    }
+    __syncthreads();
+    if (own && pw > 0 && i > 0) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) res[v] += xbuf[((wv - WPP) * 5 + v) * 64 + lane];
+    }
+    if (own) axis_faces_r<T, RECON, LIM, FAST, 1>(sv, c, sy, j, nj, u0, Ay, res, fyp, &G);
+  } else if (own) {
     axis_faces_r<T, RECON, LIM, FAST, 0, VISC, FLG>(sv, c, sx, i, ni, u0, Ax, res, fxp, &G, insx);
     axis_faces_r<T, RECON, LIM, FAST, 1, VISC, FLG>(sv, c, sy, j, nj, u0, Ay, res, fyp, &G, insy);
   }
@@ -525,6 +551,17 @@ __global__ __launch_bounds__(BS) void k_stage_cell(const PatchDev* __restrict__ 
                                                    const T* __restrict__ src, const T* base, T* dst, uint64_t vs,
                                                    T factor, int push) {
   stage_cell_item<T, RECON, LIM, FAST, BS, ZS, VISC>(pds, tiles[blockIdx.x], src, base, dst, vs, factor, push);
+}
+
+// the z-shared cell kernel over XP planes per work item, x faces shared between
+// the planes through LDS
+template <typename T, int RECON, int LIM, bool FAST, int XP>
+__global__ __launch_bounds__(256) void k_stage_cell_xs(const PatchDev* __restrict__ pds,
+                                                       const TileCell* __restrict__ tiles, const T* __restrict__ src,
+                                                       const T* base, T* dst, uint64_t vs, T factor, int push) {
+  __shared__ T xbuf[4 * 5 * 64];
+  stage_cell_item<T, RECON, LIM, FAST, 256, true, false, false, XP>(pds, tiles[blockIdx.x], src, base, dst, vs,
+                                                                    factor, push, nullptr, xbuf);
 }
 
 // the z-shared cell kernel with per-cell flags (inactive cells, split faces)

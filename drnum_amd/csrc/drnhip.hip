@@ -93,6 +93,12 @@ struct drn_ctx {
   TileCell* d_tz_chain = nullptr;  // z-shared list interleaved along the fused push chains (commit)
   TileCell* d_tz_xcd_pp = nullptr;
   // 2-plane x 252-cell items of the two-plane cell kernel
+  // items of the x-shared cell kernels, XP planes x (4/XP) x 63 cells; xs[0]: XP = 2, xs[1]: XP = 4
+  struct XsList {
+    std::vector<TileCell> nat;
+    std::vector<int> first;
+    TileCell *d = nullptr, *d_xcd = nullptr, *d_xcd_pp = nullptr, *d_chain = nullptr;
+  } xs[2];
   std::vector<TileCell> tx2;
   std::vector<int> tx2_first;
   TileCell* d_tx2 = nullptr;
@@ -263,7 +269,9 @@ enum {
   K_XM = 9,      // the z-shared cell kernel marching its (j,k) cells over an i-range
   K_XM8 = 10,    // K_XM capped at 8 waves per SIMD (FP32 fast arithmetic; else K_XM)
   K_XM7 = 11,    // K_XM capped at 7 waves per SIMD (FP32 fast arithmetic; else K_XM)
-  K_XML = 12     // K_XM with the x slope and x face flux carried in LDS
+  K_XML = 12,    // K_XM with the x slope and x face flux carried in LDS
+  K_CELLXS2 = 13,  // the z-shared cell kernel over 2 planes per item, x faces shared through LDS
+  K_CELLXS4 = 14   // the same over 4 planes per item
 };
 struct TilingChoice {
   int irange;  // planes per work item, 0 = whole patch
@@ -282,7 +290,8 @@ static const TilingChoice kTilings[] = {
     {0, 0, K_CELLYZ},  {0, 1, K_CELLX2},     {0, 0, K_CELLX2}, {0, 1, K_CELLZP}, {0, 0, K_CELLZP},
     {0, 2, K_CELLZ},   {0, 1, K_XM},         {32, 1, K_XM},    {16, 1, K_XM},    {8, 1, K_XM},
     {0, 1, K_XM8},     {16, 1, K_XM8},       {0, 1, K_XM7},    {16, 1, K_XM7},   {0, 1, K_XML},
-    {16, 1, K_XML},    {8, 1, K_XML}};
+    {16, 1, K_XML},    {8, 1, K_XML},        {0, 2, K_CELLXS4}, {0, 1, K_CELLXS4}, {0, 2, K_CELLXS2},
+    {0, 1, K_CELLXS2}};
 static const int kNumTilings = sizeof(kTilings) / sizeof(kTilings[0]);
 
 // the column kernel covers the fast MUSCL forms with a symmetric limited slope
@@ -345,7 +354,18 @@ static void launch_range(drn_ctx* ctx, int kind, const void* tiles, int ntiles, 
                                        kind == K_CELL_SLOPE);
   else if (kind == K_WTILE)
     launch_wtile_t<T, RECON, LIM, FAST>(ctx, (const TileWave*)tiles, ntiles, src, base, dst, factor);
-  else if (kind == K_CELLX2) {
+  else if (kind == K_CELLXS2 || kind == K_CELLXS4) {
+    if (ntiles > 0) {
+      if (kind == K_CELLXS2)
+        hipLaunchKernelGGL((k_stage_cell_xs<T, RECON, LIM, FAST, 2>), dim3(ntiles), dim3(256), 0, ctx->stream,
+                           ctx->pd_stage, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor,
+                           (int)ctx->push_now);
+      else
+        hipLaunchKernelGGL((k_stage_cell_xs<T, RECON, LIM, FAST, 4>), dim3(ntiles), dim3(256), 0, ctx->stream,
+                           ctx->pd_stage, (const TileCell*)tiles, src, base, dst, ctx->vstride, factor,
+                           (int)ctx->push_now);
+    }
+  } else if (kind == K_CELLX2) {
     if (ntiles > 0)
       hipLaunchKernelGGL((k_stage_cell_x2<T, RECON, LIM, FAST>), dim3(ntiles), dim3(256), 0, ctx->stream, ctx->pd_stage,
                          (const TileCell*)tiles, src, base, dst, ctx->vstride, factor);
@@ -461,7 +481,11 @@ static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int l
   };
   if (launch == DRN_LAUNCH_BATCHED) {
     const int kind = effective_kind(ctx, ctx->tiling);
-    if (kind == K_CELLX2) {
+    if (kind == K_CELLXS2 || kind == K_CELLXS4) {
+      const auto& L = ctx->xs[kind == K_CELLXS4 ? 1 : 0];
+      const int xo = kTilings[ctx->tiling].xcd;
+      go(kind, xo == 2 && L.d_chain ? L.d_chain : (xo ? L.d_xcd : L.d), (int)L.nat.size());
+    } else if (kind == K_CELLX2) {
       go(K_CELLX2, kTilings[ctx->tiling].xcd ? ctx->d_tx2_xcd : ctx->d_tx2, (int)ctx->tx2.size());
     } else if (kind == K_CELLYZ) {
       go(K_CELLYZ, kTilings[ctx->tiling].xcd ? ctx->d_tyz_xcd : ctx->d_tyz, (int)ctx->tyz.size());
@@ -484,7 +508,11 @@ static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int l
     for (int p = 0; p < np; ++p) {
       const int m0 = ctx->mt_first[p], mn = ctx->mt_first[p + 1] - m0;
       const int w0 = ctx->wt_first[p], wn = ctx->wt_first[p + 1] - w0;
-      if (kind == K_CELLX2) {
+      if (kind == K_CELLXS2 || kind == K_CELLXS4) {
+        const auto& L = ctx->xs[kind == K_CELLXS4 ? 1 : 0];
+        const int z0 = L.first[p];
+        go(kind, (kTilings[ctx->tiling_pp].xcd ? L.d_xcd_pp : L.d) + z0, L.first[p + 1] - z0);
+      } else if (kind == K_CELLX2) {
         const int z0 = ctx->tx2_first[p];
         const TileCell* tl = (kTilings[ctx->tiling_pp].xcd ? ctx->d_tx2_xcd_pp : ctx->d_tx2) + z0;
         go(K_CELLX2, tl, ctx->tx2_first[p + 1] - z0);
@@ -513,7 +541,7 @@ static void stage_t(drn_ctx* ctx, int psrc, int pbase, int pdst, T factor, int l
 // fused z-face receivers themselves; the other kinds leave them to the exchange.
 static bool push_kind(int kind) {
   return kind == K_CELL || kind == K_CELLZ || kind == K_CELLZP || kind == K_XM || kind == K_XM8 || kind == K_XM7 ||
-         kind == K_XML;
+         kind == K_XML || kind == K_CELLXS2 || kind == K_CELLXS4;
 }
 
 // whether a stage of this launch form fills the fused receivers (and the
@@ -890,6 +918,13 @@ void drn_ctx_destroy(drn_ctx* ctx) {
   dfree(ctx->d_tx2);
   dfree(ctx->d_tx2_xcd);
   dfree(ctx->d_tx2_xcd_pp);
+  for (auto& L : ctx->xs) {
+    dfree(L.d);
+    dfree(L.d_xcd);
+    dfree(L.d_xcd_pp);
+    dfree(L.d_chain);
+    L.d = L.d_xcd = L.d_xcd_pp = L.d_chain = nullptr;
+  }
   dfree(ctx->d_mt_pp);
   dfree(ctx->d_mt_launch);
   dfree(ctx->d_thin_tiles);
@@ -1035,6 +1070,20 @@ int drn_layout(drn_ctx* ctx) {
       for (int c2 = 0; c2 < njk; c2 += 504) ctx->tyz.push_back(TileCell{p, i, c2, 0});
   }
   ctx->tyz_first[np] = (int)ctx->tyz.size();
+  for (int q = 0; q < 2; ++q) {
+    const int XP = q ? 4 : 2, step = 252 / XP;
+    auto& L = ctx->xs[q];
+    L.nat.clear();
+    L.first.assign(np + 1, 0);
+    for (int p = 0; p < np; ++p) {
+      const auto& d = ctx->descs[p];
+      L.first[p] = (int)L.nat.size();
+      const int njk = d.nj * d.nk;
+      for (int i = 0; i < d.ni; i += XP)
+        for (int c2 = 0; c2 < njk; c2 += step) L.nat.push_back(TileCell{p, i, c2, 0});
+    }
+    L.first[np] = (int)L.nat.size();
+  }
   ctx->tx2.clear();
   ctx->tx2_first.assign(np + 1, 0);
   for (int p = 0; p < np; ++p) {
@@ -1100,6 +1149,18 @@ int drn_layout(drn_ctx* ctx) {
     HIPCHK(ctx, hipMemcpy(ctx->d_tyz, ctx->tyz.data(), ny * sizeof(TileCell), hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(ctx->d_tyz_xcd, yall.data(), ny * sizeof(TileCell), hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(ctx->d_tyz_xcd_pp, ypp.data(), ny * sizeof(TileCell), hipMemcpyHostToDevice));
+    for (auto& L : ctx->xs) {
+      const int n = (int)L.nat.size();
+      std::vector<TileCell> all(n), pp(n);
+      segment(L.nat.data(), n, all.data());
+      for (int p = 0; p < np; ++p) segment(L.nat.data() + L.first[p], L.first[p + 1] - L.first[p], pp.data() + L.first[p]);
+      HIPCHK(ctx, dalloc(&L.d, n));
+      HIPCHK(ctx, dalloc(&L.d_xcd, n));
+      HIPCHK(ctx, dalloc(&L.d_xcd_pp, n));
+      HIPCHK(ctx, hipMemcpy(L.d, L.nat.data(), n * sizeof(TileCell), hipMemcpyHostToDevice));
+      HIPCHK(ctx, hipMemcpy(L.d_xcd, all.data(), n * sizeof(TileCell), hipMemcpyHostToDevice));
+      HIPCHK(ctx, hipMemcpy(L.d_xcd_pp, pp.data(), n * sizeof(TileCell), hipMemcpyHostToDevice));
+    }
     const int n2 = (int)ctx->tx2.size();
     std::vector<TileCell> xall(n2), xpp(n2);
     segment(ctx->tx2.data(), n2, xall.data());
@@ -1316,6 +1377,10 @@ static int patch_of(const drn_ctx* ctx, uint64_t g) {
 static int build_chain_list(drn_ctx* ctx, const std::vector<std::pair<int, int>>& pairs) {
   dfree(ctx->d_tz_chain);
   ctx->d_tz_chain = nullptr;
+  for (auto& L : ctx->xs) {
+    dfree(L.d_chain);
+    L.d_chain = nullptr;
+  }
   const int np = (int)ctx->descs.size();
   if (pairs.empty() || np == 0) return DRN_OK;
   std::vector<int> up(np);
@@ -1338,29 +1403,37 @@ static int build_chain_list(drn_ctx* ctx, const std::vector<std::pair<int, int>>
     }
     chains[chain_of[r]].push_back(p);
   }
-  std::vector<TileCell> nat;
-  nat.reserve(ctx->tz.size());
-  for (const auto& ch : chains) {
-    int ni_max = 0;
-    for (int p : ch) ni_max = std::max(ni_max, ctx->descs[p].ni);
-    for (int i = 0; i < ni_max; ++i)
-      for (int p : ch) {
-        const auto& d = ctx->descs[p];
-        if (i >= d.ni) continue;
-        const int njk = d.nj * d.nk;
-        for (int c2 = 0; c2 < njk; c2 += 252) nat.push_back(TileCell{p, i, c2, 0});
-      }
-  }
-  if (nat.size() != ctx->tz.size()) return fail(ctx, DRN_E_STATE, "chain list does not cover the z-shared list");
-  const int n = (int)nat.size(), seg = (n + 7) / 8;
-  std::vector<TileCell> out;
-  out.reserve(n);
-  for (int r = 0; r < seg; ++r)
-    for (int x = 0; x < 8; ++x)
-      if (x * seg + r < n) out.push_back(nat[x * seg + r]);
-  HIPCHK(ctx, dalloc(&ctx->d_tz_chain, n));
-  HIPCHK(ctx, hipMemcpy(ctx->d_tz_chain, out.data(), n * sizeof(TileCell), hipMemcpyHostToDevice));
-  return DRN_OK;
+  // items of xp planes x `cells` cells, the planes of a chain's patches side by
+  // side, then in XCD segments
+  auto chain_list = [&](int xp, int cells, size_t n_expect, TileCell** dptr) -> int {
+    std::vector<TileCell> nat;
+    nat.reserve(n_expect);
+    for (const auto& ch : chains) {
+      int ni_max = 0;
+      for (int p : ch) ni_max = std::max(ni_max, ctx->descs[p].ni);
+      for (int i = 0; i < ni_max; i += xp)
+        for (int p : ch) {
+          const auto& d = ctx->descs[p];
+          if (i >= d.ni) continue;
+          const int njk = d.nj * d.nk;
+          for (int c2 = 0; c2 < njk; c2 += cells) nat.push_back(TileCell{p, i, c2, 0});
+        }
+    }
+    if (nat.size() != n_expect) return fail(ctx, DRN_E_STATE, "chain list does not cover the z-shared list");
+    const int n = (int)nat.size(), seg = (n + 7) / 8;
+    std::vector<TileCell> out;
+    out.reserve(n);
+    for (int r = 0; r < seg; ++r)
+      for (int x = 0; x < 8; ++x)
+        if (x * seg + r < n) out.push_back(nat[x * seg + r]);
+    HIPCHK(ctx, dalloc(dptr, n));
+    HIPCHK(ctx, hipMemcpy(*dptr, out.data(), n * sizeof(TileCell), hipMemcpyHostToDevice));
+    return DRN_OK;
+  };
+  int rc;
+  if ((rc = chain_list(1, 252, ctx->tz.size(), &ctx->d_tz_chain))) return rc;
+  if ((rc = chain_list(2, 126, ctx->xs[0].nat.size(), &ctx->xs[0].d_chain))) return rc;
+  return chain_list(4, 63, ctx->xs[1].nat.size(), &ctx->xs[1].d_chain);
 }
 
 int drn_exchange_commit(drn_ctx* ctx) {

@@ -317,6 +317,13 @@ int drn_post_variables(drn_ctx* ctx, int field, int patch, int mask, double ref_
  * 42, 43 as 36 / 38 capped at 7 waves per SIMD (FP32 fast arithmetic, else 36 / 38);
  * 44-46 as 36 / 38 / 39 with the carried x slope and x face flux held in LDS
  *       instead of registers.
+ * 47, 48 the z-shared cell kernel over 4 consecutive planes per work item
+ *       (one wavefront of 63 cells per plane, the same (j,k) cells): a
+ *       wavefront above the first plane takes its x face i-1/2 from the
+ *       wavefront below through LDS (one barrier), 4.27 instead of 5.02 face
+ *       evaluations per cell; on the list of 35 (push chains) / XCD segmented;
+ * 49, 50 the same over 2 planes per work item (two wavefronts per plane), 4.52
+ *       face evaluations per cell.
  * Default for both forms: 11; drn_autotune picks per device. */
 int drn_num_tilings(void);
 int drn_set_tiling(drn_ctx* ctx, int launch, int choice);

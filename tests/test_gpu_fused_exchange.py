@@ -34,8 +34,10 @@ CASES = {
 }
 # (tiling, launch form) of each fusing kernel: cell, z-shared, persistent,
 # z-shared on the push-chain list, per-patch z-shared, the i-marching z-shared
-# kernel (16-plane items; capped at 8 waves per SIMD; whole patches per patch)
-FORMS = [(11, B), (27, B), (33, B), (35, B), (28, PP), (38, B), (41, B), (36, PP)]
+# kernel (16-plane items; capped at 8 waves per SIMD; whole patches per patch),
+# the z-shared kernel over 4 and 2 planes per item sharing x faces (push-chain
+# list; per patch)
+FORMS = [(11, B), (27, B), (33, B), (35, B), (28, PP), (38, B), (41, B), (36, PP), (47, B), (49, PP)]
 
 
 def setup(w, dtype, fuse):
