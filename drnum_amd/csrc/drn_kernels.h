@@ -192,9 +192,11 @@ __device__ __forceinline__ void visc_add(const char* const* sv, const ViscGeom<T
 // its cell i1 are on the same side of the inside boundary.
 __device__ __forceinline__ bool split_bits(uint32_t ins, int a, int b) { return ((ins >> a) ^ (ins >> b)) & 1u; }
 
-// minus = false: the minus face is left out (another wavefront evaluates it as
-// its plus face); its slope is still formed where the plus face needs it
-template <typename T, int RECON, int LIM, bool FAST, int DIR, bool VISC = false, bool FLG = false>
+// MAYSKIP, minus = false: the minus face is left out (another wavefront
+// evaluates it as its plus face), and with it the outer minus neighbour's load
+// and slope
+template <typename T, int RECON, int LIM, bool FAST, int DIR, bool VISC = false, bool FLG = false,
+          bool MAYSKIP = false>
 __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, uint32_t st, int idx, int n,
                                              const T* u0, T A, T* res, T* fp, const ViscGeom<T>* G = nullptr,
                                              uint32_t ins = 0, bool minus = true) {
@@ -204,7 +206,7 @@ __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, 
   for (int v = 0; v < 5; ++v) {
     um2[v] = um1[v] = up1[v] = up2[v] = u0[v];
   }
-  if (have && idx >= 2) {
+  if (have && idx >= 2 && (!MAYSKIP || minus)) {
 #pragma unroll
     for (int v = 0; v < 5; ++v) um2[v] = ldv<T>(sv[v], c - 2 * st);
   }
@@ -240,12 +242,16 @@ __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, 
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      sm[v] = okm ? limited<T, LIM, true>(um1[v] - um2[v], u0[v] - um1[v]) : T(0);
+      if constexpr (!MAYSKIP) sm[v] = okm ? limited<T, LIM, true>(um1[v] - um2[v], u0[v] - um1[v]) : T(0);
       s0[v] = ok0 ? limited<T, LIM, true>(u0[v] - um1[v], up1[v] - u0[v]) : T(0);
       sp[v] = okp ? limited<T, LIM, true>(up1[v] - u0[v], up2[v] - up1[v]) : T(0);
     }
     if (idx > 0 && minus) {
       T vl[5], vr[5], f[5];
+      if constexpr (MAYSKIP) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) sm[v] = okm ? limited<T, LIM, true>(um1[v] - um2[v], u0[v] - um1[v]) : T(0);
+      }
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
         vl[v] = um1[v] + T(0.5) * sm[v];
@@ -378,7 +384,7 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
     }
   }
   if constexpr (XP > 1) {
-    if (own) axis_faces_r<T, RECON, LIM, FAST, 0>(sv, c, sx, i, ni, u0, Ax, res, fxp, &G, 0, pw == 0);
+    if (own) axis_faces_r<T, RECON, LIM, FAST, 0, false, false, true>(sv, c, sx, i, ni, u0, Ax, res, fxp, &G, 0, pw == 0);
     if (own) {
 #pragma unroll
       for (int v = 0; v < 5; ++v) xbuf[(wv * 5 + v) * 64 + lane] = fxp[v];

@@ -468,3 +468,34 @@ def test_z_neighbours_across_wavefront_edges(gpu, dtype, zc):
         err = rel_linf(gg.state(0), og)
         print("z lanes nk", zc + 4, "tiling", tiling, np.dtype(dtype).name, err, flush=True)
         assert err <= TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_x_shared_planes_equal_the_one_plane_kernel(gpu, dtype):
+    """The z-shared cell kernel over 2 and 4 planes per work item (choices
+    47-50) takes the x face i-1/2 of a cell from the wavefront of plane i-1,
+    which evaluated it as its plus face from the same four cells: the states
+    equal those of the one-plane kernel (choice 27) bit for bit, batched and
+    per patch, with the fused z-face exchange."""
+    w = configs.jet(core=16, blocks=2)
+    og = O.OracleGrid(w, dtype, threads=8)
+    og.compute_dependencies()
+    tables = tables_of(og)
+
+    def run(choice):
+        g = solver.GpuGrid(w, dtype)
+        g.upload_initial()
+        g.set_tables(tables)
+        g.ctx.set_tiling(choice, hip.DRN_LAUNCH_BATCHED)
+        g.ctx.set_tiling(choice, hip.DRN_LAUNCH_PER_PATCH)
+        for s in range(4):
+            g.rk_step(hip.DRN_LAUNCH_BATCHED if s % 2 == 0 else hip.DRN_LAUNCH_PER_PATCH)
+        st = g.state(0)
+        g.close()
+        return st
+
+    ref = run(27)
+    for choice in (47, 48, 49, 50):
+        print("x-shared choice", choice, np.dtype(dtype).name, flush=True)
+        for p, (a, b) in enumerate(zip(ref, run(choice))):
+            assert np.array_equal(a, b), (choice, p)
