@@ -301,6 +301,46 @@ __device__ __forceinline__ void axis_faces_r(const char* const* sv, uint32_t c, 
 //             taken by lane shuffles (15 global loads fewer per cell).
 // FLG: per-cell flags (DRN_CELL_INACTIVE = 1, DRN_CELL_INSIDE = 2) at flg[global
 // cell]: split faces reconstruct first order, inactive cells keep their state.
+// the fast symmetric-slope form of axis_faces_r from the four neighbours'
+// values already loaded (um2, um1, up1, up2: the cells at idx-2, idx-1, idx+1,
+// idx+2, or the nearest cell of the patch where one does not exist; the slopes
+// that involve a missing cell are selected to zero as in axis_faces_r)
+template <typename T, int LIM, int DIR>
+__device__ __forceinline__ void axis_faces_v(const T* um2, const T* um1, const T* up1, const T* up2, int idx, int n,
+                                             const T* u0, T A, T* res, T* fp) {
+  zero5(fp);
+  if (n <= 2) return;
+  const bool okm = idx >= 2, ok0 = idx >= 1 && idx + 1 < n, okp = idx + 2 < n;
+  T sm[5], s0[5], sp[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    sm[v] = okm ? limited<T, LIM, true>(um1[v] - um2[v], u0[v] - um1[v]) : T(0);
+    s0[v] = ok0 ? limited<T, LIM, true>(u0[v] - um1[v], up1[v] - u0[v]) : T(0);
+    sp[v] = okp ? limited<T, LIM, true>(up1[v] - u0[v], up2[v] - up1[v]) : T(0);
+  }
+  if (idx > 0) {
+    T vl[5], vr[5], f[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      vl[v] = um1[v] + T(0.5) * sm[v];
+      vr[v] = u0[v] - T(0.5) * s0[v];
+    }
+    zero5(f);
+    ausm_flux_set<T, true, DIR>(vl, vr, A, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) res[v] += f[v];
+  }
+  if (idx < n - 1) {
+    T vl[5], vr[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      vl[v] = u0[v] + T(0.5) * s0[v];
+      vr[v] = up1[v] - T(0.5) * sp[v];
+    }
+    ausm_flux_set<T, true, DIR>(vl, vr, A, fp);  // fp was zeroed above
+  }
+}
+
 // XP (z-shared form only): planes per work item.  The four wavefronts of an
 // item cover 4/XP x 63 cells of each of XP consecutive planes i0 .. i0+XP-1 at
 // the same (j,k) cells; a wavefront above the first plane takes its x face
@@ -385,6 +425,24 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
   }
   if constexpr (XP > 1) {
     if (own) axis_faces_r<T, RECON, LIM, FAST, 0, false, false, true>(sv, c, sx, i, ni, u0, Ax, res, fxp, &G, 0, pw == 0);
+    // fast symmetric form: the y neighbours are loaded before the barrier, so
+    // that their latency passes while the wavefronts wait for each other (a
+    // missing neighbour is replaced by the nearest cell of the patch)
+    constexpr bool YPRE = FAST && RECON == RECON_UPWIND2 && (LIM == LIM_VANALBADA || LIM == LIM_MINMOD);
+    T ym[4][5];
+    if constexpr (YPRE) {
+      if (own) {
+        const uint32_t om1 = j >= 1 ? c - sy : c, op1 = j + 1 < nj ? c + sy : c;
+        const uint32_t om2 = j >= 2 ? c - 2 * sy : om1, op2 = j + 2 < nj ? c + 2 * sy : op1;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          ym[0][v] = ldv<T>(sv[v], om2);
+          ym[1][v] = ldv<T>(sv[v], om1);
+          ym[2][v] = ldv<T>(sv[v], op1);
+          ym[3][v] = ldv<T>(sv[v], op2);
+        }
+      }
+    }
     if (own) {
 #pragma unroll
       for (int v = 0; v < 5; ++v) xbuf[(wv * 5 + v) * 64 + lane] = fxp[v];
@@ -394,7 +452,11 @@ __device__ __forceinline__ void stage_cell_item(const PatchDev* __restrict__ pds
 #pragma unroll
       for (int v = 0; v < 5; ++v) res[v] += xbuf[((wv - WPP) * 5 + v) * 64 + lane];
     }
-    if (own) axis_faces_r<T, RECON, LIM, FAST, 1>(sv, c, sy, j, nj, u0, Ay, res, fyp, &G);
+    if constexpr (YPRE) {
+      if (own) axis_faces_v<T, LIM, 1>(ym[0], ym[1], ym[2], ym[3], j, nj, u0, Ay, res, fyp);
+    } else {
+      if (own) axis_faces_r<T, RECON, LIM, FAST, 1>(sv, c, sy, j, nj, u0, Ay, res, fyp, &G);
+    }
   } else if (own) {
     axis_faces_r<T, RECON, LIM, FAST, 0, VISC, FLG>(sv, c, sx, i, ni, u0, Ax, res, fxp, &G, insx);
     axis_faces_r<T, RECON, LIM, FAST, 1, VISC, FLG>(sv, c, sy, j, nj, u0, Ay, res, fyp, &G, insy);
