@@ -143,8 +143,9 @@ def test_bench_reads_the_committed_traffic_figure():
     spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
     b = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(b)
-    t = b.measured_traffic(35)
-    assert t is not None and 0.9e9 < t < 1.2e9
+    # the summary holds the tiling the autotuner picked when it was taken
+    figs = [t for t in (b.measured_traffic(c) for c in range(64)) if t is not None]
+    assert figs and all(0.9e9 < t < 1.2e9 for t in figs)
     assert b.profile_hash(b.TRAFFIC_TABLE) is not None
     assert b.measured_traffic(999) is None
 
