@@ -9,7 +9,10 @@ same flags.
     state bit for bit;
   * alternating inside cells along z on a state that varies along z: the
     stage equals the Upwind1 stage within the parity tolerance;
-  * clearing the flags returns to the unflagged kernels, bit for bit.
+  * clearing the flags returns to the unflagged kernels, bit for bit;
+  * the same flags on the viscous shear layer (the flag kernel with the
+    viscous flux, whose FP32 form shares the cell gradients between faces and
+    lanes), against the oracle.
 """
 import numpy as np
 import pytest
@@ -111,3 +114,35 @@ def test_cleared_flags_return_to_the_unflagged_result(gpu):
         assert np.array_equal(a, b)
     ref.close()
     gg.close()
+
+
+@pytest.mark.parametrize("dtype,arith", [(np.float64, None), (np.float32, hip.DRN_ARITH_FAST),
+                                         (np.float32, hip.DRN_ARITH_IEEE)])
+def test_flags_with_the_viscous_flux(gpu, dtype, arith):
+    """FP64 runs 3 RK steps (about 4e-14 from the oracle).  The FP32 gap to the
+    FP32 oracle grows by rounding in this flow: 1.4e-5 / 1.9e-5 / 2.1e-5 after
+    1 / 2 / 3 steps in IEEE arithmetic, the same with per-face gradient loads.
+    So FP32 runs one step, within the 2e-5 tolerance."""
+    w = configs.viscous_shear(core=8, blocks=(2, 2, 2))
+    og, gg = make(w, dtype)
+    if arith is not None:
+        gg.ctx.set_arith(arith)
+    # an inside block across the patch boundaries and 5 % scattered inactive cells
+    rng = np.random.default_rng(4)
+    lo = np.min([c.min(axis=1) for c in (np.stack(w.cell_centres(p)) for p in range(len(w.patches)))], axis=0)
+    hi = np.max([c.max(axis=1) for c in (np.stack(w.cell_centres(p)) for p in range(len(w.patches)))], axis=0)
+    mid, half = 0.5 * (lo + hi), 0.2 * (hi - lo)
+    for p, P in enumerate(w.patches):
+        x, y, z = w.cell_centres(p)
+        inside = (np.abs(x - mid[0]) < half[0]) & (np.abs(y - mid[1]) < half[1]) & (np.abs(z - mid[2]) < half[2])
+        f = (np.where(inside, 2, 0) | np.where(rng.random(P.cells) < 0.05, 1, 0)).astype(np.uint8)
+        og.set_cell_flags(p, f)
+        gg.ctx.set_cell_flags(p, f)
+    steps = 3 if dtype == np.float64 else 1
+    for _ in range(steps):
+        og.rk_step()
+        gg.rk_step()
+    err = rel_linf(gg.state(0), og)
+    print("cell flags, viscous, %d steps" % steps, np.dtype(dtype).name, arith, err, flush=True)
+    gg.close()
+    assert err <= TOL[dtype]
