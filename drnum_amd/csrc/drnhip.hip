@@ -1593,11 +1593,12 @@ int drn_exchange_commit(drn_ctx* ctx) {
       x.single.push_back(c);  // freed with the lists, also after a failed upload
       if (rc) return fail(ctx, DRN_E_NOMEM, "exchange list upload failed");
     }
-    // the multi-donor receivers, sorted by their number of entries, so that
-    // the 64 lanes of a wavefront loop over about as many entries each, and
-    // among equal counts by their first donor cell, so that the lanes read
-    // neighbouring donor cells (0.387 instead of 0.564 ms per exchange on a
-    // 200-patch nested batch);
+    // the multi-donor receivers, sorted by their number of entries in bands
+    // of 8 (descending), so that the 64 lanes of a wavefront loop over about
+    // as many entries each, and within a band by their first donor cell, so
+    // that the lanes read neighbouring donor cells: on a 200-patch nested
+    // batch 0.356 ms per exchange, against 0.564 sorted by the count alone,
+    // 0.387 with bands of 1 and 0.385 by the donor cell alone;
     // wavefront b's entries start at tbase[b], entry q of lane L at + 64 q + L
     std::vector<std::pair<size_t, const std::pair<uint32_t, std::vector<Group>>*>> order;  // (entries, receiver)
     for (const auto& e : multi) {
@@ -1606,16 +1607,21 @@ int drn_exchange_commit(drn_ctx* ctx) {
       for (const auto& g : e.second) n += g.idx.size();
       order.push_back({n, &e});
     }
-    std::stable_sort(order.begin(), order.end(), [](const auto& a, const auto& b) {
-      if (a.first != b.first) return a.first > b.first;
-      return a.second->second[0].idx[0] < b.second->second[0].idx[0];
+    auto first_donor = [](const std::pair<uint32_t, std::vector<Group>>* e) -> uint32_t {
+      return e->second.empty() || e->second[0].idx.empty() ? 0u : e->second[0].idx[0];
+    };
+    std::stable_sort(order.begin(), order.end(), [&](const auto& a, const auto& b) {
+      const size_t ka = (a.first + 7) / 8, kb = (b.first + 7) / 8;
+      if (ka != kb) return ka > kb;
+      return first_donor(a.second) < first_donor(b.second);
     });
     std::vector<uint32_t> rec, cnt, tbase, ent, et;
     std::vector<double> w;
     const bool turned = !turns.empty();
     for (size_t b = 0; b < order.size(); b += 64) {
       const size_t nb = std::min<size_t>(64, order.size() - b);
-      const size_t emax = order[b].first;  // the first of a descending run
+      size_t emax = 0;
+      for (size_t L = 0; L < nb; ++L) emax = std::max(emax, order[b + L].first);
       if (ent.size() + 64 * emax > 0xffffffffu) return fail(ctx, DRN_E_RANGE, "exchange lists exceed 2^32 entries");
       tbase.push_back((uint32_t)ent.size());
       ent.resize(ent.size() + 64 * emax, 0u);
