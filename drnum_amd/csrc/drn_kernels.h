@@ -2568,12 +2568,13 @@ __global__ __launch_bounds__(256) void k_exchange_single(uint32_t n, int stride,
   for (int v = 0; v < 5; ++v) U[v * vs + g] = T(0) + acc[v];
 }
 
-// receivers served by several donor patches, or by a turned one: donor sums
-// added in donor order.  The receivers are sorted by their number of (donor
-// cell, weight) entries, and each wavefront's 64 receivers keep their entries
-// interleaved: entry q of lane L at tbase[r / 64] + 64 q + L, so that a lane
-// loops over its own entries only and every entry load is one coalesced
-// access.  An entry is the donor cell index with bit 31 set where it closes
+// receivers served by several donor patches, or by a turned one (and, where
+// the host puts them here, single-donor receivers): donor sums added in donor
+// order.  The receivers are sorted by their number of (donor cell, weight)
+// entries in bands of 8, then by their first donor cell, and each wavefront's
+// 64 receivers keep their entries interleaved: entry q of lane L at
+// tbase[r / 64] + 64 q + L, so that a lane loops over its own entries only and
+// every entry load is one coalesced access.  An entry is the donor cell index with bit 31 set where it closes
 // the sum of one donor patch.  TURN: at such an entry, et = t > 0 turns the
 // momentum of that donor sum into the receiver's frame with the row-major
 // matrix t - 1 of `turn` before it is added (patch.cpp:617-645: inter_vars,

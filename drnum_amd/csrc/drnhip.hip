@@ -1573,8 +1573,20 @@ int drn_exchange_commit(drn_ctx* ctx) {
     if (clash) return fail(ctx, DRN_E_RANGE, "a donor cell is also a receiver cell of the exchange");
   }
   // the full lists, and the lists without the fused receivers
+  // Single-donor receivers of stride > 1 go to the multi-donor list: one
+  // launch fewer, the same sum (0 + the donor's weighted entries in order).
+  // Stride-1 receivers join it too where the multi-donor receivers outnumber
+  // them: there the donor-ordered list reads them better than their own
+  // launch (a 200-patch nested batch: 0.326 instead of 0.347 ms per
+  // exchange), while aligned copies without multi-donor receivers keep their
+  // coalesced launch (a 64-patch jet: 0.0229 instead of 0.0268 ms).
+  const auto s1 = single_by_stride.find(1);
+  const size_t n_s1 = s1 == single_by_stride.end() ? 0 : s1->second.size();
+  const int merge_min = !multi.empty() && multi.size() >= n_s1 ? 1 : 2;
+  auto merged = [&](int stride) { return stride >= merge_min; };
   auto upload_lists = [&](drn_ctx::ExchangeLists& x, bool skip_fused) -> int {
     for (auto& kv : single_by_stride) {
+      if (merged(kv.first)) continue;
       drn_ctx::ExchangeLists::SingleClass c{};
       c.stride = kv.first;
       std::vector<uint32_t> rec, idx;
@@ -1606,6 +1618,13 @@ int drn_exchange_commit(drn_ctx* ctx) {
       size_t n = 0;
       for (const auto& g : e.second) n += g.idx.size();
       order.push_back({n, &e});
+    }
+    for (const auto& kv : single_by_stride) {
+      if (!merged(kv.first)) continue;
+      for (const auto& e : kv.second) {
+        if (skip_fused && fused[e.first]) continue;
+        order.push_back({e.second[0].idx.size(), &e});
+      }
     }
     auto first_donor = [](const std::pair<uint32_t, std::vector<Group>>* e) -> uint32_t {
       return e->second.empty() || e->second[0].idx.empty() ? 0u : e->second[0].idx[0];
